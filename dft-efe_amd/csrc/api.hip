@@ -1,0 +1,702 @@
+// C ABI of libdfx, part 1: context, multivectors, constraints, operators.
+#include <algorithm>
+#include <numeric>
+
+#include "api_util.hpp"
+#include "kernels.hpp"
+
+using namespace dfx;
+
+namespace dfx
+{
+  // ---------------------------------------------------------------- apply
+  template <class T>
+  View<T> view(dfx_mv *m)
+  {
+    return View<T>{static_cast<T *>(m->d), m->N, m->B, m->B};
+  }
+
+  template <class T>
+  View<const T> cview(const dfx_mv *m)
+  {
+    return View<const T>{static_cast<const T *>(m->d), m->N, m->B, m->B};
+  }
+
+  template <class T>
+  const T *fragOf(const dfx_op *op)
+  {
+    if (sizeof(T) == 8)
+      return reinterpret_cast<const T *>(op->frag64.p);
+    if (!op->hasF32)
+      throw Error(DFX_E_STATE, "operator has no FP32 cell matrices (create with DFX_OP_KEEP_FP32)");
+    return reinterpret_cast<const T *>(op->frag32.p);
+  }
+
+  // Y = C^T (sum_e Q_e^T A_e Q_e) C X, KohnShamOperatorContextFE::apply
+  // (src/ksdft/KohnShamOperatorContextFE.t.cpp:1326-1415)
+  template <class T>
+  void applyCellwise(dfx_op *op, View<T> X, View<T> Y)
+  {
+    hipStream_t s = op->ctx->stream;
+    k::p2c<T>(s, op->cons, X);
+    k::fill<T>(s, Y, 0.0);
+    T *yE = nullptr;
+    if (op->nEnrSlots)
+      {
+        op->yE.ensure((size_t)op->nEnrSlots * X.B * sizeof(double));
+        yE = op->yE.as<T>();
+      }
+    const T *frag = fragOf<T>(op);
+    const auto &L = sizeof(T) == 8 ? op->launches64 : op->launches32;
+    View<const T> Xc{X.p, X.N, X.B, X.ld};
+    for (const Launch &l : L)
+      k::cellApply<T>(s, op, l, frag, Xc, Y, yE);
+    if (op->nEnrSlots)
+      k::enrichedReduce<T>(s, op, yE, Y);
+    k::c2p<T>(s, op->cons, Y);
+  }
+
+  // Y = C2P(diag o P2C(X) + [0; E X_E])
+  template <class T>
+  void applyDiagonal(dfx_op *op, View<T> X, View<T> Y)
+  {
+    hipStream_t s = op->ctx->stream;
+    k::p2c<T>(s, op->cons, X);
+    const u32 nC = op->nDofs - op->nEnriched;
+    View<const T> Xc{X.p, X.N, X.B, X.ld};
+    k::diagScale<T>(s, Y, op->diag.as<double>(), Xc, nC);
+    if (op->nEnriched)
+      k::enrichedBlock<T>(s, Y, op->enrBlock.as<double>(), Xc, nC, op->nEnriched);
+    k::c2p<T>(s, op->cons, Y);
+  }
+
+  template <class T>
+  void applyOp(dfx_op *op, View<T> X, View<T> Y)
+  {
+    DFX_REQUIRE(X.N == op->nDofs && Y.N == op->nDofs, "apply: N does not match the operator");
+    DFX_REQUIRE(X.B == Y.B, "apply: X and Y have different widths");
+    if (op->kind == OP_CELLWISE)
+      applyCellwise<T>(op, X, Y);
+    else
+      applyDiagonal<T>(op, X, Y);
+  }
+
+  void ctxRelease(dfx_ctx *ctx)
+  {
+    if (!ctx || ctx->refs.fetch_sub(1) != 1)
+      return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream)
+      (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->blas)
+      rocblas_destroy_handle(ctx->blas);
+    ctx->scratchSmall.release();
+    ctx->reducePartials.release();
+    ctx->work.clear();
+    if (ctx->stream)
+      (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  }
+
+  template void applyOp<double>(dfx_op *, View<double>, View<double>);
+  template void applyOp<float>(dfx_op *, View<float>, View<float>);
+  template View<double> view<double>(dfx_mv *);
+  template View<float> view<float>(dfx_mv *);
+} // namespace dfx
+
+// =================================================================== ABI
+extern "C"
+{
+  // ------------------------------------------------------------- context
+  int dfx_ctx_create(int device, dfx_ctx **out)
+  {
+    if (!out)
+      return DFX_E_ARG;
+    *out = nullptr;
+    auto ctx = new dfx_ctx();
+    int rc = guard(ctx, [&] {
+      int n = 0;
+      DFX_HIP(hipGetDeviceCount(&n));
+      DFX_REQUIRE(device >= 0 && device < n, "no such HIP device");
+      DFX_HIP(hipSetDevice(device));
+      ctx->device = device;
+      DFX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocblas_create_handle failed");
+      rocblas_set_stream(ctx->blas, ctx->stream);
+      ctx->scratchSmall.alloc(1 << 20);
+      return 0;
+    });
+    if (rc != 0)
+      {
+        last_global_error() = ctx->err;
+        delete ctx;
+        return rc;
+      }
+    *out = ctx;
+    return 0;
+  }
+
+  int dfx_ctx_destroy(dfx_ctx *ctx)
+  {
+    if (!ctx)
+      return DFX_E_ARG;
+    ctxRelease(ctx);
+    return 0;
+  }
+
+  const char *dfx_ctx_last_error(const dfx_ctx *ctx)
+  {
+    return ctx ? ctx->err.c_str() : last_global_error().c_str();
+  }
+
+  int dfx_ctx_synchronize(dfx_ctx *ctx)
+  {
+    return guard(ctx, [&] {
+      DFX_HIP(hipStreamSynchronize(ctx->stream));
+      return 0;
+    });
+  }
+
+  int dfx_ctx_stream(dfx_ctx *ctx, void **stream)
+  {
+    if (!ctx || !stream)
+      return DFX_E_ARG;
+    *stream = (void *)ctx->stream;
+    return 0;
+  }
+
+  // -------------------------------------------------------- multivectors
+  int dfx_mv_create(dfx_ctx *ctx, uint32_t N, uint32_t B, int dtype, dfx_mv **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(out, "null output");
+      DFX_REQUIRE(dtype == DFX_F64 || dtype == DFX_F32, "bad dtype");
+      auto m   = new dfx_mv();
+      m->ctx   = ctx;
+      m->N     = N;
+      m->B     = B;
+      m->dtype = dtype;
+      m->owns  = true;
+      if (m->bytes())
+        {
+          hipError_t e = hipMalloc(&m->d, m->bytes());
+          if (e != hipSuccess)
+            {
+              delete m;
+              throw Error(e == hipErrorOutOfMemory ? DFX_E_OOM : DFX_E_HIP,
+                          std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+          DFX_HIP(hipMemsetAsync(m->d, 0, m->bytes(), ctx->stream));
+          DFX_HIP(hipStreamSynchronize(ctx->stream));
+        }
+      ctxRetain(ctx);
+      *out = m;
+      return 0;
+    });
+  }
+
+  int dfx_mv_wrap(dfx_ctx *ctx, void *ptr, uint32_t N, uint32_t B, int dtype, dfx_mv **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(out && (ptr || !N || !B), "null argument");
+      DFX_REQUIRE(dtype == DFX_F64 || dtype == DFX_F32, "bad dtype");
+      auto m   = new dfx_mv();
+      m->ctx   = ctx;
+      m->N     = N;
+      m->B     = B;
+      m->dtype = dtype;
+      m->d     = ptr;
+      m->owns  = false;
+      ctxRetain(ctx);
+      *out     = m;
+      return 0;
+    });
+  }
+
+  int dfx_mv_destroy(dfx_mv *m)
+  {
+    if (!m)
+      return DFX_E_ARG;
+    if (m->owns && m->d)
+      (void)hipFree(m->d);
+    dfx_ctx *ctx = m->ctx;
+    delete m;
+    ctxRelease(ctx);
+    return 0;
+  }
+
+  int dfx_mv_upload(dfx_mv *m, const void *host)
+  {
+    if (!m)
+      return DFX_E_ARG;
+    return guard(m->ctx, [&] {
+      DFX_REQUIRE(host || !m->bytes(), "null host buffer");
+      if (m->bytes())
+        {
+          DFX_HIP(hipMemcpyAsync(m->d, host, m->bytes(), hipMemcpyHostToDevice, m->ctx->stream));
+          DFX_HIP(hipStreamSynchronize(m->ctx->stream));
+        }
+      return 0;
+    });
+  }
+
+  int dfx_mv_download(const dfx_mv *m, void *host)
+  {
+    if (!m)
+      return DFX_E_ARG;
+    return guard(m->ctx, [&] {
+      DFX_REQUIRE(host || !m->bytes(), "null host buffer");
+      if (m->bytes())
+        {
+          DFX_HIP(hipMemcpyAsync(host, m->d, m->bytes(), hipMemcpyDeviceToHost, m->ctx->stream));
+          DFX_HIP(hipStreamSynchronize(m->ctx->stream));
+        }
+      return 0;
+    });
+  }
+
+  int dfx_mv_device_ptr(dfx_mv *m, void **ptr)
+  {
+    if (!m || !ptr)
+      return DFX_E_ARG;
+    *ptr = m->d;
+    return 0;
+  }
+
+  int dfx_mv_set(dfx_mv *m, double v)
+  {
+    if (!m)
+      return DFX_E_ARG;
+    return guard(m->ctx, [&] {
+      if (m->dtype == DFX_F64)
+        k::fill<double>(m->ctx->stream, view<double>(m), v);
+      else
+        k::fill<float>(m->ctx->stream, view<float>(m), v);
+      return 0;
+    });
+  }
+
+  int dfx_mv_copy(dfx_mv *dst, const dfx_mv *src)
+  {
+    if (!dst || !src)
+      return DFX_E_ARG;
+    return guard(dst->ctx, [&] {
+      DFX_REQUIRE(dst->N == src->N && dst->B == src->B, "copy: shape mismatch");
+      hipStream_t s = dst->ctx->stream;
+      if (dst->dtype == DFX_F64 && src->dtype == DFX_F64)
+        k::convert<double, double>(s, view<double>(dst), cview<double>(src));
+      else if (dst->dtype == DFX_F64)
+        k::convert<double, float>(s, view<double>(dst), cview<float>(src));
+      else if (src->dtype == DFX_F64)
+        k::convert<float, double>(s, view<float>(dst), cview<double>(src));
+      else
+        k::convert<float, float>(s, view<float>(dst), cview<float>(src));
+      return 0;
+    });
+  }
+
+  int dfx_mv_axpby(dfx_mv *Z, double a, const dfx_mv *X, double b, const dfx_mv *Y)
+  {
+    if (!Z || !X || !Y)
+      return DFX_E_ARG;
+    return guard(Z->ctx, [&] {
+      DFX_REQUIRE(Z->N == X->N && Z->N == Y->N && Z->B == X->B && Z->B == Y->B,
+                  "axpby: shape mismatch");
+      DFX_REQUIRE(Z->dtype == X->dtype && Z->dtype == Y->dtype, "axpby: dtype mismatch");
+      if (Z->dtype == DFX_F64)
+        k::axpby<double>(Z->ctx->stream, view<double>(Z), a, cview<double>(X), b, cview<double>(Y));
+      else
+        k::axpby<float>(Z->ctx->stream, view<float>(Z), a, cview<float>(X), b, cview<float>(Y));
+      return 0;
+    });
+  }
+
+  int dfx_mv_axpby_blocked(dfx_mv *Z, double a1, const double *a, const dfx_mv *X,
+                           double b1, const double *b, const dfx_mv *Y)
+  {
+    if (!Z || !X || !Y || !a || !b)
+      return DFX_E_ARG;
+    return guard(Z->ctx, [&] {
+      DFX_REQUIRE(Z->N == X->N && Z->N == Y->N && Z->B == X->B && Z->B == Y->B,
+                  "axpbyBlocked: shape mismatch");
+      DFX_REQUIRE(Z->dtype == X->dtype && Z->dtype == Y->dtype, "axpbyBlocked: dtype mismatch");
+      dfx_ctx *ctx = Z->ctx;
+      double  *dv  = ctx->scratchSmall.as<double>();
+      DFX_REQUIRE((size_t)2 * Z->B * sizeof(double) <= ctx->scratchSmall.bytes, "B too large");
+      std::vector<double> h(2 * (size_t)Z->B);
+      std::copy(a, a + Z->B, h.begin());
+      std::copy(b, b + Z->B, h.begin() + Z->B);
+      DFX_HIP(hipMemcpyAsync(dv, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+      if (Z->dtype == DFX_F64)
+        k::axpbyBlocked<double>(ctx->stream, view<double>(Z), a1, dv, cview<double>(X), b1,
+                                dv + Z->B, cview<double>(Y));
+      else
+        k::axpbyBlocked<float>(ctx->stream, view<float>(Z), a1, dv, cview<float>(X), b1,
+                               dv + Z->B, cview<float>(Y));
+      DFX_HIP(hipStreamSynchronize(ctx->stream));
+      return 0;
+    });
+  }
+
+  int dfx_mv_dots(const dfx_mv *X, const dfx_mv *Y, double *out)
+  {
+    if (!X || !Y || !out)
+      return DFX_E_ARG;
+    return guard(X->ctx, [&] {
+      DFX_REQUIRE(X->N == Y->N && X->B == Y->B && X->dtype == Y->dtype, "dots: mismatch");
+      colDotsHost(X->ctx, X, Y, out);
+      return 0;
+    });
+  }
+
+  int dfx_mv_l2norms(const dfx_mv *X, double *out)
+  {
+    if (!X || !out)
+      return DFX_E_ARG;
+    return guard(X->ctx, [&] {
+      colDotsHost(X->ctx, X, X, out);
+      for (u32 j = 0; j < X->B; ++j)
+        out[j] = std::sqrt(out[j]);
+      return 0;
+    });
+  }
+
+  // --------------------------------------------------------- constraints
+  int dfx_constraints_create(dfx_ctx *ctx, uint32_t nDofs, uint32_t nRows,
+                             const uint32_t *rows, const uint32_t *rowSizes,
+                             const uint32_t *rowOffsets, const uint32_t *cols,
+                             const double *weights, const double *inhom,
+                             dfx_constraints **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(out, "null output");
+      DFX_REQUIRE(!nRows || (rows && rowSizes && rowOffsets && inhom), "null constraint arrays");
+      auto c   = std::make_unique<dfx_constraints>();
+      c->ctx   = ctx;
+      c->nDofs = nDofs;
+      c->nRows = nRows;
+      u64 nEnt = 0;
+      for (u32 i = 0; i < nRows; ++i)
+        nEnt += rowSizes[i];
+      DFX_REQUIRE(nEnt < (1ull << 32), "too many constraint entries");
+      DFX_REQUIRE(!nEnt || (cols && weights), "null constraint column arrays");
+      c->nEntries = (u32)nEnt;
+      std::vector<char> isRow(nDofs, 0);
+      c->hRows.assign(rows, rows + nRows);
+      c->hInhom.assign(inhom, inhom + nRows);
+      c->hRowOffsets.resize(nRows + 1);
+      c->hCols.resize(nEnt);
+      c->hWeights.resize(nEnt);
+      // entries are re-laid contiguously in row order (the reference's
+      // rowOffsets may point anywhere into cols / weights)
+      u32 pos = 0;
+      for (u32 i = 0; i < nRows; ++i)
+        {
+          DFX_REQUIRE(rows[i] < nDofs, "constraint row out of range");
+          DFX_REQUIRE(!isRow[rows[i]], "constraint row repeated");
+          isRow[rows[i]]     = 1;
+          c->hRowOffsets[i]  = pos;
+          for (u32 j = 0; j < rowSizes[i]; ++j)
+            {
+              const u32 src = rowOffsets[i] + j;
+              DFX_REQUIRE(cols[src] < nDofs, "constraint column out of range");
+              c->hCols[pos]    = cols[src];
+              c->hWeights[pos] = weights[src];
+              ++pos;
+            }
+        }
+      c->hRowOffsets[nRows] = pos;
+      for (u32 e = 0; e < pos; ++e)
+        DFX_REQUIRE(!isRow[c->hCols[e]],
+                    "constraint chains (a parent that is itself constrained) are not closed");
+      // parent-major transpose, children in row order
+      std::vector<u32> idx(pos);
+      std::iota(idx.begin(), idx.end(), 0u);
+      std::stable_sort(idx.begin(), idx.end(),
+                       [&](u32 a, u32 b) { return c->hCols[a] < c->hCols[b]; });
+      std::vector<u32> entryRow(pos);
+      for (u32 i = 0; i < nRows; ++i)
+        for (u32 e = c->hRowOffsets[i]; e < c->hRowOffsets[i + 1]; ++e)
+          entryRow[e] = i;
+      std::vector<u32>    rowDofs(pos);
+      std::vector<double> pw(pos);
+      c->hParentRows.resize(pos);
+      c->hParentEntries.resize(pos);
+      for (u32 q = 0; q < pos; ++q)
+        {
+          const u32 e = idx[q];
+          if (q == 0 || c->hCols[e] != c->hCols[idx[q - 1]])
+            {
+              c->hParents.push_back(c->hCols[e]);
+              c->hParentOffsets.push_back(q);
+            }
+          c->hParentRows[q]    = entryRow[e];
+          c->hParentEntries[q] = e;
+          rowDofs[q]           = c->hRows[entryRow[e]];
+          pw[q]                = c->hWeights[e];
+        }
+      c->hParentOffsets.push_back(pos);
+      c->nParents = (u32)c->hParents.size();
+      hipStream_t s = ctx->stream;
+      upload(c->rows, c->hRows, s);
+      upload(c->rowOffsets, c->hRowOffsets, s);
+      upload(c->cols, c->hCols, s);
+      upload(c->weights, c->hWeights, s);
+      upload(c->inhom, c->hInhom, s);
+      upload(c->parents, c->hParents, s);
+      upload(c->parentOffsets, c->hParentOffsets, s);
+      upload(c->parentRowDofs, rowDofs, s);
+      upload(c->parentWeights, pw, s);
+      ctxRetain(ctx);
+      *out = c.release();
+      return 0;
+    });
+  }
+
+  int dfx_constraints_destroy(dfx_constraints *c)
+  {
+    if (!c)
+      return DFX_E_ARG;
+    dfx_ctx *ctx = c->ctx;
+    delete c;
+    ctxRelease(ctx);
+    return 0;
+  }
+
+  int dfx_constraints_distribute_parent_to_child(dfx_constraints *c, dfx_mv *X)
+  {
+    if (!c || !X)
+      return DFX_E_ARG;
+    return guard(c->ctx, [&] {
+      DFX_REQUIRE(X->N == c->nDofs, "constraints: N mismatch");
+      if (X->dtype == DFX_F64)
+        k::p2c<double>(c->ctx->stream, c, view<double>(X));
+      else
+        k::p2c<float>(c->ctx->stream, c, view<float>(X));
+      return 0;
+    });
+  }
+
+  int dfx_constraints_distribute_child_to_parent(dfx_constraints *c, dfx_mv *Y)
+  {
+    if (!c || !Y)
+      return DFX_E_ARG;
+    return guard(c->ctx, [&] {
+      DFX_REQUIRE(Y->N == c->nDofs, "constraints: N mismatch");
+      if (Y->dtype == DFX_F64)
+        k::c2p<double>(c->ctx->stream, c, view<double>(Y));
+      else
+        k::c2p<float>(c->ctx->stream, c, view<float>(Y));
+      return 0;
+    });
+  }
+
+  int dfx_constraints_set_constrained_nodes(dfx_constraints *c, dfx_mv *X, double v)
+  {
+    if (!c || !X)
+      return DFX_E_ARG;
+    return guard(c->ctx, [&] {
+      DFX_REQUIRE(X->N == c->nDofs, "constraints: N mismatch");
+      if (X->dtype == DFX_F64)
+        k::setRows<double>(c->ctx->stream, c, view<double>(X), v);
+      else
+        k::setRows<float>(c->ctx->stream, c, view<float>(X), v);
+      return 0;
+    });
+  }
+
+  int dfx_constraints_sizes(const dfx_constraints *c, uint32_t *nRows, uint32_t *nEntries,
+                            uint32_t *nParents)
+  {
+    if (!c)
+      return DFX_E_ARG;
+    if (nRows)
+      *nRows = c->nRows;
+    if (nEntries)
+      *nEntries = c->nEntries;
+    if (nParents)
+      *nParents = c->nParents;
+    return 0;
+  }
+
+  int dfx_constraints_export(const dfx_constraints *c, uint32_t *rows, uint32_t *rowOffsets,
+                             uint32_t *cols, uint32_t *parents, uint32_t *parentOffsets,
+                             uint32_t *parentRows, uint32_t *parentEntries)
+  {
+    if (!c)
+      return DFX_E_ARG;
+    return guard(c->ctx, [&] {
+      // read back what the device holds (not the host copies)
+      hipStream_t s = c->ctx->stream;
+      auto down = [&](uint32_t *dst, const DevBuf &b, size_t n) {
+        if (dst && n)
+          DFX_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      };
+      down(rows, c->rows, c->nRows);
+      down(rowOffsets, c->rowOffsets, c->nRows + 1);
+      down(cols, c->cols, c->nEntries);
+      down(parents, c->parents, c->nParents);
+      down(parentOffsets, c->parentOffsets, c->nParents + 1);
+      DFX_HIP(hipStreamSynchronize(s));
+      // parentRows / parentEntries: the device holds row dofs; export indices
+      if (parentRows)
+        std::copy(c->hParentRows.begin(), c->hParentRows.end(), parentRows);
+      if (parentEntries)
+        std::copy(c->hParentEntries.begin(), c->hParentEntries.end(), parentEntries);
+      return 0;
+    });
+  }
+
+  // ----------------------------------------------------------- operators
+  int dfx_op_create_cellwise(dfx_ctx *ctx, uint32_t nCells, const uint32_t *cellNdofs,
+                             const uint32_t *cellDofMap, const double *cellMatrices,
+                             uint32_t nDofs, uint32_t nClassical, dfx_constraints *cons,
+                             uint32_t flags, dfx_op **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(out && cellNdofs && cellDofMap && cellMatrices, "null argument");
+      std::unique_ptr<dfx_op> op(new dfx_op());
+      buildCellwise(ctx, op.get(), nCells, cellNdofs, cellDofMap, nDofs, nClassical, cons,
+                    flags);
+      uploadCellMatrices(op.get(), cellMatrices, /*device=*/false);
+      ctxRetain(ctx);
+      *out = op.release();
+      return 0;
+    });
+  }
+
+  int dfx_op_update_cells(dfx_op *op, const double *cellMatrices)
+  {
+    if (!op || !cellMatrices)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      DFX_REQUIRE(op->kind == OP_CELLWISE, "not a cell-wise operator");
+      uploadCellMatrices(op, cellMatrices, false);
+      return 0;
+    });
+  }
+
+  int dfx_op_create_diagonal(dfx_ctx *ctx, uint32_t nDofs, const double *diag,
+                             uint32_t nEnriched, const double *enrBlock,
+                             dfx_constraints *cons, dfx_op **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(out && diag, "null argument");
+      DFX_REQUIRE(nEnriched <= nDofs, "nEnriched > nDofs");
+      DFX_REQUIRE(!nEnriched || enrBlock, "enriched block missing");
+      DFX_REQUIRE(!cons || cons->nDofs == nDofs, "constraints built for another N");
+      std::unique_ptr<dfx_op> op(new dfx_op());
+      op->ctx        = ctx;
+      op->kind       = OP_DIAGONAL;
+      op->nDofs      = nDofs;
+      op->nEnriched  = nEnriched;
+      op->nClassical = nDofs - nEnriched;
+      op->cons       = cons;
+      upload(op->diag, diag, nDofs, ctx->stream);
+      if (nEnriched)
+        upload(op->enrBlock, enrBlock, (size_t)nEnriched * nEnriched, ctx->stream);
+      ctxRetain(ctx);
+      *out = op.release();
+      return 0;
+    });
+  }
+
+  int dfx_op_apply(dfx_op *op, dfx_mv *X, dfx_mv *Y)
+  {
+    if (!op || !X || !Y)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      DFX_REQUIRE(X->dtype == Y->dtype, "apply: X and Y dtypes differ");
+      if (X->dtype == DFX_F64)
+        applyOp<double>(op, view<double>(X), view<double>(Y));
+      else
+        applyOp<float>(op, view<float>(X), view<float>(Y));
+      return 0;
+    });
+  }
+
+  int dfx_op_destroy(dfx_op *op)
+  {
+    if (!op)
+      return DFX_E_ARG;
+    dfx_ctx *ctx = op->ctx;
+    if (ctx && ctx->stream)
+      (void)hipStreamSynchronize(ctx->stream);
+    delete op;
+    ctxRelease(ctx);
+    return 0;
+  }
+
+  int dfx_op_get_info(const dfx_op *op, dfx_op_info *info)
+  {
+    if (!op || !info)
+      return DFX_E_ARG;
+    info->nCells      = op->nCells;
+    info->nDofs       = op->nDofs;
+    info->nClassical  = op->nClassical;
+    info->nColors     = op->nColors;
+    info->maxCellDofs = op->maxN;
+    info->sumNe       = op->sumNe;
+    info->sumNe2      = op->sumNe2;
+    info->matBytesF64 = op->fragLen64 * 8;
+    info->matBytesF32 = op->hasF32 ? op->fragLen32 * 4 : 0;
+    return 0;
+  }
+
+  int dfx_op_export_map(const dfx_op *op, uint32_t *cellNdofs, uint32_t *cellDofMap)
+  {
+    if (!op || op->kind != OP_CELLWISE)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      if (cellNdofs)
+        std::copy(op->hCellNdofs.begin(), op->hCellNdofs.end(), cellNdofs);
+      if (cellDofMap)
+        {
+          DFX_HIP(hipMemcpyAsync(cellDofMap, op->map.p, op->sumNe * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, op->ctx->stream));
+          DFX_HIP(hipStreamSynchronize(op->ctx->stream));
+        }
+      return 0;
+    });
+  }
+
+  int dfx_op_gather(dfx_op *op, dfx_mv *X, dfx_mv *Xcell)
+  {
+    if (!op || !X || !Xcell || op->kind != OP_CELLWISE)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      DFX_REQUIRE(X->N == op->nDofs && Xcell->N == op->sumNe && X->B == Xcell->B &&
+                    X->dtype == Xcell->dtype,
+                  "gather: shape mismatch");
+      if (X->dtype == DFX_F64)
+        k::gather<double>(op->ctx->stream, op->map.as<u32>(), op->sumNe, cview<double>(X),
+                          view<double>(Xcell));
+      else
+        k::gather<float>(op->ctx->stream, op->map.as<u32>(), op->sumNe, cview<float>(X),
+                         view<float>(Xcell));
+      return 0;
+    });
+  }
+
+  int dfx_op_scatter_add(dfx_op *op, dfx_mv *Ycell, dfx_mv *Y)
+  {
+    if (!op || !Y || !Ycell || op->kind != OP_CELLWISE)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      DFX_REQUIRE(Y->N == op->nDofs && Ycell->N == op->sumNe && Y->B == Ycell->B &&
+                    Y->dtype == Ycell->dtype,
+                  "scatter: shape mismatch");
+      if (Y->dtype == DFX_F64)
+        k::scatterAdd<double>(op->ctx->stream, op->dofSlotOffsets.as<u32>(),
+                              op->dofSlots.as<u32>(), op->nDofs, cview<double>(Ycell),
+                              view<double>(Y));
+      else
+        k::scatterAdd<float>(op->ctx->stream, op->dofSlotOffsets.as<u32>(),
+                             op->dofSlots.as<u32>(), op->nDofs, cview<float>(Ycell),
+                             view<float>(Y));
+      return 0;
+    });
+  }
+}

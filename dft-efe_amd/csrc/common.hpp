@@ -1,0 +1,228 @@
+// Internal declarations of libdfx (MI355X / gfx950).  Not installed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dfx.h"
+
+namespace dfx
+{
+  using u32 = uint32_t;
+  using u64 = uint64_t;
+
+  // ------------------------------------------------------------ errors
+  struct Error : std::runtime_error
+  {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+  };
+
+#define DFX_HIP(call)                                                        \
+  do                                                                         \
+    {                                                                        \
+      hipError_t e_ = (call);                                                \
+      if (e_ != hipSuccess)                                                  \
+        throw ::dfx::Error(e_ == hipErrorOutOfMemory ? DFX_E_OOM : DFX_E_HIP, \
+                           std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                        \
+  while (0)
+
+#define DFX_REQUIRE(cond, msg)                                               \
+  do                                                                         \
+    {                                                                        \
+      if (!(cond))                                                           \
+        throw ::dfx::Error(DFX_E_ARG, msg);                                  \
+    }                                                                        \
+  while (0)
+
+  // launch check (kernel launches report errors through hipGetLastError)
+  inline void checkLaunch(const char *what)
+  {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      throw Error(DFX_E_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e));
+  }
+
+  // ------------------------------------------------------------ buffers
+  // Owning device allocation (hipMalloc / hipFree).
+  struct DevBuf
+  {
+    void  *p     = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes)
+    {
+      o.p     = nullptr;
+      o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+      if (this != &o)
+        {
+          release();
+          p       = o.p;
+          bytes   = o.bytes;
+          o.p     = nullptr;
+          o.bytes = 0;
+        }
+      return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n)
+    {
+      release();
+      if (n)
+        DFX_HIP(hipMalloc(&p, n));
+      bytes = n;
+    }
+    void ensure(size_t n)
+    {
+      if (n > bytes)
+        alloc(n);
+    }
+    void release()
+    {
+      if (p)
+        (void)hipFree(p);
+      p     = nullptr;
+      bytes = 0;
+    }
+    template <class T>
+    T *as() const
+    {
+      return static_cast<T *>(p);
+    }
+  };
+
+  template <class T>
+  inline void upload(DevBuf &b, const T *h, size_t n, hipStream_t s)
+  {
+    b.alloc(n * sizeof(T));
+    if (n)
+      {
+        DFX_HIP(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+        DFX_HIP(hipStreamSynchronize(s));
+      }
+  }
+  template <class T>
+  inline void upload(DevBuf &b, const std::vector<T> &h, hipStream_t s)
+  {
+    upload(b, h.data(), h.size(), s);
+  }
+
+  // a view of N x B values with leading dimension ld (>= B); element (i, j)
+  // at ptr[i*ld + j]
+  template <class T>
+  struct View
+  {
+    T  *p;
+    u32 N, B, ld;
+  };
+} // namespace dfx
+
+// --------------------------------------------------------------- handles
+// A context lives until its creator has destroyed it and every object made on
+// it (multivector, constraints, operator, solver) is gone: children hold a
+// reference, so bindings may free objects in any order.
+struct dfx_ctx
+{
+  std::atomic<int> refs{1};
+  int            device = 0;
+  hipStream_t    stream = nullptr;
+  rocblas_handle blas   = nullptr;
+  std::string    err;
+  dfx::DevBuf    scratchSmall; // host<->device small transfers (reductions)
+  dfx::DevBuf    reducePartials;
+  std::map<std::string, dfx::DevBuf> work; // named solver scratch
+};
+
+struct dfx_mv
+{
+  dfx_ctx *ctx   = nullptr;
+  dfx::u32 N     = 0, B = 0;
+  int      dtype = DFX_F64;
+  void    *d     = nullptr;
+  bool     owns  = false;
+  size_t   bytes() const { return (size_t)N * B * (dtype == DFX_F64 ? 8 : 4); }
+};
+
+struct dfx_constraints
+{
+  dfx_ctx *ctx = nullptr;
+  dfx::u32 nDofs = 0, nRows = 0, nEntries = 0, nParents = 0;
+  std::vector<dfx::u32> hRows, hRowOffsets, hCols, hParents, hParentOffsets,
+    hParentRows, hParentEntries;
+  std::vector<double> hWeights, hInhom;
+  // device
+  dfx::DevBuf rows, rowOffsets, cols, weights, inhom;           // P2C
+  dfx::DevBuf parents, parentOffsets, parentRowDofs, parentWeights; // C2P
+};
+
+namespace dfx
+{
+  enum OpKind
+  {
+    OP_CELLWISE = 0,
+    OP_DIAGONAL = 1
+  };
+
+  // per-cell launch record (one per cell, in launch order)
+  struct CellMeta
+  {
+    u32 n;       // dofs of the cell
+    u32 nc;      // classical dofs (rows scattered directly)
+    u32 mapOff;  // offset into the cell->dof map
+    u32 enrOff;  // first row of this cell's enriched partials in the yE buffer
+    u64 fragOff64; // offset (elements) of the cell's FP64 fragment-ordered matrix
+    u64 fragOff32; // offset (elements) of the cell's FP32 fragment-ordered matrix
+  };
+
+  // one kernel launch: cells [begin, end) of the launch order, all with the
+  // same colour and the same M-tile class
+  struct Launch
+  {
+    u32 begin, end;
+    u32 mtw;    // M tiles per wave (n <= 64 * mtw)
+    u32 kpadMax;
+  };
+} // namespace dfx
+
+struct dfx_op
+{
+  dfx_ctx     *ctx  = nullptr;
+  int          kind = dfx::OP_CELLWISE;
+  dfx::u32     nDofs = 0, nClassical = 0;
+  dfx_constraints *cons = nullptr;
+  // ----- cell-wise
+  dfx::u32 nCells = 0, nColors = 0, maxN = 0;
+  dfx::u64 sumNe = 0, sumNe2 = 0;
+  std::vector<dfx::u32> hCellNdofs, hMapOffset; // caller order
+  std::vector<dfx::u64> hMatOffset;             // raw (caller) layout offsets
+  std::vector<dfx::u32> hOrder;                 // launch order -> caller cell
+  std::vector<dfx::CellMeta> hMeta;             // launch order
+  std::vector<dfx::Launch> launches64, launches32;
+  dfx::u64 fragLen64 = 0, fragLen32 = 0;
+  dfx::DevBuf map, meta, frag64, frag32;
+  bool hasF32 = false;
+  // enriched partial reduction
+  dfx::u32 nEnrSlots = 0;
+  dfx::DevBuf enrSlotOffsets, enrSlots; // CSR over enriched dofs -> yE rows
+  dfx::DevBuf yE;                       // scratch, grows with B
+  // standalone gather / scatter (dof -> slots CSR, slots in cell order)
+  dfx::DevBuf dofSlotOffsets, dofSlots;
+  // ----- diagonal
+  dfx::DevBuf diag, diag32, enrBlock, enrFrag64, enrFrag32;
+  dfx::u32 nEnriched = 0;
+};

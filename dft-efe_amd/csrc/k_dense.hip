@@ -1,0 +1,249 @@
+// Tall-skinny dense kernels of the orthonormalization and Rayleigh-Ritz
+// steps, FP64 MFMA (v_mfma_f64_16x16x4f64):
+//   gram:   S = X^T Y      (computeXTransOpX, src/linearAlgebra/
+//                           OrthonormalizationFunctions.t.cpp:1178-1380 and
+//                           RayleighRitzEigenSolver.t.cpp:488-844; only the
+//                           lower block triangle is formed, as there)
+//   rotate: Xo = X Q       (the subspace rotations, OrthonormalizationFunctions.t.cpp:
+//                           380-394 and RayleighRitzEigenSolver.t.cpp:272-286)
+// The N-long reduction of gram is split over row chunks whose partial n x n
+// blocks are summed in chunk order by a second kernel: deterministic, no
+// atomics.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace dfx
+{
+  namespace k
+  {
+    namespace
+    {
+      typedef double d4_t __attribute__((ext_vector_type(4)));
+      constexpr int GB  = 64; // S block edge handled by one workgroup
+      constexpr int KS  = 32; // rows per LDS slab
+      constexpr int MAXCHUNKS = 192;
+
+      __device__ inline d4_t mfma64(double a, double b, d4_t c)
+      {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+      }
+
+      // grid.x: lower-triangular block pair index, grid.y: row chunk
+      template <class T>
+      __global__ __launch_bounds__(256) void gramK(const T *__restrict__ X, u32 ldx,
+                                                   const T *__restrict__ Y, u32 ldy,
+                                                   u32 N, u32 n, u32 rowsPerChunk,
+                                                   double *__restrict__ partials)
+      {
+        __shared__ double sX[KS][GB];
+        __shared__ double sY[KS][GB];
+        // decode block pair (ab >= bb)
+        u32 p = blockIdx.x, ab = 0;
+        while (p > ab)
+          {
+            p -= ab + 1;
+            ++ab;
+          }
+        const u32 bb = p;
+        const u32 a0 = ab * GB, b0 = bb * GB;
+        const u32 r0 = blockIdx.y * rowsPerChunk;
+        const u32 r1 = min(N, r0 + rowsPerChunk);
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+
+        d4_t acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[t] = d4_t{0, 0, 0, 0};
+
+        for (u32 i0 = r0; i0 < r1; i0 += KS)
+          {
+            for (int t = threadIdx.x; t < KS * GB; t += 256)
+              {
+                const int kk = t / GB, c = t % GB;
+                const u32 i  = i0 + kk;
+                double    xv = 0.0, yv = 0.0;
+                if (i < r1)
+                  {
+                    if (a0 + c < n)
+                      xv = (double)X[(u64)i * ldx + a0 + c];
+                    if (b0 + c < n)
+                      yv = (double)Y[(u64)i * ldy + b0 + c];
+                  }
+                sX[kk][c] = xv;
+                sY[kk][c] = yv;
+              }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < KS; kk += 4)
+              {
+                const double a = sX[kk + (l >> 4)][w * 16 + (l & 15)];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                  acc[t] = mfma64(a, sY[kk + (l >> 4)][t * 16 + (l & 15)], acc[t]);
+              }
+            __syncthreads();
+          }
+        // partial block -> partials[chunk][a + n b] (column-major n x n)
+        double *P = partials + (u64)blockIdx.y * n * n;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            {
+              const u32 a = a0 + w * 16 + (l >> 4) + 4 * v;
+              const u32 b = b0 + t * 16 + (l & 15);
+              if (a < n && b < n)
+                P[a + (u64)n * b] = acc[t][v];
+            }
+      }
+
+      // S[a + n b] = sum over chunks (in order); lower triangle mirrored up
+      __global__ void gramReduceK(const double *partials, u32 nChunks, u32 n,
+                                  double *S)
+      {
+        const u64 tot = (u64)n * n;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < tot;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            const u32 a = (u32)(t % n), b = (u32)(t / n);
+            const u32 lo = a >= b ? a : b, hi = a >= b ? b : a; // element (lo, hi), lo >= hi
+            // blocks with ab >= bb were computed; (lo, hi) lies in such a block
+            double s = 0.0;
+            for (u32 c = 0; c < nChunks; ++c)
+              s += partials[(u64)c * n * n + lo + (u64)n * hi];
+            S[t] = s;
+          }
+      }
+
+      // Qr[a * nbp + b] = Q[a + n b] (or Q[b + n a] if transposed), zero padded
+      __global__ void packQK(const double *Q, u32 n, u32 nbp, int transpose, double *Qr)
+      {
+        const u64 tot = (u64)n * nbp;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < tot;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            const u32 a = (u32)(t / nbp), b = (u32)(t % nbp);
+            double    v = 0.0;
+            if (b < n)
+              v = transpose ? Q[b + (u64)n * a] : Q[a + (u64)n * b];
+            Qr[t] = v;
+          }
+      }
+
+      // Xo[i, b] = sum_a X[i, a] Qr[a, b]; a workgroup owns 64 rows and
+      // NBT 16-column tiles of the output
+      template <class T, int NBT>
+      __global__ __launch_bounds__(256) void rotateK(const T *__restrict__ X, u32 ldx,
+                                                     u32 N, u32 n,
+                                                     const double *__restrict__ Qr,
+                                                     u32 nbp, u32 bt0,
+                                                     T *__restrict__ Xo, u32 ldo)
+      {
+        __shared__ double sX[64][KS + 1];
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+        const u32 i0 = blockIdx.x * 64;
+        d4_t acc[NBT];
+#pragma unroll
+        for (int t = 0; t < NBT; ++t)
+          acc[t] = d4_t{0, 0, 0, 0};
+        for (u32 k0 = 0; k0 < n; k0 += KS)
+          {
+            for (int t = threadIdx.x; t < 64 * KS; t += 256)
+              {
+                const int r = t / KS, c = t % KS;
+                double    v = 0.0;
+                if (i0 + r < N && k0 + c < n)
+                  v = (double)X[(u64)(i0 + r) * ldx + k0 + c];
+                sX[r][c] = v;
+              }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < KS; kk += 4)
+              {
+                const u32    ka = k0 + kk + (l >> 4);
+                const double a  = sX[w * 16 + (l & 15)][kk + (l >> 4)];
+#pragma unroll
+                for (int t = 0; t < NBT; ++t)
+                  {
+                    const u32 b  = (bt0 + t) * 16 + (l & 15);
+                    double    qv = (ka < n) ? Qr[(u64)ka * nbp + b] : 0.0;
+                    acc[t]       = mfma64(a, qv, acc[t]);
+                  }
+              }
+            __syncthreads();
+          }
+#pragma unroll
+        for (int t = 0; t < NBT; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            {
+              const u32 i = i0 + w * 16 + (l >> 4) + 4 * v;
+              const u32 b = (bt0 + t) * 16 + (l & 15);
+              if (i < N && b < n)
+                Xo[(u64)i * ldo + b] = (T)acc[t][v];
+            }
+      }
+
+      u32 chunksFor(u32 N) { return std::max(1u, std::min<u32>(MAXCHUNKS, (N + 2047) / 2048)); }
+    } // namespace
+
+    size_t gramPartialsNeeded(u32 N, u32 n) { return (size_t)chunksFor(N) * n * n; }
+
+    template <class T>
+    void gram(hipStream_t s, View<const T> X, View<const T> Y, double *partials,
+              size_t partialsCap, double *S)
+    {
+      const u32 n = X.B, N = X.N;
+      DFX_REQUIRE(Y.B == n && Y.N == N, "gram: shape mismatch");
+      const u32 nChunks = chunksFor(N);
+      DFX_REQUIRE(partialsCap >= (size_t)nChunks * n * n, "gram: partials too small");
+      const u32 nb     = (n + GB - 1) / GB;
+      const u32 nPairs = nb * (nb + 1) / 2;
+      const u32 rows   = (N + nChunks - 1) / nChunks;
+      gramK<T><<<dim3(nPairs, nChunks), 256, 0, s>>>(X.p, X.ld, Y.p, Y.ld, N, n, rows,
+                                                     partials);
+      checkLaunch("gram");
+      const u64 tot = (u64)n * n;
+      gramReduceK<<<(unsigned)std::min<u64>(4096, (tot + 255) / 256), 256, 0, s>>>(
+        partials, nChunks, n, S);
+      checkLaunch("gramReduce");
+    }
+
+    template <class T>
+    void rotate(hipStream_t s, View<const T> X, const double *Q, View<T> Xo,
+                double *qrScratch)
+    {
+      // Q: n x n column-major, device
+      const u32 n = X.B, N = X.N;
+      DFX_REQUIRE(Xo.B == n && Xo.N == N, "rotate: shape mismatch");
+      const u32 nbt = (n + 15) / 16, nbp = nbt * 16;
+      packQK<<<(unsigned)std::min<u64>(4096, ((u64)n * nbp + 255) / 256), 256, 0, s>>>(
+        Q, n, nbp, 0, qrScratch);
+      checkLaunch("packQ");
+      const unsigned gx = (N + 63) / 64;
+      for (u32 bt = 0; bt < nbt; bt += 4)
+        {
+          const u32 left = nbt - bt;
+          if (left >= 4)
+            rotateK<T, 4><<<gx, 256, 0, s>>>(X.p, X.ld, N, n, qrScratch, nbp, bt,
+                                            Xo.p, Xo.ld);
+          else if (left == 3)
+            rotateK<T, 3><<<gx, 256, 0, s>>>(X.p, X.ld, N, n, qrScratch, nbp, bt,
+                                            Xo.p, Xo.ld);
+          else if (left == 2)
+            rotateK<T, 2><<<gx, 256, 0, s>>>(X.p, X.ld, N, n, qrScratch, nbp, bt,
+                                            Xo.p, Xo.ld);
+          else
+            rotateK<T, 1><<<gx, 256, 0, s>>>(X.p, X.ld, N, n, qrScratch, nbp, bt,
+                                            Xo.p, Xo.ld);
+          checkLaunch("rotate");
+        }
+    }
+
+    template void gram<double>(hipStream_t, View<const double>, View<const double>,
+                               double *, size_t, double *);
+    template void rotate<double>(hipStream_t, View<const double>, const double *,
+                                 View<double>, double *);
+  } // namespace k
+} // namespace dfx

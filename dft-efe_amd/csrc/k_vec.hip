@@ -1,0 +1,506 @@
+// Streaming kernels of libdfx: vector updates of the Chebyshev recurrences,
+// column reductions, constraint distribution and the stand-alone cell
+// gather / scatter.  All HBM-bound; written as grid-stride loops over
+// N x B views with a leading dimension (column blocks of a wider multivector
+// are addressed in place, without the reference's per-dof copy loops,
+// src/linearAlgebra/ChebyshevFilteredEigenSolver.t.cpp:246-333).
+#include "kernels.hpp"
+
+namespace dfx
+{
+  namespace k
+  {
+    namespace
+    {
+      constexpr int TPB = 256;
+
+      inline unsigned gridFor(u64 n)
+      {
+        u64 g = (n + TPB - 1) / TPB;
+        if (g > 65536ull * 8)
+          g = 65536ull * 8;
+        return (unsigned)(g ? g : 1);
+      }
+
+      // flat index -> (row, col) of an N x B view
+      struct RC
+      {
+        u32 i, j;
+      };
+      __device__ inline RC rc(u64 t, u32 B)
+      {
+        u32 i = (u32)(t / B);
+        return {i, (u32)(t - (u64)i * B)};
+      }
+
+      template <class T>
+      __global__ void fillK(T *x, u32 N, u32 B, u32 ld, T v)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r = rc(t, B);
+            x[(u64)r.i * ld + r.j] = v;
+          }
+      }
+
+      template <class To, class Ti>
+      __global__ void convertK(To *y, u32 ldy, const Ti *x, u32 ldx, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r = rc(t, B);
+            y[(u64)r.i * ldy + r.j] = (To)x[(u64)r.i * ldx + r.j];
+          }
+      }
+
+      template <class T>
+      __global__ void axpbyK(T *z, u32 ldz, T a, const T *x, u32 ldx, T b,
+                             const T *y, u32 ldy, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r  = rc(t, B);
+            T  xv = x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
+            z[(u64)r.i * ldz + r.j] = a * xv + b * yv;
+          }
+      }
+
+      template <class T>
+      __global__ void axpbyBlockedK(T *z, u32 ldz, T a1, const double *a,
+                                    const T *x, u32 ldx, T b1, const double *b,
+                                    const T *y, u32 ldy, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r  = rc(t, B);
+            T  xv = x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
+            z[(u64)r.i * ldz + r.j] = a1 * (T)a[r.j] * xv + b1 * (T)b[r.j] * yv;
+          }
+      }
+
+      template <class T>
+      __global__ void diagScaleK(T *y, u32 ldy, const double *d, const T *x,
+                                 u32 ldx, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r = rc(t, B);
+            y[(u64)r.i * ldy + r.j] = (T)d[r.i] * x[(u64)r.i * ldx + r.j];
+          }
+      }
+
+      // y_E[i, j] = sum_k E[i + nE k] x_E[k, j]   (k ascending)
+      template <class T>
+      __global__ void enrBlockK(T *y, u32 ldy, const double *E, const T *x,
+                                u32 ldx, u32 nC, u32 nE, u32 B)
+      {
+        const u64 n = (u64)nE * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r   = rc(t, B);
+            T  acc = 0;
+            for (u32 kk = 0; kk < nE; ++kk)
+              acc += (T)E[r.i + (u64)nE * kk] * x[(u64)(nC + kk) * ldx + r.j];
+            y[(u64)(nC + r.i) * ldy + r.j] = acc;
+          }
+      }
+
+      // pass 1 of the column dots: block b owns rows [b*rows, (b+1)*rows);
+      // threads are C columns x R row-lanes
+      template <class T, int C>
+      __global__ void colDotsK(const T *x, u32 ldx, const T *y, u32 ldy, u32 N,
+                               u32 B, u32 rowsPerBlock, double *partials)
+      {
+        constexpr int R  = TPB / C;
+        const int     tx = threadIdx.x % C, ty = threadIdx.x / C;
+        __shared__ double red[TPB];
+        const u32 r0 = blockIdx.x * rowsPerBlock;
+        const u32 r1 = min(N, r0 + rowsPerBlock);
+        for (u32 j0 = 0; j0 < B; j0 += C)
+          {
+            const u32 j   = j0 + tx;
+            double    acc = 0.0;
+            if (j < B)
+              for (u32 i = r0 + ty; i < r1; i += R)
+                acc += (double)x[(u64)i * ldx + j] * (double)y[(u64)i * ldy + j];
+            red[threadIdx.x] = acc;
+            __syncthreads();
+            if (ty == 0 && j < B)
+              {
+                double s = 0.0;
+                for (int q = 0; q < R; ++q)
+                  s += red[q * C + tx];
+                partials[(u64)blockIdx.x * B + j] = s;
+              }
+            __syncthreads();
+          }
+      }
+
+      __global__ void colDotsFinalK(const double *partials, u32 nBlocks, u32 B,
+                                    double *out)
+      {
+        const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+        if (j >= B)
+          return;
+        double s = 0.0;
+        for (u32 b = 0; b < nBlocks; ++b)
+          s += partials[(u64)b * B + j];
+        out[j] = s;
+      }
+
+      template <class T>
+      __global__ void chebStepK(T *xp, u32 ldp, const T *hx, u32 ldh,
+                                const double *dinv, const T *tE, u32 ldt, u32 nC,
+                                const T *xk, u32 ldk, T alpha, T beta, T gamma,
+                                u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r = rc(t, B);
+            T  tv;
+            if (r.i < nC)
+              tv = (T)dinv[r.i] * hx[(u64)r.i * ldh + r.j];
+            else
+              tv = tE[(u64)r.i * ldt + r.j];
+            const T s1 = alpha * tv + beta * xk[(u64)r.i * ldk + r.j];
+            T      &o  = xp[(u64)r.i * ldp + r.j];
+            o          = (T)1 * s1 + gamma * o;
+          }
+      }
+
+      template <class T>
+      __global__ void resStepK(T *res, u32 ldr, const T *s2, u32 lds, const T *rn,
+                               u32 ldn, const T *r0, u32 ld0, const double *mu,
+                               T alpha1, T alpha2, T c, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r     = rc(t, B);
+            T  s1    = alpha1 * s2[(u64)r.i * lds + r.j] +
+                      (-c * alpha1) * rn[(u64)r.i * ldn + r.j];
+            T &o     = res[(u64)r.i * ldr + r.j];
+            T  v     = (T)1 * s1 + alpha2 * o;
+            o        = (T)1 * (T)1 * v + alpha1 * (T)mu[r.j] * r0[(u64)r.i * ld0 + r.j];
+          }
+      }
+
+      // ------------------------------------------------ constraints
+      // ConstraintsInternal.cpp:38-107 restated per (row, column): the row's
+      // value is inhom + sum_j w_j x[col_j] accumulated with fma in entry order.
+      template <class T>
+      __global__ void p2cK(T *x, u32 ld, const u32 *rows, const u32 *rowOff,
+                           const u32 *cols, const double *w, const double *inhom,
+                           u32 nRows, u32 B)
+      {
+        const u64 n = (u64)nRows * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC  r  = rc(t, B);
+            T   v  = (T)inhom[r.i];
+            u32 e0 = rowOff[r.i], e1 = rowOff[r.i + 1];
+            for (u32 e = e0; e < e1; ++e)
+              v = fma((T)w[e], x[(u64)cols[e] * ld + r.j], v);
+            x[(u64)rows[r.i] * ld + r.j] = v;
+          }
+      }
+
+      // ConstraintsInternal.cpp:113-170 without atomics: each parent gathers
+      // w * x[row] from its children in the rows' order, which is the order
+      // of the reference's sequential loop, so the result is identical.
+      template <class T>
+      __global__ void c2pParentsK(T *y, u32 ld, const u32 *par, const u32 *parOff,
+                                  const u32 *parRowDof, const double *parW,
+                                  u32 nPar, u32 B)
+      {
+        const u64 n = (u64)nPar * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC  r   = rc(t, B);
+            T  &dst = y[(u64)par[r.i] * ld + r.j];
+            T   acc = dst;
+            u32 e0 = parOff[r.i], e1 = parOff[r.i + 1];
+            for (u32 e = e0; e < e1; ++e)
+              acc = fma((T)parW[e], y[(u64)parRowDof[e] * ld + r.j], acc);
+            dst = acc;
+          }
+      }
+
+      template <class T>
+      __global__ void setRowsK(T *x, u32 ld, const u32 *rows, u32 nRows, u32 B, T v)
+      {
+        const u64 n = (u64)nRows * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r = rc(t, B);
+            x[(u64)rows[r.i] * ld + r.j] = v;
+          }
+      }
+
+      // ------------------------------------------------ gather / scatter
+      template <class T>
+      __global__ void gatherK(const u32 *map, u64 nSlots, const T *x, u32 ldx,
+                              T *xc, u32 ldc, u32 B)
+      {
+        const u64 n = nSlots * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            const u64 s = t / B, j = t - s * B;
+            xc[s * ldc + j] = x[(u64)map[s] * ldx + j];
+          }
+      }
+
+      template <class T>
+      __global__ void scatterAddK(const u32 *off, const u32 *slots, u32 nDofs,
+                                  const T *yc, u32 ldc, T *y, u32 ldy, u32 B)
+      {
+        const u64 n = (u64)nDofs * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC r   = rc(t, B);
+            T &dst = y[(u64)r.i * ldy + r.j];
+            T  acc = dst;
+            for (u32 e = off[r.i]; e < off[r.i + 1]; ++e)
+              acc += yc[(u64)slots[e] * ldc + r.j];
+            dst = acc;
+          }
+      }
+    } // namespace
+
+    // ==================================================== launchers
+    template <class T>
+    void fill(hipStream_t s, View<T> x, double v)
+    {
+      const u64 n = (u64)x.N * x.B;
+      if (!n)
+        return;
+      fillK<T><<<gridFor(n), TPB, 0, s>>>(x.p, x.N, x.B, x.ld, (T)v);
+      checkLaunch("fill");
+    }
+
+    template <class To, class Ti>
+    void convert(hipStream_t s, View<To> d, View<const Ti> x)
+    {
+      const u64 n = (u64)x.N * x.B;
+      if (!n)
+        return;
+      convertK<To, Ti><<<gridFor(n), TPB, 0, s>>>(d.p, d.ld, x.p, x.ld, x.N, x.B);
+      checkLaunch("convert");
+    }
+
+    template <class T>
+    void axpby(hipStream_t s, View<T> z, double a, View<const T> x, double b,
+               View<const T> y)
+    {
+      const u64 n = (u64)z.N * z.B;
+      if (!n)
+        return;
+      axpbyK<T><<<gridFor(n), TPB, 0, s>>>(z.p, z.ld, (T)a, x.p, x.ld, (T)b, y.p,
+                                           y.ld, z.N, z.B);
+      checkLaunch("axpby");
+    }
+
+    template <class T>
+    void axpbyBlocked(hipStream_t s, View<T> z, double a1, const double *a,
+                      View<const T> x, double b1, const double *b, View<const T> y)
+    {
+      const u64 n = (u64)z.N * z.B;
+      if (!n)
+        return;
+      axpbyBlockedK<T><<<gridFor(n), TPB, 0, s>>>(z.p, z.ld, (T)a1, a, x.p, x.ld,
+                                                  (T)b1, b, y.p, y.ld, z.N, z.B);
+      checkLaunch("axpbyBlocked");
+    }
+
+    template <class T>
+    void diagScale(hipStream_t s, View<T> y, const double *d, View<const T> x,
+                   u32 nRowsDiag)
+    {
+      const u64 n = (u64)nRowsDiag * x.B;
+      if (!n)
+        return;
+      diagScaleK<T><<<gridFor(n), TPB, 0, s>>>(y.p, y.ld, d, x.p, x.ld, nRowsDiag, x.B);
+      checkLaunch("diagScale");
+    }
+
+    template <class T>
+    void enrichedBlock(hipStream_t s, View<T> y, const double *E, View<const T> x,
+                       u32 nC, u32 nE)
+    {
+      const u64 n = (u64)nE * x.B;
+      if (!n)
+        return;
+      enrBlockK<T><<<gridFor(n), TPB, 0, s>>>(y.p, y.ld, E, x.p, x.ld, nC, nE, x.B);
+      checkLaunch("enrichedBlock");
+    }
+
+    template <class T>
+    void colDots(hipStream_t s, View<const T> x, View<const T> y, double *partials,
+                 double *out)
+    {
+      const u32 B = x.B;
+      if (!B)
+        return;
+      u32 nBlocks = (x.N + 255) / 256;
+      if (nBlocks > 1024)
+        nBlocks = 1024;
+      if (nBlocks == 0)
+        nBlocks = 1;
+      const u32 rows = (x.N + nBlocks - 1) / nBlocks;
+      if (B <= 16)
+        colDotsK<T, 16><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
+      else if (B <= 32)
+        colDotsK<T, 32><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
+      else
+        colDotsK<T, 64><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
+      checkLaunch("colDots");
+      colDotsFinalK<<<(B + 63) / 64, 64, 0, s>>>(partials, nBlocks, B, out);
+      checkLaunch("colDotsFinal");
+    }
+
+    template <class T>
+    void chebStep(hipStream_t s, View<T> xp, View<const T> hx, const double *dinv,
+                  View<const T> tE, u32 nC, View<const T> xk, double alpha,
+                  double beta, double gamma)
+    {
+      const u64 n = (u64)xp.N * xp.B;
+      if (!n)
+        return;
+      chebStepK<T><<<gridFor(n), TPB, 0, s>>>(xp.p, xp.ld, hx.p, hx.ld, dinv, tE.p,
+                                              tE.ld, nC, xk.p, xk.ld, (T)alpha,
+                                              (T)beta, (T)gamma, xp.N, xp.B);
+      checkLaunch("chebStep");
+    }
+
+    template <class T>
+    void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
+                 View<const T> r0, const double *mu, double alpha1, double alpha2,
+                 double c)
+    {
+      const u64 n = (u64)res.N * res.B;
+      if (!n)
+        return;
+      resStepK<T><<<gridFor(n), TPB, 0, s>>>(res.p, res.ld, s2.p, s2.ld, rn.p, rn.ld,
+                                             r0.p, r0.ld, mu, (T)alpha1, (T)alpha2,
+                                             (T)c, res.N, res.B);
+      checkLaunch("resStep");
+    }
+
+    template <class T>
+    void p2c(hipStream_t s, const dfx_constraints *c, View<T> x)
+    {
+      if (!c || !c->nRows || !x.B)
+        return;
+      const u64 n = (u64)c->nRows * x.B;
+      p2cK<T><<<gridFor(n), TPB, 0, s>>>(x.p, x.ld, c->rows.as<u32>(),
+                                         c->rowOffsets.as<u32>(), c->cols.as<u32>(),
+                                         c->weights.as<double>(), c->inhom.as<double>(),
+                                         c->nRows, x.B);
+      checkLaunch("p2c");
+    }
+
+    template <class T>
+    void c2p(hipStream_t s, const dfx_constraints *c, View<T> y)
+    {
+      if (!c || !c->nRows || !y.B)
+        return;
+      if (c->nParents)
+        {
+          const u64 n = (u64)c->nParents * y.B;
+          c2pParentsK<T><<<gridFor(n), TPB, 0, s>>>(
+            y.p, y.ld, c->parents.as<u32>(), c->parentOffsets.as<u32>(),
+            c->parentRowDofs.as<u32>(), c->parentWeights.as<double>(), c->nParents,
+            y.B);
+          checkLaunch("c2pParents");
+        }
+      const u64 n = (u64)c->nRows * y.B;
+      setRowsK<T><<<gridFor(n), TPB, 0, s>>>(y.p, y.ld, c->rows.as<u32>(), c->nRows,
+                                             y.B, (T)0);
+      checkLaunch("c2pZero");
+    }
+
+    template <class T>
+    void setRows(hipStream_t s, const dfx_constraints *c, View<T> x, double v)
+    {
+      if (!c || !c->nRows || !x.B)
+        return;
+      const u64 n = (u64)c->nRows * x.B;
+      setRowsK<T><<<gridFor(n), TPB, 0, s>>>(x.p, x.ld, c->rows.as<u32>(), c->nRows,
+                                             x.B, (T)v);
+      checkLaunch("setRows");
+    }
+
+    template <class T>
+    void gather(hipStream_t s, const u32 *map, u64 nSlots, View<const T> x, View<T> xc)
+    {
+      const u64 n = nSlots * x.B;
+      if (!n)
+        return;
+      gatherK<T><<<gridFor(n), TPB, 0, s>>>(map, nSlots, x.p, x.ld, xc.p, xc.ld, x.B);
+      checkLaunch("gather");
+    }
+
+    template <class T>
+    void scatterAdd(hipStream_t s, const u32 *off, const u32 *slots, u32 nDofs,
+                    View<const T> yc, View<T> y)
+    {
+      const u64 n = (u64)nDofs * y.B;
+      if (!n)
+        return;
+      scatterAddK<T><<<gridFor(n), TPB, 0, s>>>(off, slots, nDofs, yc.p, yc.ld, y.p,
+                                                y.ld, y.B);
+      checkLaunch("scatterAdd");
+    }
+
+#define DFX_INST_T(T)                                                              \
+  template void fill<T>(hipStream_t, View<T>, double);                             \
+  template void axpby<T>(hipStream_t, View<T>, double, View<const T>, double,      \
+                         View<const T>);                                           \
+  template void axpbyBlocked<T>(hipStream_t, View<T>, double, const double *,      \
+                                View<const T>, double, const double *,             \
+                                View<const T>);                                    \
+  template void diagScale<T>(hipStream_t, View<T>, const double *, View<const T>,  \
+                             u32);                                                 \
+  template void enrichedBlock<T>(hipStream_t, View<T>, const double *,             \
+                                 View<const T>, u32, u32);                         \
+  template void colDots<T>(hipStream_t, View<const T>, View<const T>, double *,    \
+                           double *);                                              \
+  template void chebStep<T>(hipStream_t, View<T>, View<const T>, const double *,   \
+                            View<const T>, u32, View<const T>, double, double,     \
+                            double);                                               \
+  template void resStep<T>(hipStream_t, View<T>, View<const T>, View<const T>,     \
+                           View<const T>, const double *, double, double, double); \
+  template void p2c<T>(hipStream_t, const dfx_constraints *, View<T>);             \
+  template void c2p<T>(hipStream_t, const dfx_constraints *, View<T>);             \
+  template void setRows<T>(hipStream_t, const dfx_constraints *, View<T>, double); \
+  template void gather<T>(hipStream_t, const u32 *, u64, View<const T>, View<T>);  \
+  template void scatterAdd<T>(hipStream_t, const u32 *, const u32 *, u32,          \
+                              View<const T>, View<T>);
+
+    DFX_INST_T(double)
+    DFX_INST_T(float)
+    template void convert<double, float>(hipStream_t, View<double>, View<const float>);
+    template void convert<float, double>(hipStream_t, View<float>, View<const double>);
+    template void convert<double, double>(hipStream_t, View<double>, View<const double>);
+    template void convert<float, float>(hipStream_t, View<float>, View<const float>);
+  } // namespace k
+} // namespace dfx

@@ -1,0 +1,96 @@
+// Host-side launchers of the libdfx kernels (gfx950).  All launches are
+// asynchronous on the given stream.
+#pragma once
+#include "common.hpp"
+
+namespace dfx
+{
+  namespace k
+  {
+    // ---------------------------------------------------------- vectors
+    template <class T>
+    void fill(hipStream_t s, View<T> x, double v);
+    template <class To, class Ti>
+    void convert(hipStream_t s, View<To> dst, View<const Ti> src);
+    // z = a x + b y   (z may alias x or y)
+    template <class T>
+    void axpby(hipStream_t s, View<T> z, double a, View<const T> x, double b,
+               View<const T> y);
+    // z[i,j] = a1 a[j] x[i,j] + b1 b[j] y[i,j]; a, b device arrays of length B
+    template <class T>
+    void axpbyBlocked(hipStream_t s, View<T> z, double a1, const double *a,
+                      View<const T> x, double b1, const double *b, View<const T> y);
+    // y[i,j] = d[i] x[i,j] for rows i < nRowsDiag (rows beyond are left alone)
+    template <class T>
+    void diagScale(hipStream_t s, View<T> y, const double *d, View<const T> x,
+                   u32 nRowsDiag);
+    // y_E = E x_E for the enriched rows [nC, nC + nE): E column-major nE x nE
+    template <class T>
+    void enrichedBlock(hipStream_t s, View<T> y, const double *E, View<const T> x,
+                       u32 nC, u32 nE);
+    // column dots (x_j . y_j) in FP64, deterministic two-pass; result to
+    // `out` (device, length B).  `partials` must hold >= 1024 * B doubles.
+    template <class T>
+    void colDots(hipStream_t s, View<const T> x, View<const T> y, double *partials,
+                 double *out);
+
+    // Plain Chebyshev degree step, fused (ChebyshevFilter.t.cpp:96-129):
+    //   t  = Minv hx  (diag rows < nC: dinv o hx; enriched rows: given in tE)
+    //   s1 = alpha t + beta xk;  xprev = s1 + gamma xprev
+    // xnext aliases xprev.
+    template <class T>
+    void chebStep(hipStream_t s, View<T> xprev, View<const T> hx, const double *dinv,
+                  View<const T> tE, u32 nC, View<const T> xk, double alpha,
+                  double beta, double gamma);
+    // Residual Chebyshev degree step, fused (ChebyshevFilter.t.cpp:360-397):
+    //   s1  = alpha1 s2 + (-c alpha1) rn
+    //   res = s1 + alpha2 res
+    //   res = res + alpha1 mu[j] r0
+    template <class T>
+    void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
+                 View<const T> r0, const double *mu, double alpha1, double alpha2,
+                 double c);
+
+    // -------------------------------------------------------- constraints
+    template <class T>
+    void p2c(hipStream_t s, const dfx_constraints *c, View<T> x);
+    template <class T>
+    void c2p(hipStream_t s, const dfx_constraints *c, View<T> y);
+    template <class T>
+    void setRows(hipStream_t s, const dfx_constraints *c, View<T> x, double v);
+
+    // -------------------------------------------------- gather / scatter
+    // xCell[slot, j] = x[map[slot], j]
+    template <class T>
+    void gather(hipStream_t s, const u32 *map, u64 nSlots, View<const T> x, View<T> xCell);
+    // y[d, j] += sum over the dof's slots (cell order) of yCell[slot, j]
+    template <class T>
+    void scatterAdd(hipStream_t s, const u32 *dofSlotOffsets, const u32 *dofSlots,
+                    u32 nDofs, View<const T> yCell, View<T> y);
+
+    // ------------------------------------------------------- cell apply
+    // one colour / size-class launch of the fused gather -> MFMA -> scatter
+    template <class T>
+    void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
+                   View<const T> x, View<T> y, T *yE);
+    // y[nC + g, j] = sum over the enriched dof's slots of yE[slot, j]
+    template <class T>
+    void enrichedReduce(hipStream_t s, const dfx_op *op, const T *yE, View<T> y);
+    // repack raw column-major cell matrices (device) into the fragment layout
+    template <class T>
+    void repackCells(hipStream_t s, const dfx_op *op, const double *raw,
+                     const u64 *rawOffsets, T *frag);
+
+    // --------------------------------------------------------- dense
+    // S (n x n, column-major, FP64) = X^T Y over all N rows; deterministic
+    template <class T>
+    void gram(hipStream_t s, View<const T> X, View<const T> Y, double *partials,
+              size_t partialsCap, double *S);
+    // Xo = X Q  (Q n x n column-major FP64, device); Xo must not alias X;
+    // qrScratch holds n * 16 * ceil(n / 16) doubles
+    template <class T>
+    void rotate(hipStream_t s, View<const T> X, const double *Q, View<T> Xo,
+                double *qrScratch);
+    size_t gramPartialsNeeded(u32 N, u32 n);
+  } // namespace k
+} // namespace dfx

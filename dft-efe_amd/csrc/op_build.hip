@@ -1,0 +1,256 @@
+// Construction of the device-resident cell-wise operator: validation of the
+// cell -> dof map, greedy colouring of the cell-adjacency graph (cells that
+// share a classical dof get different colours, so one colour's scatter-add is
+// race-free), size classes, launch order, fragment offsets, the enriched-dof
+// reduction lists and the dof -> slot lists of the stand-alone scatter.
+#include <algorithm>
+#include <numeric>
+
+#include "api_util.hpp"
+#include "kernels.hpp"
+
+namespace dfx
+{
+  DevBuf &work(dfx_ctx *ctx, const char *name, size_t bytes)
+  {
+    DevBuf &b = ctx->work[name];
+    b.ensure(bytes);
+    return b;
+  }
+
+  template <class T>
+  void colDotsView(dfx_ctx *ctx, View<const T> X, View<const T> Y, double *hostOut)
+  {
+    const u32 B = X.B;
+    ctx->reducePartials.ensure((size_t)1024 * B * sizeof(double) + B * sizeof(double));
+    double *part = ctx->reducePartials.as<double>();
+    double *out  = part + (size_t)1024 * B;
+    k::colDots<T>(ctx->stream, X, Y, part, out);
+    DFX_HIP(hipMemcpyAsync(hostOut, out, B * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    DFX_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  template void colDotsView<double>(dfx_ctx *, View<const double>, View<const double>,
+                                    double *);
+  template void colDotsView<float>(dfx_ctx *, View<const float>, View<const float>,
+                                   double *);
+
+  void colDotsHost(dfx_ctx *ctx, const dfx_mv *X, const dfx_mv *Y, double *out)
+  {
+    if (X->dtype == DFX_F64)
+      colDotsView<double>(ctx, View<const double>{(const double *)X->d, X->N, X->B, X->B},
+                          View<const double>{(const double *)Y->d, Y->N, Y->B, Y->B}, out);
+    else
+      colDotsView<float>(ctx, View<const float>{(const float *)X->d, X->N, X->B, X->B},
+                         View<const float>{(const float *)Y->d, Y->N, Y->B, Y->B}, out);
+  }
+
+  void buildCellwise(dfx_ctx *ctx, dfx_op *op, u32 nCells, const u32 *cellNdofs,
+                     const u32 *map, u32 nDofs, u32 nClassical, dfx_constraints *cons,
+                     u32 flags)
+  {
+    DFX_REQUIRE(nClassical <= nDofs, "nClassical > nDofs");
+    DFX_REQUIRE(!cons || cons->nDofs == nDofs, "constraints built for another N");
+    op->ctx        = ctx;
+    op->kind       = OP_CELLWISE;
+    op->nDofs      = nDofs;
+    op->nClassical = nClassical;
+    op->cons       = cons;
+    op->nCells     = nCells;
+    op->hasF32     = (flags & DFX_OP_KEEP_FP32) != 0;
+    op->hCellNdofs.assign(cellNdofs, cellNdofs + nCells);
+    op->hMapOffset.resize(nCells + 1);
+    op->hMatOffset.resize(nCells + 1);
+    op->hMapOffset[0] = 0;
+    op->hMatOffset[0] = 0;
+    u64 sumNe = 0, sumNe2 = 0;
+    u32 maxN  = 0;
+    for (u32 e = 0; e < nCells; ++e)
+      {
+        const u32 n = cellNdofs[e];
+        DFX_REQUIRE(n > 0 && n <= 256, "cell dof count must be in 1..256");
+        sumNe += n;
+        sumNe2 += (u64)n * n;
+        maxN = std::max(maxN, n);
+        DFX_REQUIRE(sumNe < (1ull << 32), "cell->dof map longer than 2^32");
+        op->hMapOffset[e + 1] = (u32)sumNe;
+        op->hMatOffset[e + 1] = sumNe2;
+      }
+    op->sumNe  = sumNe;
+    op->sumNe2 = sumNe2;
+    op->maxN   = maxN;
+
+    // classical entries first in every cell; nc = their count
+    std::vector<u32> nc(nCells);
+    for (u32 e = 0; e < nCells; ++e)
+      {
+        u32  cnt = 0;
+        bool enr = false;
+        for (u32 s = op->hMapOffset[e]; s < op->hMapOffset[e + 1]; ++s)
+          {
+            DFX_REQUIRE(map[s] < nDofs, "cell->dof map entry out of range");
+            if (map[s] < nClassical)
+              {
+                DFX_REQUIRE(!enr, "classical dofs must precede enriched dofs in a cell");
+                ++cnt;
+              }
+            else
+              enr = true;
+          }
+        nc[e] = cnt;
+      }
+
+    // ---- greedy colouring over shared classical dofs (cell order)
+    std::vector<u32> dofCellOff(nClassical + 1, 0), dofCells;
+    for (u32 e = 0; e < nCells; ++e)
+      for (u32 s = op->hMapOffset[e]; s < op->hMapOffset[e] + nc[e]; ++s)
+        ++dofCellOff[map[s] + 1];
+    for (u32 d = 0; d < nClassical; ++d)
+      dofCellOff[d + 1] += dofCellOff[d];
+    dofCells.resize(dofCellOff[nClassical]);
+    {
+      std::vector<u32> fillp(dofCellOff.begin(), dofCellOff.end() - 1);
+      for (u32 e = 0; e < nCells; ++e)
+        for (u32 s = op->hMapOffset[e]; s < op->hMapOffset[e] + nc[e]; ++s)
+          dofCells[fillp[map[s]]++] = e;
+    }
+    std::vector<int> color(nCells, -1);
+    std::vector<u32> stamp(64, 0xffffffffu);
+    int nColors = 0;
+    for (u32 e = 0; e < nCells; ++e)
+      {
+        for (u32 s = op->hMapOffset[e]; s < op->hMapOffset[e] + nc[e]; ++s)
+          for (u32 q = dofCellOff[map[s]]; q < dofCellOff[map[s] + 1]; ++q)
+            {
+              const int c = color[dofCells[q]];
+              if (c >= 0)
+                {
+                  if ((size_t)c >= stamp.size())
+                    stamp.resize(c + 1, 0xffffffffu);
+                  stamp[c] = e;
+                }
+            }
+        int c = 0;
+        while ((size_t)c < stamp.size() && stamp[c] == e)
+          ++c;
+        color[e] = c;
+        nColors  = std::max(nColors, c + 1);
+      }
+    op->nColors = (u32)nColors;
+
+    // ---- launch order: (colour, M-tile class, cell)
+    auto mtwOf = [](u32 n) { return (((n + 15) / 16) + 3) / 4; };
+    op->hOrder.resize(nCells);
+    std::iota(op->hOrder.begin(), op->hOrder.end(), 0u);
+    std::stable_sort(op->hOrder.begin(), op->hOrder.end(), [&](u32 a, u32 b) {
+      if (color[a] != color[b])
+        return color[a] < color[b];
+      return mtwOf(cellNdofs[a]) > mtwOf(cellNdofs[b]);
+    });
+
+    // enriched partial rows, in caller cell order
+    std::vector<u32> enrOff(nCells + 1, 0);
+    for (u32 e = 0; e < nCells; ++e)
+      enrOff[e + 1] = enrOff[e] + (cellNdofs[e] - nc[e]);
+    op->nEnrSlots = enrOff[nCells];
+
+    op->hMeta.resize(nCells);
+    u64 f64 = 0, f32 = 0;
+    for (u32 q = 0; q < nCells; ++q)
+      {
+        const u32 e = op->hOrder[q];
+        const u32 n = cellNdofs[e];
+        CellMeta  m;
+        m.n         = n;
+        m.nc        = nc[e];
+        m.mapOff    = op->hMapOffset[e];
+        m.enrOff    = enrOff[e];
+        m.fragOff64 = f64;
+        m.fragOff32 = f32;
+        const u64 mpad = 16 * (u64)((n + 15) / 16);
+        f64 += mpad * (8 * (u64)((n + 7) / 8));
+        f32 += mpad * (16 * (u64)((n + 15) / 16));
+        op->hMeta[q] = m;
+      }
+    op->fragLen64 = f64;
+    op->fragLen32 = f32;
+
+    op->launches64.clear();
+    op->launches32.clear();
+    for (u32 q = 0; q < nCells;)
+      {
+        const u32 e0 = op->hOrder[q];
+        u32       r  = q;
+        u32 kp64 = 0, kp32 = 0;
+        while (r < nCells && color[op->hOrder[r]] == color[e0] &&
+               mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0]))
+          {
+            const u32 n = cellNdofs[op->hOrder[r]];
+            kp64        = std::max(kp64, 8 * ((n + 7) / 8));
+            kp32        = std::max(kp32, 16 * ((n + 15) / 16));
+            ++r;
+          }
+        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64});
+        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32});
+        q = r;
+      }
+
+    // enriched dof -> slots (cell order)
+    const u32 nE = nDofs - nClassical;
+    std::vector<u32> eOff(nE + 1, 0), eSlots(op->nEnrSlots);
+    for (u32 e = 0; e < nCells; ++e)
+      for (u32 i = nc[e]; i < cellNdofs[e]; ++i)
+        ++eOff[map[op->hMapOffset[e] + i] - nClassical + 1];
+    for (u32 g = 0; g < nE; ++g)
+      eOff[g + 1] += eOff[g];
+    {
+      std::vector<u32> fp(eOff.begin(), eOff.end() - 1);
+      for (u32 e = 0; e < nCells; ++e)
+        for (u32 i = nc[e]; i < cellNdofs[e]; ++i)
+          eSlots[fp[map[op->hMapOffset[e] + i] - nClassical]++] = enrOff[e] + (i - nc[e]);
+    }
+
+    // dof -> map slots (slot order = cell order)
+    std::vector<u32> sOff(nDofs + 1, 0), sList(sumNe);
+    for (u64 s = 0; s < sumNe; ++s)
+      ++sOff[map[s] + 1];
+    for (u32 d = 0; d < nDofs; ++d)
+      sOff[d + 1] += sOff[d];
+    {
+      std::vector<u32> fp(sOff.begin(), sOff.end() - 1);
+      for (u64 s = 0; s < sumNe; ++s)
+        sList[fp[map[s]]++] = (u32)s;
+    }
+
+    hipStream_t st = ctx->stream;
+    upload(op->map, map, sumNe, st);
+    upload(op->meta, op->hMeta, st);
+    upload(op->enrSlotOffsets, eOff, st);
+    upload(op->enrSlots, eSlots, st);
+    upload(op->dofSlotOffsets, sOff, st);
+    upload(op->dofSlots, sList, st);
+  }
+
+  void uploadCellMatrices(dfx_op *op, const double *mats, bool device)
+  {
+    hipStream_t st = op->ctx->stream;
+    DevBuf      raw, rawOff;
+    const double *dmats = mats;
+    if (!device)
+      {
+        upload(raw, mats, op->sumNe2, st);
+        dmats = raw.as<double>();
+      }
+    upload(rawOff, op->hMatOffset, st);
+    if (!op->frag64.p)
+      op->frag64.alloc(op->fragLen64 * sizeof(double));
+    k::repackCells<double>(st, op, dmats, rawOff.as<u64>(), op->frag64.as<double>());
+    if (op->hasF32)
+      {
+        if (!op->frag32.p)
+          op->frag32.alloc(op->fragLen32 * sizeof(float));
+        k::repackCells<float>(st, op, dmats, rawOff.as<u64>(), op->frag32.as<float>());
+      }
+    DFX_HIP(hipStreamSynchronize(st));
+  }
+} // namespace dfx

@@ -1,0 +1,753 @@
+// C ABI of libdfx, part 2: the eigensolver and linear-solver drivers on one
+// GPU.  All vector work is on the device; only per-column scalars (dots,
+// eigenvalues, recurrence coefficients) cross to the host, as in the
+// reference's host-side control flow.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+
+#include "api_util.hpp"
+#include "kernels.hpp"
+
+using namespace dfx;
+
+struct dfx_chfsi
+{
+  dfx_ctx         *ctx = nullptr;
+  dfx_chfsi_params p{};
+  dfx_mv          *guess = nullptr;
+  double           ms[3] = {0, 0, 0};
+};
+
+namespace dfx
+{
+  namespace
+  {
+    template <class T>
+    View<T> sub(View<T> v, u32 s, u32 nb)
+    {
+      return View<T>{v.p + s, v.N, nb, v.ld};
+    }
+    template <class T>
+    View<const T> cv(View<T> v)
+    {
+      return View<const T>{v.p, v.N, v.B, v.ld};
+    }
+
+    template <class T>
+    View<T> workView(dfx_ctx *ctx, const char *name, u32 N, u32 B)
+    {
+      DevBuf &b = work(ctx, name, (size_t)N * B * sizeof(T) + 256);
+      return View<T>{b.as<T>(), N, B, B};
+    }
+
+    double *workDoubles(dfx_ctx *ctx, const char *name, size_t n)
+    {
+      return work(ctx, name, n * sizeof(double) + 64).as<double>();
+    }
+
+    // small host -> device upload; completes before returning, so the host
+    // buffer (often a temporary) may be released right after
+    void toDevice(dfx_ctx *ctx, double *d, const double *h, size_t n)
+    {
+      if (!n)
+        return;
+      DFX_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      DFX_HIP(hipStreamSynchronize(ctx->stream));
+    }
+
+    struct Timer
+    {
+      hipEvent_t a, b;
+      hipStream_t s;
+      explicit Timer(hipStream_t st) : s(st)
+      {
+        DFX_HIP(hipEventCreate(&a));
+        DFX_HIP(hipEventCreate(&b));
+        DFX_HIP(hipEventRecord(a, s));
+      }
+      double stop()
+      {
+        DFX_HIP(hipEventRecord(b, s));
+        DFX_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        DFX_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+      }
+      ~Timer()
+      {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    };
+
+    // M^-1 of a diagonal (+ enriched block) operator applied in the fused
+    // plain-filter step: only the enriched rows need the block GEMM; the
+    // diagonal rows are scaled inside chebStep.
+    template <class T>
+    void enrichedPart(dfx_op *BInv, View<const T> x, View<T> tE)
+    {
+      if (BInv->nEnriched)
+        k::enrichedBlock<T>(BInv->ctx->stream, tE, BInv->enrBlock.as<double>(), x,
+                            BInv->nDofs - BInv->nEnriched, BInv->nEnriched);
+    }
+
+    // ------------------------------------------------------------ filters
+    // ChebyshevFilter (src/linearAlgebra/ChebyshevFilter.t.cpp:39-135) on one
+    // block, bulk in T.  Xin (FP64) receives the filtered block as well (the
+    // reference assigns eigenSubspaceGuess = filteredSubspace at the end).
+    template <class T>
+    void plainFilter(dfx_ctx *ctx, dfx_op *A, dfx_op *BInv, View<double> Xin, u32 m,
+                     double a, double b, double c, View<double> Xout)
+    {
+      hipStream_t s  = ctx->stream;
+      const u32   N  = Xin.N, nb = Xin.B;
+      DFX_REQUIRE(BInv->kind == OP_DIAGONAL,
+                  "the fused filter needs a diagonal (+ enriched block) BInv");
+      const double e  = 0.5 * (c - b);
+      const double c0 = 0.5 * (c + b);
+      double       sigma  = e / (a - c0);
+      const double sigma1 = sigma;
+      const double gamma  = 2.0 / sigma1;
+      View<T> xk = workView<T>(ctx, "pf.xk", N, nb);
+      View<T> yk = workView<T>(ctx, "pf.yk", N, nb);
+      View<T> s1 = workView<T>(ctx, "pf.s1", N, nb);
+      View<T> tE = workView<T>(ctx, "pf.tE", N, nb);
+      k::convert<T, double>(s, xk, cv(Xin));
+      const double *dinv = BInv->diag.as<double>();
+      const u32     nC   = BInv->nDofs - BInv->nEnriched;
+      // Y1 = (sigma1/e)(BInv A X - c0 X)
+      applyOp<T>(A, xk, s1);
+      enrichedPart<T>(BInv, cv(s1), tE);
+      // yk = alpha * Minv s1 + beta * xk + 0 * yk  (yk zeroed first)
+      k::fill<T>(s, yk, 0.0);
+      k::chebStep<T>(s, yk, cv(s1), dinv, cv(tE), nC, cv(xk), sigma1 / e,
+                     -sigma1 / e * c0, 0.0);
+      for (u32 deg = 2; deg <= m; ++deg)
+        {
+          const double sigma2 = 1.0 / (gamma - sigma);
+          applyOp<T>(A, yk, s1);
+          enrichedPart<T>(BInv, cv(s1), tE);
+          // xk <- (2 sigma2/e)(Minv s1 - c0 yk) - sigma sigma2 xk
+          k::chebStep<T>(s, xk, cv(s1), dinv, cv(tE), nC, cv(yk), 2.0 * sigma2 / e,
+                         -2.0 * sigma2 / e * c0, -sigma * sigma2);
+          std::swap(xk, yk);
+          sigma = sigma2;
+        }
+      k::convert<double, T>(s, Xout, cv(yk));
+      k::convert<double, T>(s, Xin, cv(yk));
+    }
+
+    // ResidualChebyshevFilterGEP (src/linearAlgebra/ChebyshevFilter.t.cpp:
+    // 242-444) on one block.  The residual seed R0 = A X - lambda B X and the
+    // final X_m = BInv R_m + mu_m X are FP64; the degree loop runs in T.
+    template <class T>
+    void residualFilter(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv,
+                        const double *lam, View<double> Xin, u32 m, double a,
+                        double b, double c, View<double> Y)
+    {
+      hipStream_t s  = ctx->stream;
+      const u32   N  = Xin.N, nb = Xin.B;
+      const double e  = 0.5 * (c - b);
+      const double c0 = 0.5 * (c + b);
+      double       sigma  = e / (a - c0);
+      const double sigma1 = sigma;
+      const double gamma  = 2.0 / sigma1;
+
+      // ---- scalar recurrence (eigenValuesFiltered1/2), host, reference order
+      std::vector<double> lamF(lam, lam + nb), l1(nb, 1.0), l2(nb), ones(nb, 1.0);
+      double alpha1 = sigma1 / e, alpha2 = -c0;
+      for (u32 j = 0; j < nb; ++j)
+        l2[j] = alpha1 * alpha2;
+      for (u32 j = 0; j < nb; ++j)
+        l2[j] = 1.0 * ones[j] * l2[j] + alpha1 * lamF[j] * l1[j];
+      const double rnScale = alpha1;
+      std::vector<double> table((size_t)(m + 2) * nb + 2 * nb, 0.0);
+      std::vector<double> a1(m + 1, 0.0), a2(m + 1, 0.0);
+      {
+        double sg = sigma;
+        for (u32 deg = 2; deg <= m; ++deg)
+          {
+            const double sigma2 = 1.0 / (gamma - sg);
+            const double al1    = 2.0 * sigma2 / e, al2 = -(sg * sigma2);
+            a1[deg]             = al1;
+            a2[deg]             = al2;
+            std::copy(l2.begin(), l2.end(), table.begin() + (size_t)deg * nb);
+            for (u32 j = 0; j < nb; ++j)
+              l1[j] = (-c0 * al1) * l2[j] + al2 * l1[j];
+            for (u32 j = 0; j < nb; ++j)
+              l1[j] = 1.0 * ones[j] * l1[j] + al1 * lamF[j] * l2[j];
+            std::swap(l1, l2);
+            sg = sigma2;
+          }
+      }
+      // final mu (l2) in row m+1; ones and lambda after it
+      std::copy(l2.begin(), l2.end(), table.begin() + (size_t)(m + 1) * nb);
+      std::copy(ones.begin(), ones.end(), table.begin() + (size_t)(m + 2) * nb);
+      std::copy(lamF.begin(), lamF.end(), table.begin() + (size_t)(m + 2) * nb + nb);
+      double *dtab = workDoubles(ctx, "rf.table", table.size());
+      toDevice(ctx, dtab, table.data(), table.size());
+      const double *dOnes = dtab + (size_t)(m + 2) * nb;
+      const double *dLam  = dOnes + nb;
+
+      // ---- FP64 seed: Y = B X, S3 = A X, Y = S3 - lambda o Y
+      View<double> s3 = workView<double>(ctx, "rf.s3", N, nb);
+      applyOp<double>(Bop, Xin, Y);
+      applyOp<double>(A, Xin, s3);
+      k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(s3), -1.0, dLam, cv(Y));
+
+      View<T> r0 = workView<T>(ctx, "rf.r0", N, nb);
+      View<T> rn = workView<T>(ctx, "rf.rn", N, nb);
+      View<T> rs = workView<T>(ctx, "rf.res", N, nb);
+      View<T> s1 = workView<T>(ctx, "rf.s1", N, nb);
+      View<T> s2 = workView<T>(ctx, "rf.s2", N, nb);
+      // ResidualNew = alpha1 * Y ; Residual = 0
+      k::convert<T, double>(s, r0, cv(Y));
+      View<double> rn64 = workView<double>(ctx, "rf.rn64", N, nb);
+      k::axpby<double>(s, rn64, rnScale, cv(Y), 0.0, cv(Y));
+      k::convert<T, double>(s, rn, cv(rn64));
+      k::fill<T>(s, rs, 0.0);
+      for (u32 deg = 2; deg <= m; ++deg)
+        {
+          applyOp<T>(BInv, rn, s1);
+          applyOp<T>(A, s1, s2);
+          k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb, a1[deg],
+                        a2[deg], c0);
+          std::swap(rn, rs);
+        }
+      applyOp<T>(BInv, rn, rs);
+      View<double> rs64 = workView<double>(ctx, "rf.res64", N, nb);
+      k::convert<double, T>(s, rs64, cv(rs));
+      k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(rs64), 1.0,
+                              dtab + (size_t)(m + 1) * nb, cv(Xin));
+    }
+
+    // --------------------------------------------------- CholGS and RR
+    int cholGS(dfx_ctx *ctx, View<double> X, View<double> Xo, dfx_op *Bop)
+    {
+      hipStream_t s = ctx->stream;
+      const u32   N = X.N, n = X.B;
+      if (N < n)
+        return DFX_ORTHO_NON_ORTHONORMALIZABLE_MULTIVECTOR;
+      View<double> MX = workView<double>(ctx, "cg.mx", N, n);
+      applyOp<double>(Bop, X, MX);
+      double *S    = workDoubles(ctx, "cg.S", (size_t)n * n);
+      double *part = workDoubles(ctx, "gram.part", k::gramPartialsNeeded(N, n));
+      k::gram<double>(s, cv(X), cv(MX), part, k::gramPartialsNeeded(N, n), S);
+      int *info = work(ctx, "info", 64).as<int>();
+      if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, (rocblas_int)n, S, (rocblas_int)n,
+                           info) != rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocsolver_dpotrf failed");
+      int hinfo = 0;
+      DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+      std::vector<double> hS((size_t)n * n);
+      DFX_HIP(hipMemcpyAsync(hS.data(), S, hS.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      DFX_HIP(hipStreamSynchronize(s));
+      if (hinfo != 0)
+        return DFX_ORTHO_LAPACK_ERROR;
+      // |L_ii| < 1e-14 is reported as failure (OrthonormalizationFunctions.t.cpp:274-306)
+      for (u32 i = 0; i < n; ++i)
+        if (!(std::fabs(hS[i + (size_t)n * i]) >= 1e-14))
+          return DFX_ORTHO_LAPACK_ERROR;
+      for (u32 j = 0; j < n; ++j)
+        for (u32 i = 0; i < j; ++i)
+          hS[i + (size_t)n * j] = 0.0;
+      toDevice(ctx, S, hS.data(), hS.size());
+      if (rocsolver_dtrtri(ctx->blas, rocblas_fill_lower, rocblas_diagonal_non_unit,
+                           (rocblas_int)n, S, (rocblas_int)n, info) != rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocsolver_dtrtri failed");
+      DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+      // Xo = X L^-T: Q = (L^-1)^T
+      double *Q = workDoubles(ctx, "cg.Q", (size_t)n * n);
+      DFX_HIP(hipMemcpyAsync(hS.data(), S, hS.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      DFX_HIP(hipStreamSynchronize(s));
+      if (hinfo != 0)
+        return DFX_ORTHO_LAPACK_ERROR;
+      std::vector<double> hQ((size_t)n * n, 0.0);
+      for (u32 j = 0; j < n; ++j)
+        for (u32 i = j; i < n; ++i)
+          hQ[j + (size_t)n * i] = hS[i + (size_t)n * j];
+      toDevice(ctx, Q, hQ.data(), hQ.size());
+      double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
+      k::rotate<double>(s, cv(X), Q, Xo, qr);
+      return 0;
+    }
+
+    int rayleighRitz(dfx_ctx *ctx, dfx_op *A, View<double> X, double *eig,
+                     View<double> Xo)
+    {
+      hipStream_t s = ctx->stream;
+      const u32   N = X.N, n = X.B;
+      View<double> HX = workView<double>(ctx, "cg.mx", N, n);
+      applyOp<double>(A, X, HX);
+      double *S    = workDoubles(ctx, "rr.S", (size_t)n * n);
+      double *part = workDoubles(ctx, "gram.part", k::gramPartialsNeeded(N, n));
+      k::gram<double>(s, cv(X), cv(HX), part, k::gramPartialsNeeded(N, n), S);
+      double *D    = workDoubles(ctx, "rr.D", 2 * (size_t)n);
+      int    *info = work(ctx, "info", 64).as<int>();
+      if (rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower,
+                           (rocblas_int)n, S, (rocblas_int)n, D, D + n, info) !=
+          rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocsolver_dsyevd failed");
+      int hinfo = 0;
+      DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+      DFX_HIP(hipMemcpyAsync(eig, D, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      DFX_HIP(hipStreamSynchronize(s));
+      if (hinfo != 0)
+        return DFX_EIG_LAPACK_ERROR;
+      double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
+      k::rotate<double>(s, cv(X), S, Xo, qr);
+      return 0;
+    }
+
+    View<double> mvView(dfx_mv *m)
+    {
+      DFX_REQUIRE(m->dtype == DFX_F64, "FP64 multivector expected");
+      return View<double>{static_cast<double *>(m->d), m->N, m->B, m->B};
+    }
+
+    // ChebyshevFilteredEigenSolver::solve (ChebyshevFilteredEigenSolver.t.cpp:194-437)
+    int chfsiSolve(dfx_chfsi *cs, dfx_op *A, double *eig, dfx_mv *X, dfx_op *Bop,
+                   dfx_op *BInv)
+    {
+      dfx_ctx *ctx = cs->ctx;
+      const auto &p = cs->p;
+      View<double> G  = mvView(cs->guess);
+      View<double> XV = mvView(X);
+      DFX_REQUIRE(G.N == XV.N && G.B == XV.B, "guess and eigenvectors differ in shape");
+      DFX_REQUIRE(A->nDofs == XV.N, "operator size does not match the eigenvectors");
+      const u32 n  = XV.B;
+      const u32 bs = (p.eigenVectorBatchSize == 0 || p.eigenVectorBatchSize > n)
+                       ? n
+                       : p.eigenVectorBatchSize;
+      const bool mixed = p.precision == DFX_PREC_MIXED;
+      Timer tf(ctx->stream);
+      for (u32 s0 = 0; s0 < n; s0 += bs)
+        {
+          const u32    nb  = std::min(bs, n - s0);
+          View<double> Gin = sub(G, s0, nb), Xo = sub(XV, s0, nb);
+          if (p.isResidualChebyshevFilter)
+            {
+              if (mixed)
+                residualFilter<float>(ctx, A, Bop, BInv, eig + s0, Gin, p.polynomialDegree,
+                                      p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
+                                      p.unWantedSpectrumUpperBound, Xo);
+              else
+                residualFilter<double>(ctx, A, Bop, BInv, eig + s0, Gin, p.polynomialDegree,
+                                       p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
+                                       p.unWantedSpectrumUpperBound, Xo);
+            }
+          else
+            {
+              if (mixed)
+                plainFilter<float>(ctx, A, BInv, Gin, p.polynomialDegree,
+                                   p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
+                                   p.unWantedSpectrumUpperBound, Xo);
+              else
+                plainFilter<double>(ctx, A, BInv, Gin, p.polynomialDegree,
+                                    p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
+                                    p.unWantedSpectrumUpperBound, Xo);
+            }
+        }
+      cs->ms[0] = tf.stop();
+      Timer to(ctx->stream);
+      int oc = cholGS(ctx, XV, G, Bop);
+      cs->ms[1] = to.stop();
+      if (oc != 0)
+        return DFX_EIG_CHFSI_ORTHONORMALIZATION_ERROR;
+      Timer tr(ctx->stream);
+      int rc = rayleighRitz(ctx, A, G, eig, XV);
+      cs->ms[2] = tr.stop();
+      if (rc != 0)
+        return DFX_EIG_CHFSI_RAYLEIGH_RITZ_ERROR;
+      return 0;
+    }
+
+    // LanczosExtremeEigenSolver::solve, non-adaptive (LanczosExtremeEigenSolver.t.cpp:224-540)
+    void lanczos(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv, View<double> guess,
+                 u32 maxK, double *lo, double *hi, double *betaLast)
+    {
+      hipStream_t s = ctx->stream;
+      const u32   N = guess.N;
+      DFX_REQUIRE(guess.B == 1, "Lanczos guess must be a single vector");
+      View<double> g     = workView<double>(ctx, "lz.g", N, 1);
+      View<double> temp  = workView<double>(ctx, "lz.t", N, 1);
+      View<double> v     = workView<double>(ctx, "lz.v", N, 1);
+      View<double> q     = workView<double>(ctx, "lz.q", N, 1);
+      View<double> qPrev = workView<double>(ctx, "lz.qp", N, 1);
+      k::convert<double, double>(s, g, cv(guess));
+      k::fill<double>(s, qPrev, 0.0);
+      double dot;
+      applyOp<double>(Bop, g, temp);
+      colDotsView<double>(ctx, cv(g), cv(temp), &dot);
+      double alpha = std::sqrt(dot);
+      k::axpby<double>(s, q, 1.0 / alpha, cv(g), 0.0, cv(g));
+      double              beta = 0.0;
+      std::vector<double> av, bv;
+      for (u32 it = 1; it <= maxK; ++it)
+        {
+          applyOp<double>(A, q, temp);
+          applyOp<double>(BInv, temp, v);
+          colDotsView<double>(ctx, cv(q), cv(temp), &dot);
+          alpha = dot;
+          av.push_back(alpha);
+          k::axpby<double>(s, v, -alpha, cv(q), 1.0, cv(v));
+          k::axpby<double>(s, v, -beta, cv(qPrev), 1.0, cv(v));
+          applyOp<double>(Bop, v, temp);
+          colDotsView<double>(ctx, cv(v), cv(temp), &dot);
+          beta = std::sqrt(dot);
+          bv.push_back(beta);
+          k::convert<double, double>(s, qPrev, cv(q));
+          k::axpby<double>(s, q, 1.0 / beta, cv(v), 0.0, cv(v));
+        }
+      // eigenvalues of the tridiagonal (alpha, beta[0..k-2]) on the device
+      const u32 kk = (u32)av.size();
+      double   *d  = workDoubles(ctx, "lz.d", 2 * (size_t)kk + 2);
+      toDevice(ctx, d, av.data(), kk);
+      if (kk > 1)
+        toDevice(ctx, d + kk, bv.data(), kk - 1);
+      int *info = work(ctx, "info", 64).as<int>();
+      if (rocsolver_dsterf(ctx->blas, (rocblas_int)kk, d, d + kk, info) !=
+          rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocsolver_dsterf failed");
+      std::vector<double> w(kk);
+      DFX_HIP(hipMemcpyAsync(w.data(), d, kk * sizeof(double), hipMemcpyDeviceToHost, s));
+      DFX_HIP(hipStreamSynchronize(s));
+      std::sort(w.begin(), w.end());
+      *lo       = w.front();
+      *hi       = w.back();
+      *betaLast = bv.back();
+    }
+
+    // CHEBY_ORDER_LOOKUP (src/ksdft/Defaults.cpp:51-58), lower_bound semantics
+    // of KohnShamEigenSolver.t.cpp:36-46
+    u32 chebyDegree(double upper)
+    {
+      static const std::map<double, u32> lookup = {
+        {10, 6},      {50, 9},      {100, 12},    {200, 16},    {300, 19},
+        {500, 24},    {750, 30},    {1000, 39},   {1500, 50},   {2000, 53},
+        {3000, 57},   {4000, 62},   {5000, 69},   {9000, 77},   {14000, 104},
+        {20000, 119}, {30000, 162}, {50000, 300}, {80000, 450}, {100000, 550},
+        {200000, 700}, {500000, 1000}};
+      // the reference looks the bound up as a size_type key
+      auto it = lookup.lower_bound((double)(u32)upper);
+      return it != lookup.end() ? it->second : 1250;
+    }
+
+    void eigenResidualNorms(dfx_ctx *ctx, dfx_op *H, dfx_op *M, View<double> X,
+                       const double *eig, double *res)
+    {
+      hipStream_t s = ctx->stream;
+      const u32   N = X.N, n = X.B;
+      View<double> x  = workView<double>(ctx, "rn.x", N, n);
+      View<double> hx = workView<double>(ctx, "rn.hx", N, n);
+      View<double> mx = workView<double>(ctx, "rn.mx", N, n);
+      k::convert<double, double>(s, x, cv(X));
+      applyOp<double>(H, x, hx);
+      applyOp<double>(M, x, mx);
+      std::vector<double> h(2 * (size_t)n);
+      for (u32 j = 0; j < n; ++j)
+        {
+          h[j]     = -1.0;
+          h[n + j] = eig[j];
+        }
+      double *d = workDoubles(ctx, "rn.coef", 2 * (size_t)n);
+      toDevice(ctx, d, h.data(), h.size());
+      k::axpbyBlocked<double>(s, x, 1.0, d, cv(hx), 1.0, d + n, cv(mx));
+      colDotsView<double>(ctx, cv(x), cv(x), res);
+      for (u32 j = 0; j < n; ++j)
+        res[j] = std::sqrt(res[j]);
+    }
+  } // namespace
+} // namespace dfx
+
+extern "C"
+{
+  int dfx_chfsi_create(dfx_ctx *ctx, const dfx_chfsi_params *p, dfx_mv *guess,
+                       dfx_chfsi **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(p && guess && out, "null argument");
+      DFX_REQUIRE(guess->dtype == DFX_F64, "the subspace guess must be FP64");
+      auto cs   = new dfx_chfsi();
+      cs->ctx   = ctx;
+      cs->p     = *p;
+      cs->guess = guess;
+      ctxRetain(ctx);
+      *out      = cs;
+      return 0;
+    });
+  }
+
+  int dfx_chfsi_reinit(dfx_chfsi *cs, double a, double b, double c, uint32_t degree,
+                       double tol, dfx_mv *guess)
+  {
+    if (!cs)
+      return DFX_E_ARG;
+    cs->p.wantedSpectrumLowerBound   = a;
+    cs->p.wantedSpectrumUpperBound   = b;
+    cs->p.unWantedSpectrumUpperBound = c;
+    cs->p.polynomialDegree           = degree;
+    cs->p.illConditionTolerance      = tol;
+    if (guess)
+      cs->guess = guess;
+    return 0;
+  }
+
+  int dfx_chfsi_solve(dfx_chfsi *cs, dfx_op *A, double *eig, dfx_mv *X, int computeEV,
+                      dfx_op *Bop, dfx_op *BInv)
+  {
+    if (!cs || !A || !eig || !X || !Bop || !BInv)
+      return DFX_E_ARG;
+    (void)computeEV;
+    return guard(cs->ctx, [&] { return chfsiSolve(cs, A, eig, X, Bop, BInv); });
+  }
+
+  int dfx_chfsi_timings(const dfx_chfsi *cs, double *ms3)
+  {
+    if (!cs || !ms3)
+      return DFX_E_ARG;
+    std::copy(cs->ms, cs->ms + 3, ms3);
+    return 0;
+  }
+
+  int dfx_chfsi_destroy(dfx_chfsi *cs)
+  {
+    if (!cs)
+      return DFX_E_ARG;
+    dfx_ctx *ctx = cs->ctx;
+    delete cs;
+    ctxRelease(ctx);
+    return 0;
+  }
+
+  int dfx_cholesky_gram_schmidt(dfx_ctx *ctx, dfx_mv *X, dfx_mv *Xo, dfx_op *Bop)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(X && Xo && Bop, "null argument");
+      return cholGS(ctx, mvView(X), mvView(Xo), Bop);
+    });
+  }
+
+  int dfx_rayleigh_ritz(dfx_ctx *ctx, dfx_op *A, dfx_mv *X, double *eig, dfx_mv *Xo)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(A && X && Xo && eig, "null argument");
+      return rayleighRitz(ctx, A, mvView(X), eig, mvView(Xo));
+    });
+  }
+
+  int dfx_lanczos_extreme(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv,
+                          dfx_mv *guess, uint32_t maxK, double *lo, double *hi,
+                          double *betaLast)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(A && Bop && BInv && guess && lo && hi && betaLast, "null argument");
+      DFX_REQUIRE(maxK >= 2, "maxKrylov must be >= 2");
+      lanczos(ctx, A, Bop, BInv, mvView(guess), maxK, lo, hi, betaLast);
+      return 0;
+    });
+  }
+
+  int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H, dfx_op *M,
+                        dfx_op *MInv, dfx_op *MLanczos, dfx_mv *guess,
+                        dfx_mv *lanczosGuess, double *energies, dfx_mv *waveFns,
+                        double *residualNorms, dfx_ks_result *result)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(p && H && M && MInv && MLanczos && guess && lanczosGuess && energies &&
+                    waveFns,
+                  "null argument");
+      const u32 n = waveFns->B, N = waveFns->N;
+      DFX_REQUIRE(p->numWantedEigenvalues == n, "numWantedEigenvalues must equal the width");
+      dfx_ks_result R{};
+      // ---- spectrum bounds (KohnShamEigenSolver.t.cpp:240-297)
+      auto t0 = std::chrono::steady_clock::now();
+      double lo, hi, beta;
+      lanczos(ctx, H, MLanczos, MInv, mvView(lanczosGuess),
+              p->lanczosMaxKrylov ? p->lanczosMaxKrylov : 20, &lo, &hi, &beta);
+      R.lanczosMs = std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t0).count();
+      const double residual = beta / 10.0;
+      double a = lo;
+      double b = (hi + residual - lo) * ((double)(n * 200.0) / N) + lo;
+      if (b >= hi + residual)
+        b = (hi + residual + lo) * 0.5;
+      const double c  = hi + residual;
+      const u32    m  = p->polynomialDegree ? p->polynomialDegree : chebyDegree(c);
+      R.lanczosLow  = lo;
+      R.lanczosHigh = hi;
+      R.lanczosBeta = beta;
+      R.degree      = m;
+
+      dfx_chfsi cs;
+      cs.ctx   = ctx;
+      cs.guess = guess;
+      cs.p.illConditionTolerance     = 1e-14;
+      cs.p.eigenVectorBatchSize      = p->waveFunctionBatchSize;
+      cs.p.precision                 = p->precision;
+      cs.p.useGraph                  = 0;
+      // residual-filter runs start from one degree-1 plain-filter pass
+      // (src/ksdft/KohnShamDFT.t.cpp:462-487)
+      if (p->isResidualChebyshevFilter)
+        {
+          cs.p.isResidualChebyshevFilter  = 0;
+          cs.p.wantedSpectrumLowerBound   = a;
+          cs.p.wantedSpectrumUpperBound   = b;
+          cs.p.unWantedSpectrumUpperBound = c;
+          cs.p.polynomialDegree           = 1;
+          int rc = chfsiSolve(&cs, H, energies, waveFns, M, MInv);
+          if (rc != 0)
+            return DFX_EIG_KS_CHFSI_ERROR;
+          k::convert<double, double>(ctx->stream, mvView(guess), cv(mvView(waveFns)));
+        }
+      cs.p.isResidualChebyshevFilter  = p->isResidualChebyshevFilter;
+      cs.p.wantedSpectrumLowerBound   = a;
+      cs.p.wantedSpectrumUpperBound   = b;
+      cs.p.unWantedSpectrumUpperBound = c;
+      cs.p.polynomialDegree           = m;
+      std::vector<double> res(n, 0.0);
+      u32 pass = 0;
+      int status = DFX_EIG_KS_MAX_PASS_ERROR;
+      for (; pass < p->maxChebyshevFilterPass; ++pass)
+        {
+          int rc = chfsiSolve(&cs, H, energies, waveFns, M, MInv);
+          R.filterMs += cs.ms[0];
+          R.orthoMs += cs.ms[1];
+          R.rrMs += cs.ms[2];
+          if (rc != 0)
+            {
+              status = DFX_EIG_KS_CHFSI_ERROR;
+              break;
+            }
+          auto tr = std::chrono::steady_clock::now();
+          eigenResidualNorms(ctx, H, M, mvView(waveFns), energies, res.data());
+          R.residualMs += std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - tr).count();
+          k::convert<double, double>(ctx->stream, mvView(guess), cv(mvView(waveFns)));
+          bool conv = true;
+          for (u32 j = 0; j < n; ++j)
+            conv = conv && (res[j] <= p->eigenSolveResidualTolerance);
+          if (conv)
+            {
+              status = 0;
+              ++pass;
+              break;
+            }
+          cs.p.wantedSpectrumLowerBound = energies[0];
+          cs.p.wantedSpectrumUpperBound = energies[n - 1];
+        }
+      R.passes = pass;
+      R.a      = cs.p.wantedSpectrumLowerBound;
+      R.b      = cs.p.wantedSpectrumUpperBound;
+      R.c      = c;
+      if (residualNorms)
+        std::copy(res.begin(), res.end(), residualNorms);
+      if (result)
+        *result = R;
+      return status;
+    });
+  }
+
+  // CGLinearSolver::solve (src/linearAlgebra/CGLinearSolver.t.cpp:68-313)
+  int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *bmv, dfx_mv *xmv,
+                   double absTol, double relTol, double divTol, uint32_t maxIter,
+                   uint32_t *iterations, double *residualNorms)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(A && pc && bmv && xmv, "null argument");
+      hipStream_t  s = ctx->stream;
+      View<double> b = mvView(bmv), xo = mvView(xmv);
+      DFX_REQUIRE(b.N == xo.N && b.B == xo.B, "b and x differ in shape");
+      const u32 N = b.N, B = b.B;
+      View<double> x  = workView<double>(ctx, "cgs.x", N, B);
+      View<double> r  = workView<double>(ctx, "cgs.r", N, B);
+      View<double> w  = workView<double>(ctx, "cgs.w", N, B);
+      View<double> z  = workView<double>(ctx, "cgs.z", N, B);
+      View<double> pv = workView<double>(ctx, "cgs.p", N, B);
+      double *coef = workDoubles(ctx, "cgs.coef", 4 * (size_t)B);
+      k::convert<double, double>(s, x, cv(xo));
+      std::vector<double> bNorm(B), rNorm(B), zr(B), pw(B), zrNew(B), h(4 * (size_t)B);
+      colDotsView<double>(ctx, cv(b), cv(b), bNorm.data());
+      for (auto &v : bNorm)
+        v = std::sqrt(v);
+      std::vector<char> conv(B, 0);
+      bool diverged = false, allConv = false;
+      int  err  = DFX_LS_OTHER_ERROR;
+      u32  iter = 0;
+      for (; iter <= maxIter; ++iter)
+        {
+          if (iter == 0)
+            {
+              applyOp<double>(A, x, w);
+              k::axpby<double>(s, r, 1.0, cv(b), -1.0, cv(w));
+              applyOp<double>(pc, r, z);
+              k::convert<double, double>(s, pv, cv(z));
+            }
+          else
+            {
+              applyOp<double>(A, pv, w);
+              colDotsView<double>(ctx, cv(z), cv(r), zr.data());
+              colDotsView<double>(ctx, cv(pv), cv(w), pw.data());
+              for (u32 j = 0; j < B; ++j)
+                {
+                  h[j]         = 1.0;
+                  h[B + j]     = zr[j] / pw[j];
+                  h[2 * B + j] = -zr[j] / pw[j];
+                }
+              toDevice(ctx, coef, h.data(), 3 * (size_t)B);
+              k::axpbyBlocked<double>(s, x, 1.0, coef, cv(x), 1.0, coef + B, cv(pv));
+              k::axpbyBlocked<double>(s, r, 1.0, coef, cv(r), 1.0, coef + 2 * B, cv(w));
+              applyOp<double>(pc, r, z);
+              colDotsView<double>(ctx, cv(z), cv(r), zrNew.data());
+              for (u32 j = 0; j < B; ++j)
+                h[3 * B + j] = zrNew[j] / zr[j];
+              toDevice(ctx, coef + 3 * B, h.data() + 3 * B, B);
+              k::axpbyBlocked<double>(s, pv, 1.0, coef, cv(z), 1.0, coef + 3 * B, cv(pv));
+            }
+          colDotsView<double>(ctx, cv(r), cv(r), rNorm.data());
+          for (auto &v : rNorm)
+            v = std::sqrt(v);
+          std::vector<double> mask(2 * (size_t)B, 0.0);
+          bool any = false;
+          for (u32 j = 0; j < B; ++j)
+            {
+              mask[B + j] = 1.0;
+              if (rNorm[j] < std::max(absTol, bNorm[j] * relTol) && !conv[j])
+                {
+                  err         = 0;
+                  conv[j]     = 1;
+                  mask[j]     = 1.0;
+                  mask[B + j] = 0.0;
+                  any         = true;
+                }
+              if (rNorm[j] > divTol && !diverged)
+                {
+                  err      = DFX_LS_RESIDUAL_DIVERGENCE;
+                  diverged = true;
+                }
+            }
+          if (any)
+            {
+              double *dm = workDoubles(ctx, "cgs.mask", 2 * (size_t)B);
+              toDevice(ctx, dm, mask.data(), mask.size());
+              k::axpbyBlocked<double>(s, xo, 1.0, dm, cv(x), 1.0, dm + B, cv(xo));
+            }
+          allConv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
+          if (diverged || allConv)
+            break;
+        }
+      DFX_HIP(hipStreamSynchronize(s));
+      if (iter > maxIter)
+        err = DFX_LS_FAILED_TO_CONVERGE;
+      if (iterations)
+        *iterations = iter;
+      if (residualNorms)
+        std::copy(rNorm.begin(), rNorm.end(), residualNorms);
+      return err;
+    });
+  }
+}

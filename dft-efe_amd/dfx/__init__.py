@@ -1,0 +1,512 @@
+"""Python front end of libdfx, the MI355X-native Kohn-Sham eigensolver hot path.
+
+Thin ctypes wrappers over the C ABI in include/dfx.h.  Class names follow the
+reference's operator / solver interfaces (dft-efe):
+
+  CellOperator                  KohnShamOperatorContextFE, CFEOverlapOperatorContext,
+                                LaplaceOperatorContextFE   (cell-wise apply)
+  DiagonalOperator              CFEOverlapInverseOpContextGLL,
+                                OrthoEFEOverlapInverseOpContextGLL, PreconditionerJacobi
+  ChebyshevFilteredEigenSolver  linearAlgebra::ChebyshevFilteredEigenSolver
+  KohnShamEigenSolver           ksdft::KohnShamEigenSolver
+  CGLinearSolver                linearAlgebra::CGLinearSolver
+
+The compute path is the HIP library only: if libdfx.so is missing, importing
+this module raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdfx.so")
+
+F64, F32 = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+OP_KEEP_FP32 = 0x1
+
+EIG_CODES = {0: "SUCCESS", 1: "LAPACK_ERROR", 2: "LANCZOS_BETA_ZERO",
+             3: "LANCZOS_SUBSPACE_INSUFFICIENT", 4: "CHFSI_ORTHONORMALIZATION_ERROR",
+             5: "CHFSI_RAYLEIGH_RITZ_ERROR", 6: "KS_MAX_PASS_ERROR",
+             7: "KS_CHFSI_ERROR", 8: "KS_LANCZOS_ERROR"}
+LS_CODES = {0: "SUCCESS", 1: "FAILED_TO_CONVERGE", 2: "RESIDUAL_DIVERGENCE",
+            3: "DIVISON_BY_ZERO", 4: "OTHER_ERROR"}
+
+
+class DfxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dfx error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libdfx.so not found at {LIB_PATH}; build it with "
+            "`python dft-efe_amd/build.py` (or __graft_entry__.build())")
+    return C.CDLL(LIB_PATH)
+
+
+_lib = _load()
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_dbl = C.c_double
+_pp = C.POINTER(C.c_void_p)
+
+
+def _sig(name, *args, res=C.c_int):
+    f = getattr(_lib, name)
+    f.argtypes = list(args)
+    f.restype = res
+    return f
+
+
+_sig("dfx_ctx_create", C.c_int, _pp)
+_sig("dfx_ctx_destroy", _vp)
+_sig("dfx_ctx_last_error", _vp, res=C.c_char_p)
+_sig("dfx_ctx_synchronize", _vp)
+_sig("dfx_ctx_stream", _vp, _pp)
+_sig("dfx_mv_create", _vp, _u32, _u32, C.c_int, _pp)
+_sig("dfx_mv_wrap", _vp, _vp, _u32, _u32, C.c_int, _pp)
+_sig("dfx_mv_destroy", _vp)
+_sig("dfx_mv_upload", _vp, _vp)
+_sig("dfx_mv_download", _vp, _vp)
+_sig("dfx_mv_device_ptr", _vp, _pp)
+_sig("dfx_mv_set", _vp, _dbl)
+_sig("dfx_mv_copy", _vp, _vp)
+_sig("dfx_mv_axpby", _vp, _dbl, _vp, _dbl, _vp)
+_sig("dfx_mv_axpby_blocked", _vp, _dbl, _vp, _vp, _dbl, _vp, _vp)
+_sig("dfx_mv_l2norms", _vp, _vp)
+_sig("dfx_mv_dots", _vp, _vp, _vp)
+_sig("dfx_constraints_create", _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _pp)
+_sig("dfx_constraints_destroy", _vp)
+_sig("dfx_constraints_distribute_parent_to_child", _vp, _vp)
+_sig("dfx_constraints_distribute_child_to_parent", _vp, _vp)
+_sig("dfx_constraints_set_constrained_nodes", _vp, _vp, _dbl)
+_sig("dfx_constraints_sizes", _vp, _vp, _vp, _vp)
+_sig("dfx_constraints_export", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_op_create_cellwise", _vp, _u32, _vp, _vp, _vp, _u32, _u32, _vp, _u32, _pp)
+_sig("dfx_op_update_cells", _vp, _vp)
+_sig("dfx_op_create_diagonal", _vp, _u32, _vp, _u32, _vp, _vp, _pp)
+_sig("dfx_op_apply", _vp, _vp, _vp)
+_sig("dfx_op_destroy", _vp)
+_sig("dfx_op_get_info", _vp, _vp)
+_sig("dfx_op_export_map", _vp, _vp, _vp)
+_sig("dfx_op_gather", _vp, _vp, _vp)
+_sig("dfx_op_scatter_add", _vp, _vp, _vp)
+_sig("dfx_chfsi_create", _vp, _vp, _vp, _pp)
+_sig("dfx_chfsi_reinit", _vp, _dbl, _dbl, _dbl, _u32, _dbl, _vp)
+_sig("dfx_chfsi_solve", _vp, _vp, _vp, _vp, C.c_int, _vp, _vp)
+_sig("dfx_chfsi_timings", _vp, _vp)
+_sig("dfx_chfsi_destroy", _vp)
+_sig("dfx_cholesky_gram_schmidt", _vp, _vp, _vp, _vp)
+_sig("dfx_rayleigh_ritz", _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_lanczos_extreme", _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp)
+_sig("dfx_ks_eigensolve", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_cg_solve", _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _u32, _vp, _vp)
+
+
+def exported_symbols():
+    """Names this front end binds (all must exist in libdfx.so)."""
+    return sorted(n for n in dir(_lib) if n.startswith("dfx_"))
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def _u32a(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64a(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One GPU and one HIP stream (LinAlgOpContext<DEVICE>)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = _lib.dfx_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise DfxError(rc, _lib.dfx_ctx_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def check(self, rc, allow_positive=False):
+        if rc < 0 or (rc > 0 and not allow_positive):
+            raise DfxError(rc, _lib.dfx_ctx_last_error(self.h).decode())
+        return rc
+
+    def synchronize(self):
+        self.check(_lib.dfx_ctx_synchronize(self.h))
+
+    @property
+    def stream(self):
+        s = C.c_void_p()
+        _lib.dfx_ctx_stream(self.h, C.byref(s))
+        return s.value or 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultiVector:
+    """N x B device multivector, row-major, vector index fastest."""
+
+    def __init__(self, ctx, N, B, dtype=F64):
+        self.ctx, self.N, self.B, self.dtype = ctx, int(N), int(B), dtype
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_mv_create(ctx.h, self.N, self.B, dtype, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_numpy(cls, ctx, a, dtype=None):
+        a = np.asarray(a)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        if dtype is None:
+            dtype = F32 if a.dtype == np.float32 else F64
+        mv = cls(ctx, a.shape[0], a.shape[1], dtype)
+        mv.upload(a)
+        return mv
+
+    @property
+    def np_dtype(self):
+        return np.float64 if self.dtype == F64 else np.float32
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.np_dtype).reshape(self.N, self.B)
+        self.ctx.check(_lib.dfx_mv_upload(self.h, _ptr(a)))
+
+    def to_numpy(self):
+        out = np.empty((self.N, self.B), dtype=self.np_dtype)
+        self.ctx.check(_lib.dfx_mv_download(self.h, _ptr(out)))
+        return out
+
+    def set(self, v):
+        self.ctx.check(_lib.dfx_mv_set(self.h, float(v)))
+
+    def copy_from(self, other):
+        self.ctx.check(_lib.dfx_mv_copy(self.h, other.h))
+
+    def axpby(self, a, X, b, Y):
+        """self = a X + b Y"""
+        self.ctx.check(_lib.dfx_mv_axpby(self.h, float(a), X.h, float(b), Y.h))
+
+    def axpby_blocked(self, a1, a, X, b1, b, Y):
+        a = _f64a(a); b = _f64a(b)
+        self.ctx.check(_lib.dfx_mv_axpby_blocked(self.h, float(a1), _ptr(a), X.h,
+                                                 float(b1), _ptr(b), Y.h))
+
+    def l2norms(self):
+        out = np.zeros(self.B)
+        self.ctx.check(_lib.dfx_mv_l2norms(self.h, _ptr(out)))
+        return out
+
+    def dots(self, Y):
+        out = np.zeros(self.B)
+        self.ctx.check(_lib.dfx_mv_dots(self.h, Y.h, _ptr(out)))
+        return out
+
+    @property
+    def device_ptr(self):
+        p = C.c_void_p()
+        _lib.dfx_mv_device_ptr(self.h, C.byref(p))
+        return p.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_mv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Constraints:
+    """Hanging-node + Dirichlet constraints (basis::ConstraintsLocal), flattened."""
+
+    def __init__(self, ctx, nDofs, rows, rowSizes, rowOffsets, cols, weights, inhom=None):
+        self.ctx = ctx
+        rows, rowSizes, rowOffsets = _u32a(rows), _u32a(rowSizes), _u32a(rowOffsets)
+        cols, weights = _u32a(cols), _f64a(weights)
+        inhom = np.zeros(len(rows)) if inhom is None else _f64a(inhom)
+        self._keep = (rows, rowSizes, rowOffsets, cols, weights, inhom)
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_constraints_create(
+            ctx.h, int(nDofs), len(rows), _ptr(rows), _ptr(rowSizes), _ptr(rowOffsets),
+            _ptr(cols), _ptr(weights), _ptr(inhom), C.byref(h)))
+        self.h = h
+        self.nDofs = int(nDofs)
+
+    def distribute_parent_to_child(self, X):
+        self.ctx.check(_lib.dfx_constraints_distribute_parent_to_child(self.h, X.h))
+
+    def distribute_child_to_parent(self, Y):
+        self.ctx.check(_lib.dfx_constraints_distribute_child_to_parent(self.h, Y.h))
+
+    def set_constrained_nodes(self, X, v):
+        self.ctx.check(_lib.dfx_constraints_set_constrained_nodes(self.h, X.h, float(v)))
+
+    def export(self):
+        nr, ne, npar = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _lib.dfx_constraints_sizes(self.h, C.byref(nr), C.byref(ne), C.byref(npar))
+        nr, ne, npar = nr.value, ne.value, npar.value
+        d = dict(rows=np.zeros(nr, np.uint32), rowOffsets=np.zeros(nr + 1, np.uint32),
+                 cols=np.zeros(ne, np.uint32), parents=np.zeros(npar, np.uint32),
+                 parentOffsets=np.zeros(npar + 1, np.uint32),
+                 parentRows=np.zeros(ne, np.uint32), parentEntries=np.zeros(ne, np.uint32))
+        self.ctx.check(_lib.dfx_constraints_export(
+            self.h, _ptr(d["rows"]), _ptr(d["rowOffsets"]), _ptr(d["cols"]),
+            _ptr(d["parents"]), _ptr(d["parentOffsets"]), _ptr(d["parentRows"]),
+            _ptr(d["parentEntries"])))
+        return d
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_constraints_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _OpInfo(C.Structure):
+    _fields_ = [("nCells", C.c_uint32), ("nDofs", C.c_uint32), ("nClassical", C.c_uint32),
+                ("nColors", C.c_uint32), ("maxCellDofs", C.c_uint32),
+                ("sumNe", C.c_uint64), ("sumNe2", C.c_uint64),
+                ("matBytesF64", C.c_uint64), ("matBytesF32", C.c_uint64)]
+
+
+class _Operator:
+    def apply(self, X, Y):
+        """Y = Op X (X's constrained rows are set by parent->child, as in the
+        reference OperatorContext::apply)."""
+        self.ctx.check(_lib.dfx_op_apply(self.h, X.h, Y.h))
+
+    def info(self):
+        i = _OpInfo()
+        _lib.dfx_op_get_info(self.h, C.byref(i))
+        return {k: getattr(i, k) for k, _ in _OpInfo._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CellOperator(_Operator):
+    """Cell-wise operator Y = C^T (sum_e Q_e^T A_e Q_e) C X
+    (KohnShamOperatorContextFE / CFEOverlapOperatorContext / LaplaceOperatorContextFE)."""
+
+    def __init__(self, ctx, cellNdofs, cellMap, cellMatrices, nDofs, nClassical=None,
+                 constraints=None, keep_fp32=False):
+        self.ctx = ctx
+        cellNdofs, cellMap = _u32a(cellNdofs), _u32a(cellMap)
+        mats = _f64a(cellMatrices)
+        nClassical = int(nDofs if nClassical is None else nClassical)
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_op_create_cellwise(
+            ctx.h, len(cellNdofs), _ptr(cellNdofs), _ptr(cellMap), _ptr(mats), int(nDofs),
+            nClassical, constraints.h if constraints else None,
+            OP_KEEP_FP32 if keep_fp32 else 0, C.byref(h)))
+        self.h = h
+        self.constraints = constraints
+        self.nDofs = int(nDofs)
+        self.sumNe = int(cellNdofs.astype(np.uint64).sum())
+        self.nCells = len(cellNdofs)
+
+    def update_cells(self, cellMatrices):
+        mats = _f64a(cellMatrices)
+        self.ctx.check(_lib.dfx_op_update_cells(self.h, _ptr(mats)))
+
+    def export_map(self):
+        n = np.zeros(self.nCells, np.uint32)
+        m = np.zeros(self.sumNe, np.uint32)
+        self.ctx.check(_lib.dfx_op_export_map(self.h, _ptr(n), _ptr(m)))
+        return n, m
+
+    def gather(self, X, Xcell):
+        self.ctx.check(_lib.dfx_op_gather(self.h, X.h, Xcell.h))
+
+    def scatter_add(self, Ycell, Y):
+        self.ctx.check(_lib.dfx_op_scatter_add(self.h, Ycell.h, Y.h))
+
+
+class DiagonalOperator(_Operator):
+    """Y = C2P(diag o P2C(X) + [0; E X_enr])  (CFEOverlapInverseOpContextGLL,
+    OrthoEFEOverlapInverseOpContextGLL, PreconditionerJacobi when constraints is None)."""
+
+    def __init__(self, ctx, diag, nEnriched=0, enrBlock=None, constraints=None):
+        self.ctx = ctx
+        diag = _f64a(diag)
+        blk = _f64a(enrBlock) if nEnriched else None
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_op_create_diagonal(
+            ctx.h, len(diag), _ptr(diag), int(nEnriched), _ptr(blk) if blk is not None else None,
+            constraints.h if constraints else None, C.byref(h)))
+        self.h = h
+        self.nDofs = len(diag)
+        self.constraints = constraints
+
+
+class _ChfsiParams(C.Structure):
+    _fields_ = [("wantedSpectrumLowerBound", C.c_double),
+                ("wantedSpectrumUpperBound", C.c_double),
+                ("unWantedSpectrumUpperBound", C.c_double),
+                ("polynomialDegree", C.c_uint32), ("illConditionTolerance", C.c_double),
+                ("isResidualChebyshevFilter", C.c_int),
+                ("eigenVectorBatchSize", C.c_uint32), ("precision", C.c_int),
+                ("useGraph", C.c_int)]
+
+
+class ChebyshevFilteredEigenSolver:
+    """linearAlgebra::ChebyshevFilteredEigenSolver on one GPU."""
+
+    def __init__(self, ctx, wantedSpectrumLowerBound, wantedSpectrumUpperBound,
+                 unWantedSpectrumUpperBound, polynomialDegree, illConditionTolerance,
+                 eigenSubspaceGuess, isResidualChebyshevFilter=True,
+                 eigenVectorBatchSize=0, precision=PREC_FP64, useGraph=False):
+        self.ctx = ctx
+        p = _ChfsiParams(wantedSpectrumLowerBound, wantedSpectrumUpperBound,
+                         unWantedSpectrumUpperBound, int(polynomialDegree),
+                         illConditionTolerance, 1 if isResidualChebyshevFilter else 0,
+                         int(eigenVectorBatchSize), int(precision), 1 if useGraph else 0)
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_chfsi_create(ctx.h, C.byref(p), eigenSubspaceGuess.h, C.byref(h)))
+        self.h = h
+        self.guess = eigenSubspaceGuess
+
+    def reinit(self, a, b, c, degree, tol, guess=None):
+        if guess is not None:
+            self.guess = guess
+        _lib.dfx_chfsi_reinit(self.h, a, b, c, int(degree), tol,
+                              guess.h if guess is not None else None)
+
+    def solve(self, A, eigenValues, eigenVectors, computeEigenVectors=True, B=None, BInv=None):
+        """Returns the EigenSolverErrorCode; eigenValues (numpy, in/out) updated in place."""
+        ev = eigenValues
+        assert ev.dtype == np.float64 and ev.flags.c_contiguous
+        rc = _lib.dfx_chfsi_solve(self.h, A.h, _ptr(ev), eigenVectors.h,
+                                  1 if computeEigenVectors else 0, B.h, BInv.h)
+        return self.ctx.check(rc, allow_positive=True)
+
+    def timings(self):
+        out = np.zeros(3)
+        _lib.dfx_chfsi_timings(self.h, _ptr(out))
+        return {"filter_ms": out[0], "ortho_ms": out[1], "rr_ms": out[2]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_chfsi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cholesky_gram_schmidt(ctx, X, Xo, B):
+    return ctx.check(_lib.dfx_cholesky_gram_schmidt(ctx.h, X.h, Xo.h, B.h), allow_positive=True)
+
+
+def rayleigh_ritz(ctx, A, X, Xo):
+    eig = np.zeros(X.B)
+    rc = ctx.check(_lib.dfx_rayleigh_ritz(ctx.h, A.h, X.h, _ptr(eig), Xo.h), allow_positive=True)
+    return rc, eig
+
+
+def lanczos_extreme(ctx, A, B, BInv, guess, max_krylov=20):
+    lo, hi, beta = C.c_double(), C.c_double(), C.c_double()
+    ctx.check(_lib.dfx_lanczos_extreme(ctx.h, A.h, B.h, BInv.h, guess.h, int(max_krylov),
+                                       C.byref(lo), C.byref(hi), C.byref(beta)))
+    return lo.value, hi.value, beta.value
+
+
+class _KsParams(C.Structure):
+    _fields_ = [("numWantedEigenvalues", C.c_uint32), ("maxChebyshevFilterPass", C.c_uint32),
+                ("polynomialDegree", C.c_uint32), ("eigenSolveResidualTolerance", C.c_double),
+                ("waveFunctionBatchSize", C.c_uint32), ("isResidualChebyshevFilter", C.c_int),
+                ("precision", C.c_int), ("lanczosMaxKrylov", C.c_uint32)]
+
+
+class _KsResult(C.Structure):
+    _fields_ = [("passes", C.c_uint32), ("lanczosLow", C.c_double),
+                ("lanczosHigh", C.c_double), ("lanczosBeta", C.c_double),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
+                ("degree", C.c_uint32), ("filterMs", C.c_double), ("orthoMs", C.c_double),
+                ("rrMs", C.c_double), ("residualMs", C.c_double), ("lanczosMs", C.c_double)]
+
+
+class KohnShamEigenSolver:
+    """ksdft::KohnShamEigenSolver (Lanczos bounds, degree lookup, ChFSI passes)."""
+
+    def __init__(self, ctx, numWantedEigenvalues, eigenSolveResidualTolerance,
+                 maxChebyshevFilterPass, waveFunctionSubspaceGuess, lanczosGuess,
+                 polynomialDegree=0, waveFunctionBatchSize=0,
+                 isResidualChebyshevFilter=True, precision=PREC_FP64, lanczosMaxKrylov=20):
+        self.ctx = ctx
+        self.p = _KsParams(int(numWantedEigenvalues), int(maxChebyshevFilterPass),
+                           int(polynomialDegree), float(eigenSolveResidualTolerance),
+                           int(waveFunctionBatchSize), 1 if isResidualChebyshevFilter else 0,
+                           int(precision), int(lanczosMaxKrylov))
+        self.guess = waveFunctionSubspaceGuess
+        self.lanczosGuess = lanczosGuess
+        self.result = None
+        self.residualNorms = None
+
+    def solve(self, H, energies, waveFunctions, M, MInv, MLanczos):
+        res = np.zeros(self.p.numWantedEigenvalues)
+        r = _KsResult()
+        rc = _lib.dfx_ks_eigensolve(self.ctx.h, C.byref(self.p), H.h, M.h, MInv.h, MLanczos.h,
+                                    self.guess.h, self.lanczosGuess.h, _ptr(energies),
+                                    waveFunctions.h, _ptr(res), C.byref(r))
+        self.ctx.check(rc, allow_positive=True)
+        self.residualNorms = res
+        self.result = {k: getattr(r, k) for k, _ in _KsResult._fields_}
+        return rc
+
+
+class CGLinearSolver:
+    """linearAlgebra::CGLinearSolver (multi-RHS PCG, per-column convergence)."""
+
+    def __init__(self, ctx, maxIter, absoluteTol, relativeTol, divergenceTol):
+        self.ctx, self.maxIter = ctx, int(maxIter)
+        self.absTol, self.relTol, self.divTol = absoluteTol, relativeTol, divergenceTol
+        self.iterations = 0
+        self.residualNorms = None
+
+    def solve(self, A, pc, b, x):
+        it = C.c_uint32()
+        res = np.zeros(b.B)
+        rc = _lib.dfx_cg_solve(self.ctx.h, A.h, pc.h, b.h, x.h, self.absTol, self.relTol,
+                               self.divTol, self.maxIter, C.byref(it), _ptr(res))
+        self.ctx.check(rc, allow_positive=True)
+        self.iterations = it.value
+        self.residualNorms = res
+        return rc

@@ -1,0 +1,294 @@
+/*
+ * dfx.h -- C ABI of the MI355X-native Kohn-Sham eigensolver hot path
+ * (gfx950 / CDNA4).  Plain pointers and sizes only; no exceptions cross this
+ * boundary.  Every entry point returns an int status:
+ *     0            success
+ *   < 0            library error (DFX_E_*), message in dfx_ctx_last_error()
+ *   > 0            the reference's own error code for that solver
+ *                  (EigenSolverErrorCode, OrthonormalizationErrorCode or
+ *                  LinearSolverErrorCode, src/linearAlgebra/LinearAlgebraTypes.h:84-125)
+ *
+ * Layout conventions (same as the reference):
+ *  - multivectors are N x B row-major, vector index fastest: entry (dof i,
+ *    vector j) at data[i*B + j]  (src/linearAlgebra/MultiVector.h:50-53);
+ *  - cell matrices are concatenated per cell, cell e's n_e x n_e block
+ *    column-major at offset sum_{e'<e} n_{e'}^2
+ *    (src/ksdft/KohnShamOperatorContextFE.t.cpp:1194-1195);
+ *  - cell->dof maps are concatenated per cell, classical dofs first, then
+ *    enriched ids (+ nClassical)  (src/basis/FEBasisManager.h:116-145);
+ *  - constraints are the flattened arrays of
+ *    src/basis/CFEConstraintsLocalDealii.h:160-170 (rows, rowSizes,
+ *    rowOffsets, cols, weights, inhomogeneities), rows ascending.
+ *
+ * Threading: calls on one dfx_ctx are serialized on that context's HIP
+ * stream; different contexts may be driven from different host threads.
+ */
+#ifndef DFX_H
+#define DFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfx_ctx         dfx_ctx;
+typedef struct dfx_mv          dfx_mv;
+typedef struct dfx_constraints dfx_constraints;
+typedef struct dfx_op          dfx_op;
+typedef struct dfx_chfsi       dfx_chfsi;
+typedef struct dfx_problem     dfx_problem;
+typedef struct dfx_server      dfx_server;
+
+/* ---- status codes ---------------------------------------------------- */
+#define DFX_SUCCESS          0
+#define DFX_E_ARG           (-1)
+#define DFX_E_HIP           (-2)
+#define DFX_E_OOM           (-3)
+#define DFX_E_STATE         (-4)
+#define DFX_E_UNSUPPORTED   (-5)
+#define DFX_E_NONFINITE     (-6)
+
+/* EigenSolverErrorCode (LinearAlgebraTypes.h:93-106) */
+#define DFX_EIG_LAPACK_ERROR                   1
+#define DFX_EIG_LANCZOS_BETA_ZERO              2
+#define DFX_EIG_LANCZOS_SUBSPACE_INSUFFICIENT  3
+#define DFX_EIG_CHFSI_ORTHONORMALIZATION_ERROR 4
+#define DFX_EIG_CHFSI_RAYLEIGH_RITZ_ERROR      5
+#define DFX_EIG_KS_MAX_PASS_ERROR              6
+#define DFX_EIG_KS_CHFSI_ERROR                 7
+#define DFX_EIG_KS_LANCZOS_ERROR               8
+/* OrthonormalizationErrorCode (LinearAlgebraTypes.h:108-115) */
+#define DFX_ORTHO_LAPACK_ERROR                      1
+#define DFX_ORTHO_NON_ORTHONORMALIZABLE_MULTIVECTOR 2
+/* LinearSolverErrorCode (LinearAlgebraTypes.h:84-91) */
+#define DFX_LS_FAILED_TO_CONVERGE  1
+#define DFX_LS_RESIDUAL_DIVERGENCE 2
+#define DFX_LS_DIVISON_BY_ZERO     3
+#define DFX_LS_OTHER_ERROR         4
+
+/* ---- value types ----------------------------------------------------- */
+#define DFX_F64 0
+#define DFX_F32 1
+
+/* ======================================================================
+ * Context: one GPU, one HIP stream.
+ * Replaces LinAlgOpContext<DEVICE> (src/linearAlgebra/LinAlgOpContext.h)
+ * and the BlasQueue it carries (src/linearAlgebra/BlasLapackTypedef.h:188-211).
+ * ==================================================================== */
+int         dfx_ctx_create(int device, dfx_ctx **ctx);
+int         dfx_ctx_destroy(dfx_ctx *ctx);
+const char *dfx_ctx_last_error(const dfx_ctx *ctx);
+int         dfx_ctx_synchronize(dfx_ctx *ctx);
+/* the hipStream_t all work of this context is launched on */
+int         dfx_ctx_stream(dfx_ctx *ctx, void **stream);
+
+/* ======================================================================
+ * Device multivector: N x B, row-major, vector index fastest.
+ * Replaces linearAlgebra::MultiVector<T, DEVICE> storage
+ * (src/linearAlgebra/MultiVector.h:133-507).
+ * ==================================================================== */
+int dfx_mv_create(dfx_ctx *ctx, uint32_t N, uint32_t B, int dtype, dfx_mv **mv);
+/* wrap caller-owned device memory (not freed by dfx_mv_destroy) */
+int dfx_mv_wrap(dfx_ctx *ctx, void *device_ptr, uint32_t N, uint32_t B,
+                int dtype, dfx_mv **mv);
+int dfx_mv_destroy(dfx_mv *mv);
+int dfx_mv_upload(dfx_mv *mv, const void *host);
+int dfx_mv_download(const dfx_mv *mv, void *host);
+int dfx_mv_device_ptr(dfx_mv *mv, void **ptr);
+int dfx_mv_set(dfx_mv *mv, double value);
+/* dst = src with value-type conversion; same N and B */
+int dfx_mv_copy(dfx_mv *dst, const dfx_mv *src);
+/* Z = a X + b Y  (blasLapack::axpby, src/linearAlgebra/BlasLapack.t.cpp:228-237) */
+int dfx_mv_axpby(dfx_mv *Z, double a, const dfx_mv *X, double b, const dfx_mv *Y);
+/* Z[i,j] = a1 a[j] X[i,j] + b1 b[j] Y[i,j]  (blasLapack::axpbyBlocked,
+ * src/linearAlgebra/BlasLapack.t.cpp:242-257); a, b host arrays of length B */
+int dfx_mv_axpby_blocked(dfx_mv *Z, double a1, const double *a, const dfx_mv *X,
+                         double b1, const double *b, const dfx_mv *Y);
+/* column 2-norms and column dots, FP64 results on the host
+ * (MultiVector::l2Norms / linearAlgebra::dot, src/linearAlgebra/MultiVector.h:620) */
+int dfx_mv_l2norms(const dfx_mv *X, double *norms);
+int dfx_mv_dots(const dfx_mv *X, const dfx_mv *Y, double *dots);
+
+/* ======================================================================
+ * Constraints (hanging-node + Dirichlet), device resident.
+ * Replaces basis::ConstraintsLocal (src/basis/ConstraintsLocal.h:77-85) with
+ * the flattened arrays of CFEConstraintsLocalDealii
+ * (src/basis/CFEConstraintsLocalDealii.t.cpp:290-460).
+ * distribute_child_to_parent is deterministic: each parent gathers its
+ * children's contributions in row order (no atomics), which reproduces the
+ * reference's sequential loop (src/basis/ConstraintsInternal.cpp:113-170).
+ * ==================================================================== */
+int dfx_constraints_create(dfx_ctx *ctx, uint32_t nDofs, uint32_t nRows,
+                           const uint32_t *rows, const uint32_t *rowSizes,
+                           const uint32_t *rowOffsets, const uint32_t *cols,
+                           const double *weights, const double *inhom,
+                           dfx_constraints **out);
+int dfx_constraints_destroy(dfx_constraints *c);
+/* x[row,:] = inhom + sum_j w_j x[col_j,:]  (ConstraintsInternal.cpp:38-107) */
+int dfx_constraints_distribute_parent_to_child(dfx_constraints *c, dfx_mv *X);
+/* x[col_j,:] += w_j x[row,:]; x[row,:] = 0  (ConstraintsInternal.cpp:113-170) */
+int dfx_constraints_distribute_child_to_parent(dfx_constraints *c, dfx_mv *Y);
+/* x[row,:] = value  (ConstraintsInternal.cpp:176-217) */
+int dfx_constraints_set_constrained_nodes(dfx_constraints *c, dfx_mv *X, double value);
+/* Download the device-resident index sets: rows (nRows), rowOffsets
+ * (nRows + 1), cols (nEntries), and the parent-major transpose used by
+ * child->parent: parents (nParents), parentOffsets (nParents + 1),
+ * parentRows (nEntries, index into rows), parentEntries (nEntries, index into
+ * cols/weights).  Pass NULL for arrays not wanted; sizes via dfx_constraints_sizes. */
+int dfx_constraints_sizes(const dfx_constraints *c, uint32_t *nRows,
+                          uint32_t *nEntries, uint32_t *nParents);
+int dfx_constraints_export(const dfx_constraints *c, uint32_t *rows,
+                           uint32_t *rowOffsets, uint32_t *cols,
+                           uint32_t *parents, uint32_t *parentOffsets,
+                           uint32_t *parentRows, uint32_t *parentEntries);
+
+/* ======================================================================
+ * Operators: OperatorContext<ValueTypeOperator, ValueTypeOperand, DEVICE>::apply
+ * (src/linearAlgebra/OperatorContext.h:104-108).  apply(X, Y) overwrites Y
+ * and may modify X (constrained rows of X are set by parent->child), as the
+ * reference allows (OperatorContext.h:79-82).
+ * ==================================================================== */
+#define DFX_OP_KEEP_FP32 0x1u /* also keep an FP32 copy of the cell matrices */
+
+/* Cell-wise operator Y = C^T (sum_e Q_e^T A_e Q_e) C X: KohnShamOperatorContextFE
+ * (src/ksdft/KohnShamOperatorContextFE.h:102-127, apply .t.cpp:1326-1443),
+ * CFEOverlapOperatorContext (src/basis/CFEOverlapOperatorContext.t.cpp:539-640)
+ * and LaplaceOperatorContextFE (src/electrostatics/LaplaceOperatorContextFE.t.cpp:397-467).
+ * cellMatrices: sum_e n_e^2 doubles (host).  nClassical: dofs >= nClassical are
+ * enrichment dofs (shared by many cells; reduced deterministically). */
+int dfx_op_create_cellwise(dfx_ctx *ctx, uint32_t nCells,
+                           const uint32_t *cellNdofs, const uint32_t *cellDofMap,
+                           const double *cellMatrices, uint32_t nDofs,
+                           uint32_t nClassical, dfx_constraints *cons,
+                           uint32_t flags, dfx_op **op);
+/* replace the cell matrices (KohnShamOperatorContextFE::reinit, .t.cpp:1240-1310) */
+int dfx_op_update_cells(dfx_op *op, const double *cellMatrices);
+
+/* Diagonal (+ dense enriched block) operator
+ *   Y = C2P( diag o P2C(X) + [0 ; E X_enr] )
+ * as CFEOverlapInverseOpContextGLL::apply (src/basis/CFEOverlapInverseOpContextGLL.t.cpp:525-556)
+ * and OrthoEFEOverlapInverseOpContextGLL::apply
+ * (src/basis/OrthoEFEOverlapInverseOpContextGLL.t.cpp:1188-1273).  With
+ * cons == NULL it is PreconditionerJacobi::apply (src/linearAlgebra/PreconditionerJacobi.t.cpp:56-80).
+ * enrBlock: nEnriched x nEnriched column-major (may be NULL if nEnriched == 0);
+ * enriched dofs are the last nEnriched of nDofs. */
+int dfx_op_create_diagonal(dfx_ctx *ctx, uint32_t nDofs, const double *diag,
+                           uint32_t nEnriched, const double *enrBlock,
+                           dfx_constraints *cons, dfx_op **op);
+int dfx_op_apply(dfx_op *op, dfx_mv *X, dfx_mv *Y);
+int dfx_op_destroy(dfx_op *op);
+
+typedef struct dfx_op_info
+{
+  uint32_t nCells, nDofs, nClassical, nColors;
+  uint32_t maxCellDofs;
+  uint64_t sumNe;       /* sum_e n_e */
+  uint64_t sumNe2;      /* sum_e n_e^2 */
+  uint64_t matBytesF64; /* device bytes of the FP64 cell-matrix store */
+  uint64_t matBytesF32; /* device bytes of the FP32 cell-matrix store (0 if none) */
+} dfx_op_info;
+int dfx_op_get_info(const dfx_op *op, dfx_op_info *info);
+/* Download the device-resident cell->dof map in the caller's cell order
+ * (bit-exact round trip of what dfx_op_create_cellwise uploaded). */
+int dfx_op_export_map(const dfx_op *op, uint32_t *cellNdofs, uint32_t *cellDofMap);
+/* Stand-alone cell gather: Xcell[slot, :] = X[map[slot], :], Xcell is
+ * (sum_e n_e) x B  (FECellWiseDataOperations::copyFieldToCellWiseData,
+ * src/basis/FECellWiseDataOperations.t.cpp:31-84). */
+int dfx_op_gather(dfx_op *op, dfx_mv *X, dfx_mv *Xcell);
+/* Stand-alone cell scatter-add: Y[d, :] += sum of Ycell over the slots of d in
+ * cell order (addCellWiseDataToFieldData, FECellWiseDataOperations.t.cpp:89-151);
+ * bit-identical to the reference's sequential loop. */
+int dfx_op_scatter_add(dfx_op *op, dfx_mv *Ycell, dfx_mv *Y);
+
+/* ======================================================================
+ * Chebyshev-filtered subspace iteration.
+ * Replaces linearAlgebra::ChebyshevFilteredEigenSolver
+ * (src/linearAlgebra/ChebyshevFilteredEigenSolver.h:90-122, solve .t.cpp:194-437).
+ * ==================================================================== */
+#define DFX_PREC_FP64  0 /* everything FP64 */
+#define DFX_PREC_MIXED 1 /* filter bulk FP32 (FP32 MFMA, FP32 accumulate);
+                            residual seed, CholGS, RR, residual norms FP64 */
+
+typedef struct dfx_chfsi_params
+{
+  double   wantedSpectrumLowerBound;
+  double   wantedSpectrumUpperBound;
+  double   unWantedSpectrumUpperBound;
+  uint32_t polynomialDegree;
+  double   illConditionTolerance;
+  int      isResidualChebyshevFilter; /* reference default: 1 */
+  uint32_t eigenVectorBatchSize;      /* 0 -> all vectors in one block */
+  int      precision;                 /* DFX_PREC_* */
+  int      useGraph;                  /* capture the degree loop in a hipGraph */
+} dfx_chfsi_params;
+
+int dfx_chfsi_create(dfx_ctx *ctx, const dfx_chfsi_params *params,
+                     dfx_mv *eigenSubspaceGuess, dfx_chfsi **out);
+int dfx_chfsi_reinit(dfx_chfsi *s, double wantedLower, double wantedUpper,
+                     double unwantedUpper, uint32_t degree,
+                     double illConditionTolerance, dfx_mv *eigenSubspaceGuess);
+/* eigenValues: host, length nVectors (in: filter seeds for the residual
+ * filter; out: Ritz values).  eigenVectors: in/out device multivector. */
+int dfx_chfsi_solve(dfx_chfsi *s, dfx_op *A, double *eigenValues,
+                    dfx_mv *eigenVectors, int computeEigenVectors, dfx_op *B,
+                    dfx_op *BInv);
+/* wall times (ms) of the last solve: filter, orthonormalization, RR */
+int dfx_chfsi_timings(const dfx_chfsi *s, double *ms3);
+int dfx_chfsi_destroy(dfx_chfsi *s);
+
+/* Cholesky-Gram-Schmidt (OrthonormalizationFunctions.t.cpp:153-431):
+ * S = X^T B X, S = L L^T, Xo = X L^-T.  Returns OrthonormalizationErrorCode. */
+int dfx_cholesky_gram_schmidt(dfx_ctx *ctx, dfx_mv *X, dfx_mv *Xo, dfx_op *B);
+/* Rayleigh-Ritz (RayleighRitzEigenSolver.t.cpp:67-302): Hp = X^T A X,
+ * eig(Hp) = Q diag(w) Q^T, Xo = X Q.  Returns EigenSolverErrorCode. */
+int dfx_rayleigh_ritz(dfx_ctx *ctx, dfx_op *A, dfx_mv *X, double *eigenValues,
+                      dfx_mv *Xo);
+
+/* LanczosExtremeEigenSolver::solve, non-adaptive
+ * (src/linearAlgebra/LanczosExtremeEigenSolver.t.cpp:224-540). */
+int dfx_lanczos_extreme(dfx_ctx *ctx, dfx_op *A, dfx_op *B, dfx_op *BInv,
+                        dfx_mv *guess, uint32_t maxKrylov, double *lowest,
+                        double *highest, double *betaLast);
+
+/* KohnShamEigenSolver::solve (src/ksdft/KohnShamEigenSolver.t.cpp:214-536):
+ * Lanczos bounds, degree lookup, ChFSI passes, residual norms. */
+typedef struct dfx_ks_params
+{
+  uint32_t numWantedEigenvalues;
+  uint32_t maxChebyshevFilterPass;
+  uint32_t polynomialDegree;          /* 0 -> CHEBY_ORDER_LOOKUP */
+  double   eigenSolveResidualTolerance;
+  uint32_t waveFunctionBatchSize;     /* 0 -> all */
+  int      isResidualChebyshevFilter;
+  int      precision;                 /* DFX_PREC_* */
+  uint32_t lanczosMaxKrylov;          /* 0 -> 20 */
+} dfx_ks_params;
+typedef struct dfx_ks_result
+{
+  uint32_t passes;
+  double   lanczosLow, lanczosHigh, lanczosBeta;
+  double   a, b, c;                   /* last filter bounds */
+  uint32_t degree;
+  double   filterMs, orthoMs, rrMs, residualMs, lanczosMs;
+} dfx_ks_result;
+/* residualNorms: host, length numWantedEigenvalues (may be NULL) */
+int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,
+                      dfx_op *M, dfx_op *MInv, dfx_op *MLanczos,
+                      dfx_mv *waveFunctionGuess, dfx_mv *lanczosGuess,
+                      double *energies, dfx_mv *waveFunctions,
+                      double *residualNorms, dfx_ks_result *result);
+
+/* CGLinearSolver::solve with a Jacobi (or any) preconditioner operator
+ * (src/linearAlgebra/CGLinearSolver.t.cpp:68-313).  b, x: N x nRhs; x holds
+ * the initial guess and receives the per-column converged solution.
+ * Returns LinearSolverErrorCode. */
+int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *b, dfx_mv *x,
+                 double absTol, double relTol, double divTol, uint32_t maxIter,
+                 uint32_t *iterations, double *residualNorms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFX_H */

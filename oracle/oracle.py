@@ -48,6 +48,23 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+_CHEBY_ORDER_LOOKUP = [(10, 6), (50, 9), (100, 12), (200, 16), (300, 19), (500, 24),
+                       (750, 30), (1000, 39), (1500, 50), (2000, 53), (3000, 57),
+                       (4000, 62), (5000, 69), (9000, 77), (14000, 104), (20000, 119),
+                       (30000, 162), (50000, 300), (80000, 450), (100000, 550),
+                       (200000, 700), (500000, 1000)]
+
+
+def cheby_degree(upper):
+    """CHEBY_ORDER_LOOKUP (src/ksdft/Defaults.cpp:51-58) with the lower_bound
+    lookup on a size_type key (src/ksdft/KohnShamEigenSolver.t.cpp:36-46)."""
+    key = int(upper) & 0xFFFFFFFF
+    for k, v in _CHEBY_ORDER_LOOKUP:
+        if k >= key:
+            return v
+    return 1250
+
+
 def rng_uniform(seed, a, b, n):
     out = np.empty(n, dtype=np.float64)
     lib().orc_rng_uniform(C.c_uint64(seed), C.c_double(a), C.c_double(b),
@@ -296,6 +313,46 @@ class Problem:
                           C.c_double(rel_tol), C.c_double(div_tol),
                           C.c_uint32(max_iter), C.byref(it))
         return rc, x, it.value
+
+    def ks_eigensolve(self, X0, lanczos_guess, degree=0, tol=1e-6, max_pass=10,
+                      residual=True, batch=0, max_krylov=20):
+        """KohnShamEigenSolver::solve (src/ksdft/KohnShamEigenSolver.t.cpp:214-536)
+        composed from the oracle's pieces: Lanczos bounds and the upper-bound
+        heuristic (:264-297), degree lookup (:36-46), one degree-1 plain pass
+        before residual-filter passes (src/ksdft/KohnShamDFT.t.cpp:462-487),
+        then ChFSI passes until every residual norm is within tol (all states
+        counted as occupied)."""
+        n = X0.shape[1]
+        N = self.nDofs
+        lo, hi, beta = self.lanczos(lanczos_guess, max_krylov)
+        r = beta / 10.0
+        a = lo
+        b = (hi + r - lo) * ((n * 200.0) / N) + lo
+        if b >= hi + r:
+            b = (hi + r + lo) * 0.5
+        c = hi + r
+        m = degree or cheby_degree(c)
+        eig = np.zeros(n)
+        G = np.array(X0, dtype=np.float64, order="C")
+        if residual:
+            rc, eig, X, G = self.chfsi(a, b, c, 1, G, eig, residual=False, batch=batch)
+            G = X.copy()
+        passes, res, rc = 0, None, 6
+        for it in range(max_pass):
+            rc, eig, X, G = self.chfsi(a, b, c, m, G, eig, residual=residual, batch=batch)
+            res = self.residual_norms(X, eig)
+            G = X.copy()
+            passes = it + 1
+            if rc != 0:
+                rc = 7
+                break
+            if np.all(res <= tol):
+                rc = 0
+                break
+            a, b = eig[0], eig[-1]
+            rc = 6
+        return dict(rc=rc, eig=eig, X=X, passes=passes, res=res,
+                    bounds=(lo, hi, beta), a=a, b=b, c=c, degree=m)
 
     def time_apply(self, kind, X, reps=3, threads=1):
         X = np.array(X, dtype=np.float64, order="C")

@@ -1,0 +1,303 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on identical
+seeded inputs.  Integer / index work must be bit-exact; FP64 within the
+tolerances written below; FP32 against its own error model."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import dfx
+import oracle as O
+from gpu_helpers import Ops
+from problems import c1_like, enriched, guess, hanging
+
+pytestmark = pytest.mark.gpu
+
+# FP64 tolerances (relative to the largest entry of the reference result)
+TOL_APPLY64 = 1e-13
+TOL_FILTER64 = 1e-10
+TOL_EIG64 = 1e-10          # Ha, FP64 eigenvalues vs the oracle
+TOL_ANGLE64 = 1e-8         # principal angle of the eigenvector subspaces
+TOL_APPLY32 = 2e-6
+
+
+@pytest.fixture(scope="module")
+def probs():
+    return {"c1": c1_like(n=4, p=3), "hang3": hanging(p=3), "hang4": hanging(p=4, n=3),
+            "enr": enriched(p=2, n=4)}
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def mv(ctx, a, dtype=dfx.F64):
+    return dfx.MultiVector.from_numpy(ctx, a, dtype)
+
+
+# ------------------------------------------------------------ bit-exact
+@pytest.mark.parametrize("name", ["hang3", "hang4", "c1"])
+@pytest.mark.parametrize("B", [1, 7, 64])
+def test_constraint_distribution_bit_exact(ctx, probs, name, B):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    X = O.rng_uniform(17 + B, -1, 1, P.nDofs * B).reshape(P.nDofs, B)
+    Xg = mv(ctx, X)
+    ops.cons.distribute_parent_to_child(Xg)
+    assert np.array_equal(Xg.to_numpy(), P.p2c(X.copy()))
+    Yg = mv(ctx, X)
+    ops.cons.distribute_child_to_parent(Yg)
+    assert np.array_equal(Yg.to_numpy(), P.c2p(X.copy()))
+
+
+@pytest.mark.parametrize("name", ["hang3", "hang4", "enr"])
+def test_index_sets_round_trip_bit_exact(ctx, probs, name):
+    P = probs[name]
+    d = P.arrays()
+    ops = Ops(ctx, P)
+    n, m = ops.H.export_map()
+    assert np.array_equal(n, d["cellNdofs"]) and np.array_equal(m, d["cellMap"])
+    e = ops.cons.export()
+    assert np.array_equal(e["rows"], d["rows"])
+    assert np.array_equal(e["cols"], d["cols"])
+    assert np.array_equal(np.diff(e["rowOffsets"].astype(np.int64)), d["rowSizes"])
+    # the parent-major transpose lists every entry exactly once, parents
+    # ascending, children in row order
+    pe = e["parentEntries"]
+    assert np.array_equal(np.sort(pe), np.arange(len(d["cols"])))
+    assert np.array_equal(e["parents"], np.unique(d["cols"]))
+    for q in range(len(e["parents"])):
+        s, t = e["parentOffsets"][q], e["parentOffsets"][q + 1]
+        assert np.all(d["cols"][pe[s:t]] == e["parents"][q])
+        assert np.all(np.diff(e["parentRows"][s:t].astype(np.int64)) >= 0)
+
+
+@pytest.mark.parametrize("name", ["c1", "enr", "hang3"])
+@pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
+def test_gather_and_scatter_add_bit_exact(ctx, probs, name, dtype):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    B = 5
+    npd = np.float64 if dtype == dfx.F64 else np.float32
+    X = O.rng_uniform(3, -1, 1, P.nDofs * B).reshape(P.nDofs, B).astype(npd)
+    Xc = dfx.MultiVector(ctx, P.mapLen, B, dtype)
+    ops.H.gather(mv(ctx, X, dtype), Xc)
+    want = X[P.arrays()["cellMap"]]
+    assert np.array_equal(Xc.to_numpy(), want)
+    if dtype == dfx.F64:
+        Yc = O.rng_uniform(4, -1, 1, P.mapLen * B).reshape(P.mapLen, B)
+        Y0 = O.rng_uniform(5, -1, 1, P.nDofs * B).reshape(P.nDofs, B)
+        Yg = mv(ctx, Y0)
+        ops.H.scatter_add(mv(ctx, Yc), Yg)
+        ref = Y0.copy()
+        O.lib().orc_scatter_add(P.h, Yc.ctypes.data_as(O.C.c_void_p), O.C.c_uint32(B),
+                                ref.ctypes.data_as(O.C.c_void_p))
+        assert np.array_equal(Yg.to_numpy(), ref)
+
+
+# ------------------------------------------------------------- FP apply
+@pytest.mark.parametrize("name", ["c1", "hang3", "hang4", "enr"])
+@pytest.mark.parametrize("B", [1, 10, 64, 70])
+def test_cell_apply_fp64(ctx, probs, name, B):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    X = guess(P, B, seed=B)
+    for kind, op in ((O.OP_H, ops.H), (O.OP_M, ops.M), (O.OP_K, ops.K)):
+        Xg, Yg = mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, B)
+        op.apply(Xg, Yg)
+        Xr = X.copy()
+        ref = P.apply(kind, Xr)
+        assert rel(Yg.to_numpy(), ref) < TOL_APPLY64
+        assert np.array_equal(Xg.to_numpy(), Xr)  # the same X mutation (P2C)
+
+
+@pytest.mark.parametrize("name", ["c1", "hang3", "enr"])
+def test_cell_apply_fp32(ctx, probs, name):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    B = 24
+    X = guess(P, B, seed=9)
+    Xg = mv(ctx, X.astype(np.float32), dfx.F32)
+    Yg = dfx.MultiVector(ctx, P.nDofs, B, dfx.F32)
+    ops.H.apply(Xg, Yg)
+    ref = P.apply(O.OP_H, X.astype(np.float32).astype(np.float64))
+    assert rel(Yg.to_numpy().astype(np.float64), ref) < TOL_APPLY32
+
+
+def test_cell_apply_deterministic(ctx, probs):
+    P = probs["enr"]
+    ops = Ops(ctx, P)
+    X = guess(P, 40, seed=2)
+    outs = []
+    for _ in range(3):
+        Xg, Yg = mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, 40)
+        ops.H.apply(Xg, Yg)
+        outs.append(Yg.to_numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("name", ["hang3", "enr"])
+def test_overlap_inverse_and_lumped_mass(ctx, probs, name):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    X = guess(P, 6, seed=12)
+    for kind, op in ((O.OP_MINV, ops.Minv), (O.OP_MLUMP, ops.Mlump)):
+        Xg, Yg = mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, 6)
+        op.apply(Xg, Yg)
+        ref = P.apply(kind, X.copy())
+        assert rel(Yg.to_numpy(), ref) < 1e-14
+
+
+# ----------------------------------------------------------- filters
+@pytest.mark.parametrize("name", ["c1", "enr"])
+@pytest.mark.parametrize("residual", [False, True])
+def test_chebyshev_filter_fp64(ctx, probs, name, residual):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    nv, m = 8, 12
+    X = guess(P, nv, seed=21)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    lam = np.linspace(a, b, nv)
+    # one ChFSI pass vs the oracle's (filter + CholGS + RR)
+    G = mv(ctx, X)
+    V = dfx.MultiVector(ctx, P.nDofs, nv)
+    s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, m, 1e-14, G,
+                                         isResidualChebyshevFilter=residual)
+    eig = lam.copy()
+    rc = s.solve(ops.H, eig, V, True, ops.M, ops.Minv)
+    assert rc == 0
+    rco, eigo, Xo, Go = P.chfsi(a, b, c, m, X, lam, residual=residual)
+    assert rco == 0
+    assert np.abs(eig - eigo).max() < TOL_EIG64 * max(1.0, np.abs(eigo).max())
+    # eigenvector subspace agreement
+    ang = sl.subspace_angles(V.to_numpy(), Xo)
+    assert ang.max() < TOL_ANGLE64
+
+
+def test_chebyshev_filter_blocks_equal_fused(ctx, probs):
+    """Filtering block by block (eigenVectorBatchSize) gives the same bits as
+    filtering all columns at once: columns are independent in every kernel."""
+    P = probs["c1"]
+    ops = Ops(ctx, P)
+    nv = 20
+    X = guess(P, nv, seed=5)
+    res = []
+    for bs in (0, 8):
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, -0.6, 0.5, 40.0, 8, 1e-14, G,
+                                             eigenVectorBatchSize=bs)
+        eig = np.linspace(-0.5, 0.2, nv)
+        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        res.append((eig.copy(), V.to_numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_cholgs_and_rr(ctx, probs):
+    P = probs["hang3"]
+    ops = Ops(ctx, P)
+    X = guess(P, 12, seed=8)
+    Xg, Xo = mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, 12)
+    assert dfx.cholesky_gram_schmidt(ctx, Xg, Xo, ops.M) == 0
+    rc, Xor = P.cholgs(X.copy())
+    assert rc == 0 and rel(Xo.to_numpy(), Xor) < 1e-11
+    Xr = dfx.MultiVector(ctx, P.nDofs, 12)
+    rc, eig = dfx.rayleigh_ritz(ctx, ops.H, Xo, Xr)
+    rco, eigo, Xro = P.rayleigh_ritz(Xor)
+    assert rc == 0 and np.abs(eig - eigo).max() < 1e-11
+    assert sl.subspace_angles(Xr.to_numpy(), Xro).max() < 1e-9
+
+
+def test_cholgs_rank_deficient_reports_error(ctx, probs):
+    P = probs["c1"]
+    ops = Ops(ctx, P)
+    X = guess(P, 4, seed=1)
+    X[:, 3] = 0.0  # exactly singular overlap: the Cholesky pivot is exactly zero
+    rc = dfx.cholesky_gram_schmidt(ctx, mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, 4), ops.M)
+    assert rc == 1  # OrthonormalizationErrorCode::LAPACK_ERROR
+    assert P.cholgs(X)[0] == 1
+
+
+def test_lanczos_bounds(ctx, probs):
+    P = probs["c1"]
+    ops = Ops(ctx, P)
+    g = O.rng_uniform(2, -1, 1, P.nDofs)
+    lo, hi, beta = dfx.lanczos_extreme(ctx, ops.H, ops.Mlump, ops.Minv,
+                                       mv(ctx, g.reshape(-1, 1)), 20)
+    lo_o, hi_o, beta_o = P.lanczos(g, 20)
+    assert abs(lo - lo_o) < 1e-9 * abs(hi_o) and abs(hi - hi_o) < 1e-9 * abs(hi_o)
+    assert abs(beta - beta_o) < 1e-7 * abs(beta_o)
+
+
+def test_cg_poisson(ctx, probs):
+    P = probs["hang3"]
+    ops = Ops(ctx, P)
+    d = ops.d
+    # Jacobi diagonal: assembled K diagonal, C2P, constrained rows 1
+    diag = np.zeros(P.nDofs)
+    off = moff = 0
+    for n in d["cellNdofs"]:
+        ids = d["cellMap"][off:off + n]
+        diag[ids] += np.diag(d["K"][moff:moff + n * n].reshape(n, n, order="F"))
+        off += n; moff += n * n
+    diag = P.c2p(diag.reshape(-1, 1))[:, 0]
+    diag[d["rows"]] = 1.0
+    b = P.c2p(guess(P, 4, seed=6))
+    pc = dfx.DiagonalOperator(ctx, 1.0 / diag)
+    x = dfx.MultiVector(ctx, P.nDofs, 4)
+    cg = dfx.CGLinearSolver(ctx, 2000, 1e-12, 1e-10, 1e10)
+    rc = cg.solve(ops.K, pc, mv(ctx, b), x)
+    rco, xo, ito = P.cg(O.OP_K, 1.0 / diag, b, abs_tol=1e-12, rel_tol=1e-10)
+    assert rc == 0 and rco == 0
+    assert abs(cg.iterations - ito) <= 1
+    assert rel(x.to_numpy(), xo) < 1e-8
+
+
+# ----------------------------------------------------- full KS solve
+@pytest.mark.parametrize("name", ["c1", "enr"])
+def test_ks_eigensolve_fp64_matches_oracle(ctx, probs, name):
+    P = probs[name]
+    ops = Ops(ctx, P)
+    # Converged solves are compared: before convergence the first pass may
+    # amplify a deep state lying below the Lanczos interval, and the
+    # ill-conditioned CholGS that follows makes unconverged iterates depend on
+    # rounding on either side.
+    n, tol = 6, 1e-6
+    X0 = guess(P, n, seed=1)
+    lg = P.p2c(O.rng_uniform(77, -1, 1, P.nDofs).reshape(-1, 1))
+    ref = P.ks_eigensolve(X0, lg[:, 0], degree=20, tol=tol, max_pass=60)
+    assert ref["rc"] == 0
+    G, L = mv(ctx, X0), mv(ctx, lg)
+    W = dfx.MultiVector(ctx, P.nDofs, n)
+    ks = dfx.KohnShamEigenSolver(ctx, n, tol, 60, G, L, polynomialDegree=20)
+    e = np.zeros(n)
+    rc = ks.solve(ops.H, e, W, ops.M, ops.Minv, ops.Mlump)
+    assert rc == 0
+    assert abs(ks.result["passes"] - ref["passes"]) <= 1
+    assert abs(ks.result["lanczosHigh"] - ref["bounds"][1]) < 1e-9 * abs(ref["bounds"][1])
+    assert np.abs(e - ref["eig"]).max() < TOL_EIG64
+    Wn = W.to_numpy()
+    for j in range(n):  # each converged eigenvector, up to sign
+        cosang = abs(Wn[:, j] @ ref["X"][:, j]) / (np.linalg.norm(Wn[:, j]) *
+                                                  np.linalg.norm(ref["X"][:, j]))
+        assert np.sqrt(max(0.0, 1 - cosang ** 2)) < 1e-4
+
+
+def test_mixed_precision_matches_fp64_eigenvalues(ctx, probs):
+    P = probs["c1"]
+    ops = Ops(ctx, P)
+    n = 6
+    X0 = guess(P, n, seed=1)
+    lg = P.p2c(O.rng_uniform(77, -1, 1, P.nDofs).reshape(-1, 1))
+    out = {}
+    for prec in (dfx.PREC_FP64, dfx.PREC_MIXED):
+        G, L = mv(ctx, X0), mv(ctx, lg)
+        W = dfx.MultiVector(ctx, P.nDofs, n)
+        ks = dfx.KohnShamEigenSolver(ctx, n, 1e-6, 60, G, L, polynomialDegree=20,
+                                     precision=prec)
+        e = np.zeros(n)
+        rc = ks.solve(ops.H, e, W, ops.M, ops.Minv, ops.Mlump)
+        assert rc == 0
+        out[prec] = e
+    assert np.abs(out[dfx.PREC_MIXED] - out[dfx.PREC_FP64]).max() < 1e-6
