@@ -50,7 +50,28 @@ namespace dfx
     const auto &L = sizeof(T) == 8 ? op->launches64 : op->launches32;
     View<const T> Xc{X.p, X.N, X.B, X.ld};
     for (const Launch &l : L)
-      k::cellApply<T>(s, op, l, frag, Xc, Y, yE);
+      {
+        ProfRec    r{};
+        const bool prof = op->profiling && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
+        if (prof)
+          {
+            DFX_HIP(hipEventCreate(&r.a));
+            DFX_HIP(hipEventCreate(&r.b));
+            DFX_HIP(hipEventRecord(r.a, s));
+          }
+        k::cellApply<T>(s, op, l, frag, Xc, Y, yE);
+        if (prof)
+          {
+            DFX_HIP(hipEventRecord(r.b, s));
+            const double B = X.B, sz = sizeof(T);
+            r.flops = 2.0 * B * (double)l.sumN2;
+            // matrices once, map, gathered X, RMW of classical rows of Y,
+            // stored enriched partials
+            r.bytes = sz * (double)l.sumN2 + 4.0 * (double)l.sumN + sz * B * (double)l.sumN +
+                      2.0 * sz * B * (double)l.sumNc + sz * B * (double)(l.sumN - l.sumNc);
+            op->prof.push_back(r);
+          }
+      }
     if (op->nEnrSlots)
       k::enrichedReduce<T>(s, op, yE, Y);
     k::c2p<T>(s, op->cons, Y);
@@ -96,6 +117,15 @@ namespace dfx
     if (ctx->stream)
       (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+  }
+
+  void consRelease(dfx_constraints *c)
+  {
+    if (!c || c->refs.fetch_sub(1) != 1)
+      return;
+    dfx_ctx *ctx = c->ctx;
+    delete c;
+    ctxRelease(ctx);
   }
 
   template void applyOp<double>(dfx_op *, View<double>, View<double>);
@@ -273,6 +303,19 @@ extern "C"
         k::fill<double>(m->ctx->stream, view<double>(m), v);
       else
         k::fill<float>(m->ctx->stream, view<float>(m), v);
+      return 0;
+    });
+  }
+
+  int dfx_mv_fill_random(dfx_mv *m, uint64_t seed, double a, double b)
+  {
+    if (!m)
+      return DFX_E_ARG;
+    return guard(m->ctx, [&] {
+      if (m->dtype == DFX_F64)
+        k::fillRandom<double>(m->ctx->stream, view<double>(m), seed, a, b);
+      else
+        k::fillRandom<float>(m->ctx->stream, view<float>(m), seed, a, b);
       return 0;
     });
   }
@@ -459,9 +502,7 @@ extern "C"
   {
     if (!c)
       return DFX_E_ARG;
-    dfx_ctx *ctx = c->ctx;
-    delete c;
-    ctxRelease(ctx);
+    consRelease(c);
     return 0;
   }
 
@@ -594,6 +635,7 @@ extern "C"
       op->nEnriched  = nEnriched;
       op->nClassical = nDofs - nEnriched;
       op->cons       = cons;
+      consRetain(cons);
       upload(op->diag, diag, nDofs, ctx->stream);
       if (nEnriched)
         upload(op->enrBlock, enrBlock, (size_t)nEnriched * nEnriched, ctx->stream);
@@ -621,10 +663,12 @@ extern "C"
   {
     if (!op)
       return DFX_E_ARG;
-    dfx_ctx *ctx = op->ctx;
+    dfx_ctx         *ctx  = op->ctx;
+    dfx_constraints *cons = op->cons;
     if (ctx && ctx->stream)
       (void)hipStreamSynchronize(ctx->stream);
     delete op;
+    consRelease(cons);
     ctxRelease(ctx);
     return 0;
   }
@@ -643,6 +687,46 @@ extern "C"
     info->matBytesF64 = op->fragLen64 * 8;
     info->matBytesF32 = op->hasF32 ? op->fragLen32 * 4 : 0;
     return 0;
+  }
+
+  int dfx_op_set_profiling(dfx_op *op, int enable)
+  {
+    if (!op)
+      return DFX_E_ARG;
+    op->profiling = enable != 0;
+    op->profMask  = enable & 3 ? (enable & 3) : 3;
+    return 0;
+  }
+
+  int dfx_op_get_profile(dfx_op *op, dfx_op_profile *out, int reset)
+  {
+    if (!op || !out)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      for (auto &r : op->prof)
+        {
+          DFX_HIP(hipEventSynchronize(r.b));
+          float ms = 0.f;
+          DFX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+          op->profMs += ms;
+          op->profFlops += r.flops;
+          op->profBytes += r.bytes;
+          ++op->profLaunches;
+          (void)hipEventDestroy(r.a);
+          (void)hipEventDestroy(r.b);
+        }
+      op->prof.clear();
+      out->launches = op->profLaunches;
+      out->kernelMs = op->profMs;
+      out->flops    = op->profFlops;
+      out->bytes    = op->profBytes;
+      if (reset)
+        {
+          op->profLaunches = 0;
+          op->profMs = op->profFlops = op->profBytes = 0;
+        }
+      return 0;
+    });
   }
 
   int dfx_op_export_map(const dfx_op *op, uint32_t *cellNdofs, uint32_t *cellDofMap)

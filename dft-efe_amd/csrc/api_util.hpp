@@ -54,6 +54,12 @@ namespace dfx
       c->refs.fetch_add(1);
   }
   void ctxRelease(dfx_ctx *c);
+  inline void consRetain(dfx_constraints *c)
+  {
+    if (c)
+      c->refs.fetch_add(1);
+  }
+  void consRelease(dfx_constraints *c);
 
   template <class T>
   View<T> view(dfx_mv *m);

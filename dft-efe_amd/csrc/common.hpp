@@ -160,6 +160,7 @@ struct dfx_mv
 
 struct dfx_constraints
 {
+  std::atomic<int> refs{1}; // creator + every operator built on it
   dfx_ctx *ctx = nullptr;
   dfx::u32 nDofs = 0, nRows = 0, nEntries = 0, nParents = 0;
   std::vector<dfx::u32> hRows, hRowOffsets, hCols, hParents, hParentOffsets,
@@ -196,6 +197,16 @@ namespace dfx
     u32 begin, end;
     u32 mtw;    // M tiles per wave (n <= 64 * mtw)
     u32 kpadMax;
+    u64 sumN2;  // sum of n_e^2 over the launch's cells
+    u64 sumN;   // sum of n_e
+    u64 sumNc;  // sum of classical rows
+  };
+
+  // one profiled kernel launch
+  struct ProfRec
+  {
+    hipEvent_t a, b;
+    double     flops, bytes;
   };
 } // namespace dfx
 
@@ -225,4 +236,18 @@ struct dfx_op
   // ----- diagonal
   dfx::DevBuf diag, diag32, enrBlock, enrFrag64, enrFrag32;
   dfx::u32 nEnriched = 0;
+  // ----- profiling of the cell-apply launches
+  bool profiling = false;
+  int  profMask  = 3; // 1: FP64 launches, 2: FP32 launches
+  std::vector<dfx::ProfRec> prof;
+  dfx::u64 profLaunches = 0;
+  double profMs = 0, profFlops = 0, profBytes = 0;
+  ~dfx_op()
+  {
+    for (auto &r : prof)
+      {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+      }
+  }
 };

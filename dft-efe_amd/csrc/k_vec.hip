@@ -45,6 +45,25 @@ namespace dfx
           }
       }
 
+      // SplitMix64 (Steele, Lea, Flood 2014): output t is the mix of
+      // seed + (t + 1) * golden gamma, so every element is independent
+      template <class T>
+      __global__ void fillRandomK(T *x, u32 N, u32 B, u32 ld, u64 seed, double a, double b)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            u64 z = seed + (t + 1) * 0x9E3779B97F4A7C15ULL;
+            z     = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z     = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            z     = z ^ (z >> 31);
+            const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+            RC r = rc(t, B);
+            x[(u64)r.i * ld + r.j] = (T)(a + (b - a) * u);
+          }
+      }
+
       template <class To, class Ti>
       __global__ void convertK(To *y, u32 ldy, const Ti *x, u32 ldx, u32 N, u32 B)
       {
@@ -297,6 +316,16 @@ namespace dfx
       checkLaunch("fill");
     }
 
+    template <class T>
+    void fillRandom(hipStream_t s, View<T> x, u64 seed, double a, double b)
+    {
+      const u64 n = (u64)x.N * x.B;
+      if (!n)
+        return;
+      fillRandomK<T><<<gridFor(n), TPB, 0, s>>>(x.p, x.N, x.B, x.ld, seed, a, b);
+      checkLaunch("fillRandom");
+    }
+
     template <class To, class Ti>
     void convert(hipStream_t s, View<To> d, View<const Ti> x)
     {
@@ -473,6 +502,7 @@ namespace dfx
 
 #define DFX_INST_T(T)                                                              \
   template void fill<T>(hipStream_t, View<T>, double);                             \
+  template void fillRandom<T>(hipStream_t, View<T>, u64, double, double);          \
   template void axpby<T>(hipStream_t, View<T>, double, View<const T>, double,      \
                          View<const T>);                                           \
   template void axpbyBlocked<T>(hipStream_t, View<T>, double, const double *,      \

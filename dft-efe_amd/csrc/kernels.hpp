@@ -10,6 +10,9 @@ namespace dfx
     // ---------------------------------------------------------- vectors
     template <class T>
     void fill(hipStream_t s, View<T> x, double v);
+    // x[t] = a + (b - a) * u53(splitmix64_t(seed)),  t = i * B + j
+    template <class T>
+    void fillRandom(hipStream_t s, View<T> x, u64 seed, double a, double b);
     template <class To, class Ti>
     void convert(hipStream_t s, View<To> dst, View<const Ti> src);
     // z = a x + b y   (z may alias x or y)

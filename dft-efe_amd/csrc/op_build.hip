@@ -56,6 +56,7 @@ namespace dfx
     op->nDofs      = nDofs;
     op->nClassical = nClassical;
     op->cons       = cons;
+    consRetain(cons);
     op->nCells     = nCells;
     op->hasF32     = (flags & DFX_OP_KEEP_FP32) != 0;
     op->hCellNdofs.assign(cellNdofs, cellNdofs + nCells);
@@ -182,16 +183,20 @@ namespace dfx
         const u32 e0 = op->hOrder[q];
         u32       r  = q;
         u32 kp64 = 0, kp32 = 0;
+        u64 s2 = 0, s1 = 0, sc = 0;
         while (r < nCells && color[op->hOrder[r]] == color[e0] &&
                mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0]))
           {
             const u32 n = cellNdofs[op->hOrder[r]];
             kp64        = std::max(kp64, 8 * ((n + 7) / 8));
             kp32        = std::max(kp32, 16 * ((n + 15) / 16));
+            s2 += (u64)n * n;
+            s1 += n;
+            sc += nc[op->hOrder[r]];
             ++r;
           }
-        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64});
-        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32});
+        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64, s2, s1, sc});
+        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32, s2, s1, sc});
         q = r;
       }
 

@@ -1,0 +1,960 @@
+// Problem builder of libdfx: the structured-mesh FE setup that feeds the hot
+// path (maps, Dirichlet constraints, cell matrices), built on the GPU.
+//
+// Host: lattice-indexed node numbering (first appearance over cells, cell-local
+// tensor-lexicographic order), Dirichlet rows, enrichment supports.
+// GPU:  basis values / gradients at Gauss points (classical tensor products and
+// analytic Slater x real Y_lm x smooth-cutoff enrichment), then the cell
+// matrices as batched GEMMs  M_e = F^T W F,  K_e = G^T W G,  H_e = K_e / 2 +
+// F^T (W V) F  (FEBasisOperations::computeFEMatrices, src/basis/
+// FEBasisOperations.t.cpp:41-433; KineticFE.t.cpp:90-100;
+// ElectrostaticLocalFE.t.cpp:1625-1660).
+// Enrichment orthogonalization (the OrthoEFE construction,
+// src/basis/EnrichmentClassicalInterfaceSpherical.h:76-97): per atom, the
+// classical L2 projection of its enrichment functions on the nodes interior
+// to the enrichment support, M_II alpha = d, solved with this library's own
+// cell-wise mass operator and PCG; the enriched basis is phi - sum alpha N.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "api_util.hpp"
+#include "kernels.hpp"
+
+using namespace dfx;
+
+namespace
+{
+  constexpr int NFUN = 5; // 1s, 2s, 2p_y, 2p_z, 2p_x
+
+  // ----------------------------------------------------------- quadrature
+  void legendre(int n, double x, double &p, double &dp)
+  {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k)
+      {
+        const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0              = p1;
+        p1              = p2;
+      }
+    p  = n == 0 ? 1.0 : p1;
+    dp = n == 0 ? 0.0 : n * (x * p1 - p0) / (x * x - 1.0);
+  }
+
+  void gaussRule(int n, std::vector<double> &x, std::vector<double> &w)
+  {
+    x.resize(n);
+    w.resize(n);
+    for (int i = 0; i < n; ++i)
+      {
+        double z = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 200; ++it)
+          {
+            double p, dp;
+            legendre(n, z, p, dp);
+            const double dz = p / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16)
+              break;
+          }
+        double p, dp;
+        legendre(n, z, p, dp);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+      }
+  }
+
+  // GLL points: +-1 and the roots of P'_{n-1} (Newton on (1 - x^2) P'_{n-1})
+  void gllRule(int n, std::vector<double> &x, std::vector<double> &w)
+  {
+    const int N = n - 1;
+    x.resize(n);
+    w.resize(n);
+    for (int i = 0; i < n; ++i)
+      {
+        double z = -std::cos(M_PI * i / N);
+        if (i > 0 && i < N)
+          for (int it = 0; it < 200; ++it)
+            {
+              double p, dp, pm, dpm;
+              legendre(N, z, p, dp);
+              legendre(N - 1, z, pm, dpm);
+              // (1-z^2) P_N' = N (P_{N-1} - z P_N); derivative -N(N+1) P_N
+              const double f = N * (pm - z * p), fp = -N * (N + 1.0) * p;
+              const double dz = f / fp;
+              z -= dz;
+              if (std::fabs(dz) < 1e-16)
+                break;
+            }
+        x[i] = (i == 0) ? -1.0 : (i == N ? 1.0 : z);
+      }
+    if (n % 2 == 1)
+      x[N / 2] = 0.0;
+    for (int i = 0; i < n; ++i)
+      {
+        double p, dp;
+        legendre(N, x[i], p, dp);
+        w[i] = 2.0 / (N * (N + 1.0) * p * p);
+      }
+  }
+
+  double lag(const std::vector<double> &nd, int k, double x)
+  {
+    double v = 1.0;
+    for (int m = 0; m < (int)nd.size(); ++m)
+      if (m != k)
+        v *= (x - nd[m]) / (nd[k] - nd[m]);
+    return v;
+  }
+  double lagD(const std::vector<double> &nd, int k, double x)
+  {
+    double s = 0.0;
+    for (int j = 0; j < (int)nd.size(); ++j)
+      {
+        if (j == k)
+          continue;
+        double t = 1.0 / (nd[k] - nd[j]);
+        for (int m = 0; m < (int)nd.size(); ++m)
+          if (m != k && m != j)
+            t *= (x - nd[m]) / (nd[k] - nd[m]);
+        s += t;
+      }
+    return s;
+  }
+
+  // ----------------------------------------------------------- device
+  struct TabCell
+  {
+    double lo[3], h[3];
+    u32    n;       // functions of the cell
+    u32    nq;      // Gauss points per direction
+    u32    tabOff;  // offset of this rule's 1-D tables
+    u32    mapOff;  // into the (device) cell map
+    u64    bufOff;  // into the tabulation buffers (elements)
+  };
+
+  struct AtomDev
+  {
+    double r[3];
+    int    Z;
+    int    pad;
+  };
+
+  __device__ double cutoffDev(double r, double rc, double d, double &dv)
+  {
+    auto f1 = [](double x) { return x <= 0.0 ? 0.0 : exp(-1.0 / x); };
+    auto f1d = [&](double x) { return x <= 0.0 ? 0.0 : f1(x) / (x * x); };
+    const double y = 1.0 - d * (r - rc) / rc;
+    const double a = f1(y), b = f1(1.0 - y);
+    const double v = a / (a + b);
+    const double s = a + b;
+    dv = ((f1d(y) * b + a * f1d(1.0 - y)) / (s * s)) * (-d / rc);
+    return v;
+  }
+
+  // value and gradient of enrichment function k of atom A at x
+  __device__ void enrichDev(const AtomDev &A, int k, double rc, double sm,
+                            const double *x, double &val, double *g)
+  {
+    const double dx[3] = {x[0] - A.r[0], x[1] - A.r[1], x[2] - A.r[2]};
+    const double r     = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    const double Z     = A.Z;
+    double       dfc;
+    const double fc  = cutoffDev(r, rc, sm, dfc);
+    const double ez  = exp(-Z * r);
+    const double y00 = 0.28209479177387814;  // 1 / (2 sqrt(pi))
+    const double c1  = 0.48860251190291992;  // sqrt(3 / (4 pi))
+    if (k < 2)
+      {
+        const double gr  = k == 0 ? ez : r * ez;
+        const double dgr = k == 0 ? -Z * ez : ez - Z * r * ez;
+        val              = y00 * gr * fc;
+        const double dr  = y00 * (dgr * fc + gr * dfc);
+        for (int d = 0; d < 3; ++d)
+          g[d] = r > 1e-14 ? dr * dx[d] / r : 0.0;
+        return;
+      }
+    const int    comp = k == 2 ? 1 : (k == 3 ? 2 : 0);
+    const double hh = ez * fc, dh = -Z * ez * fc + ez * dfc;
+    val             = c1 * dx[comp] * hh;
+    for (int d = 0; d < 3; ++d)
+      {
+        double gg = r > 1e-14 ? dx[comp] * dh * dx[d] / r : 0.0;
+        if (d == comp)
+          gg += hh;
+        g[d] = c1 * gg;
+      }
+  }
+
+  // One block per cell; threads over Gauss points.  Writes, column-major in
+  // (point, function):  F, FW = W F, FV = W V F  (nQ x n) and G, GW (3 nQ x n).
+  __global__ void tabulateK(const TabCell *cells, const u32 *map, u32 nClassical,
+                            const double *tabL, const double *tabD, const double *tabX,
+                            const double *tabW, int np, const AtomDev *atoms, u32 nAtoms,
+                            double smear, double rc, double sm, const double *alpha,
+                            double *F, double *FW, double *FV, double *G, double *GW)
+  {
+    const TabCell C  = cells[blockIdx.x];
+    const u32     nq = C.nq, nQ = nq * nq * nq, n = C.n;
+    const int     ncl = np * np * np;
+    const double  detJ = C.h[0] * C.h[1] * C.h[2] / 8.0;
+    const double *L = tabL + C.tabOff, *D = tabD + C.tabOff;
+    const double *X = tabX + (C.tabOff / np), *Wt = tabW + (C.tabOff / np);
+    const u32    *cmap = map + C.mapOff;
+    for (u32 q = threadIdx.x; q < nQ; q += blockDim.x)
+      {
+        const u32 qa = q % nq, qb = (q / nq) % nq, qc = q / (nq * nq);
+        const double x[3] = {C.lo[0] + C.h[0] * (X[qa] + 1.0) * 0.5,
+                             C.lo[1] + C.h[1] * (X[qb] + 1.0) * 0.5,
+                             C.lo[2] + C.h[2] * (X[qc] + 1.0) * 0.5};
+        const double w = ((Wt[qa] * Wt[qb]) * Wt[qc]) * detJ;
+        double v = 0.0;
+        for (u32 a = 0; a < nAtoms; ++a)
+          {
+            const double dx = x[0] - atoms[a].r[0], dy = x[1] - atoms[a].r[1],
+                         dz = x[2] - atoms[a].r[2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            v += r < 1e-12 ? -atoms[a].Z * 2.0 / (smear * 1.7724538509055160)
+                           : -atoms[a].Z * erf(r / smear) / r;
+          }
+        auto put = [&](u32 j, double f, double gx, double gy, double gz) {
+          const u64 o = C.bufOff;
+          F[o + q + (u64)nQ * j]  = f;
+          FW[o + q + (u64)nQ * j] = w * f;
+          FV[o + q + (u64)nQ * j] = (w * v) * f;
+          const u64 og = 3 * C.bufOff + (u64)3 * nQ * j;
+          G[og + q]            = gx;
+          G[og + nQ + q]       = gy;
+          G[og + 2 * nQ + q]   = gz;
+          GW[og + q]           = w * gx;
+          GW[og + nQ + q]      = w * gy;
+          GW[og + 2 * nQ + q]  = w * gz;
+        };
+        const double sx = 2.0 / C.h[0], sy = 2.0 / C.h[1], sz = 2.0 / C.h[2];
+        for (int c = 0; c < np; ++c)
+          for (int b = 0; b < np; ++b)
+            for (int a = 0; a < np; ++a)
+              {
+                const double La = L[a * nq + qa], Lb = L[b * nq + qb], Lc = L[c * nq + qc];
+                put(a + np * (b + np * c), (La * Lb) * Lc,
+                    (D[a * nq + qa] * Lb) * Lc * sx, (La * D[b * nq + qb]) * Lc * sy,
+                    (La * Lb) * D[c * nq + qc] * sz);
+              }
+        for (u32 j = ncl; j < n; ++j)
+          {
+            const u32 g = cmap[j] - nClassical;
+            double    f, gr[3];
+            enrichDev(atoms[g / NFUN], g % NFUN, rc, sm, x, f, gr);
+            if (alpha)
+              {
+                const double *al = alpha + (u64)g * nClassical;
+                for (int c = 0; c < np; ++c)
+                  for (int b = 0; b < np; ++b)
+                    for (int a = 0; a < np; ++a)
+                      {
+                        const double s = al[cmap[a + np * (b + np * c)]];
+                        if (s == 0.0)
+                          continue;
+                        const double La = L[a * nq + qa], Lb = L[b * nq + qb],
+                                     Lc = L[c * nq + qc];
+                        f -= s * ((La * Lb) * Lc);
+                        gr[0] -= s * ((D[a * nq + qa] * Lb) * Lc * sx);
+                        gr[1] -= s * ((La * D[b * nq + qb]) * Lc * sy);
+                        gr[2] -= s * ((La * Lb) * D[c * nq + qc] * sz);
+                      }
+              }
+            put(j, f, gr[0], gr[1], gr[2]);
+          }
+      }
+  }
+
+  // H = 0.5 K + V assembled in place: H already holds F^T W V F
+  __global__ void addHalfK(double *const *H, double *const *K, const u32 *n, u32 count)
+  {
+    const u32 e = blockIdx.x;
+    if (e >= count)
+      return;
+    const u64 nn = (u64)n[e] * n[e];
+    for (u64 t = threadIdx.x; t < nn; t += blockDim.x)
+      H[e][t] += 0.5 * K[e][t];
+  }
+
+  // copy a column block: out[e-slot rows, cols] from raw cell matrices
+  __global__ void extractClassicalBlock(const double *raw, const u64 *rawOff,
+                                        const u32 *ne, const u32 *cellList, u32 ncl,
+                                        double *out)
+  {
+    const u32 i = blockIdx.x, e = cellList[i];
+    const u64 nn = (u64)ncl * ncl;
+    for (u64 t = threadIdx.x; t < nn; t += blockDim.x)
+      {
+        const u32 r = (u32)(t % ncl), c = (u32)(t / ncl);
+        out[(u64)i * nn + t] = raw[rawOff[e] + r + (u64)ne[e] * c];
+      }
+  }
+
+  // rhs slots: out[(i*ncl + r)*5 + k] = raw_e[r, col_k]
+  __global__ void extractMixedBlock(const double *raw, const u64 *rawOff, const u32 *ne,
+                                    const u32 *cellList, const u32 *colStart, u32 ncl,
+                                    double *out)
+  {
+    const u32 i = blockIdx.x, e = cellList[i];
+    for (u32 t = threadIdx.x; t < ncl * NFUN; t += blockDim.x)
+      {
+        const u32 r = t / NFUN, k = t % NFUN;
+        out[(u64)i * ncl * NFUN + t] =
+          raw[rawOff[e] + r + (u64)ne[e] * (colStart[i] + k)];
+      }
+  }
+
+  __global__ void scatterAlpha(const double *x, const u32 *glob, const u32 *interior,
+                               u32 nLocal, u32 g0, u32 nClassical, double *alpha)
+  {
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < nLocal * NFUN;
+         t += gridDim.x * blockDim.x)
+      {
+        const u32 i = t / NFUN, k = t % NFUN;
+        if (interior[i])
+          alpha[(u64)(g0 + k) * nClassical + glob[i]] = x[t];
+      }
+  }
+} // namespace
+
+// ------------------------------------------------------------------ problem
+struct dfx_problem
+{
+  dfx_ctx         *ctx = nullptr;
+  dfx_problem_spec spec{};
+  std::vector<double> atomXYZ;
+  std::vector<int>    atomZ;
+  u32 p = 0, nCells = 0, nClassical = 0, nEnriched = 0, nDofs = 0;
+  std::vector<u32>    cellNdofs, cellMap, mapOff;
+  std::vector<u64>    matOff;
+  std::vector<double> cellLo, cellH;
+  std::vector<u32>    rows, rowSizes, rowOffsets, cols;
+  std::vector<double> weights, inhom, mgll, dinv, enrM, enrMinv;
+  std::vector<std::vector<u32>> atomCells; // enrichment supports
+  DevBuf H, M, K, alpha;                    // raw (caller-layout) cell matrices
+};
+
+namespace
+{
+  void buildHost(dfx_problem *P)
+  {
+    const auto &S  = P->spec;
+    const u32   p  = S.p, np = p + 1, ncl = np * np * np;
+    const u32   nx = S.n[0], ny = S.n[1], nz = S.n[2];
+    DFX_REQUIRE(p >= 1 && p <= 6, "p must be in 1..6");
+    DFX_REQUIRE(nx && ny && nz, "empty mesh");
+    P->p      = p;
+    P->nCells = nx * ny * nz;
+    const u64 NX = (u64)p * nx + 1, NY = (u64)p * ny + 1, NZ = (u64)p * nz + 1;
+    DFX_REQUIRE(NX * NY * NZ < (1ull << 32), "mesh too large for 32-bit dof ids");
+    std::vector<u32> lat(NX * NY * NZ, UINT32_MAX);
+    std::vector<u32> cmapC((size_t)P->nCells * ncl);
+    u32 next = 0;
+    double h[3];
+    for (int d = 0; d < 3; ++d)
+      h[d] = (S.hi[d] - S.lo[d]) / S.n[d];
+    P->cellLo.resize(3 * (size_t)P->nCells);
+    P->cellH.resize(3 * (size_t)P->nCells);
+    u32 e = 0;
+    for (u32 cz = 0; cz < nz; ++cz)
+      for (u32 cy = 0; cy < ny; ++cy)
+        for (u32 cx = 0; cx < nx; ++cx, ++e)
+          {
+            const u32 cc[3] = {cx, cy, cz};
+            for (int d = 0; d < 3; ++d)
+              {
+                P->cellLo[3 * (size_t)e + d] = S.lo[d] + h[d] * cc[d];
+                P->cellH[3 * (size_t)e + d]  = h[d];
+              }
+            for (u32 c = 0; c < np; ++c)
+              for (u32 b = 0; b < np; ++b)
+                for (u32 a = 0; a < np; ++a)
+                  {
+                    const u64 li = (cx * p + a) + NX * ((cy * p + b) + NY * (u64)(cz * p + c));
+                    if (lat[li] == UINT32_MAX)
+                      lat[li] = next++;
+                    cmapC[(size_t)e * ncl + a + np * (b + np * c)] = lat[li];
+                  }
+          }
+    P->nClassical = next;
+    // Dirichlet: every lattice point on the domain boundary, rows ascending
+    for (u64 k = 0; k < NZ; ++k)
+      for (u64 j = 0; j < NY; ++j)
+        for (u64 i = 0; i < NX; ++i)
+          if (i == 0 || j == 0 || k == 0 || i == NX - 1 || j == NY - 1 || k == NZ - 1)
+            P->rows.push_back(lat[i + NX * (j + NY * k)]);
+    std::sort(P->rows.begin(), P->rows.end());
+    P->rowSizes.assign(P->rows.size(), 0);
+    P->rowOffsets.assign(P->rows.size(), 0);
+    P->inhom.assign(P->rows.size(), 0.0);
+
+    // enrichment supports: cells whose box meets the atom's support cube
+    const u32    nA = S.enrich ? S.nAtoms : 0;
+    const double rs = S.rcut + S.rcut / S.smooth;
+    P->nEnriched    = nA * NFUN;
+    P->nDofs        = P->nClassical + P->nEnriched;
+    P->atomCells.assign(nA, {});
+    std::vector<std::vector<u32>> cellAtoms(P->nCells);
+    for (u32 a = 0; a < nA; ++a)
+      for (u32 c = 0; c < P->nCells; ++c)
+        {
+          bool in = true;
+          for (int d = 0; d < 3 && in; ++d)
+            {
+              const double lo = P->cellLo[3 * (size_t)c + d], hi = lo + P->cellH[3 * (size_t)c + d];
+              const double s0 = S.atomXYZ[3 * a + d] - rs, s1 = S.atomXYZ[3 * a + d] + rs;
+              in = !((s0 < lo && s1 < lo) || (s0 > hi && s1 > hi));
+            }
+          if (in)
+            {
+              P->atomCells[a].push_back(c);
+              cellAtoms[c].push_back(a);
+            }
+        }
+    P->cellNdofs.resize(P->nCells);
+    P->mapOff.assign(P->nCells + 1, 0);
+    P->matOff.assign(P->nCells + 1, 0);
+    for (u32 c = 0; c < P->nCells; ++c)
+      {
+        const u32 n = ncl + NFUN * (u32)cellAtoms[c].size();
+        P->cellNdofs[c]     = n;
+        P->mapOff[c + 1]    = P->mapOff[c] + n;
+        P->matOff[c + 1]    = P->matOff[c] + (u64)n * n;
+        P->cellMap.insert(P->cellMap.end(), cmapC.begin() + (size_t)c * ncl,
+                          cmapC.begin() + (size_t)(c + 1) * ncl);
+        for (u32 a : cellAtoms[c])
+          for (u32 k = 0; k < NFUN; ++k)
+            P->cellMap.push_back(P->nClassical + a * NFUN + k);
+      }
+
+    // GLL lumped mass, C2P (no hanging rows here), constrained -> 1, inverse,
+    // constrained -> 0  (CFEOverlapInverseOpContextGLL.t.cpp:380-439)
+    std::vector<double> gx, gw;
+    gllRule(np, gx, gw);
+    P->mgll.assign(P->nDofs, 0.0);
+    for (u32 c = 0; c < P->nCells; ++c)
+      {
+        const double *hh   = &P->cellH[3 * (size_t)c];
+        const double  detJ = hh[0] * hh[1] * hh[2] / 8.0;
+        for (u32 k = 0; k < np; ++k)
+          for (u32 j = 0; j < np; ++j)
+            for (u32 i = 0; i < np; ++i)
+              P->mgll[cmapC[(size_t)c * ncl + i + np * (j + np * k)]] +=
+                ((gw[i] * gw[j]) * gw[k]) * detJ;
+      }
+    P->dinv.assign(P->nDofs, 0.0);
+    std::vector<char> isRow(P->nClassical, 0);
+    for (u32 r : P->rows)
+      isRow[r] = 1;
+    for (u32 i = 0; i < P->nClassical; ++i)
+      P->dinv[i] = isRow[i] ? 0.0 : 1.0 / P->mgll[i];
+  }
+
+  // Tabulate + batched GEMMs for a list of cells (caller indices), writing
+  // their raw blocks of H, M, K on the device.
+  void assembleCells(dfx_problem *P, const std::vector<u32> &cellList, const double *alpha)
+  {
+    if (cellList.empty())
+      return;
+    dfx_ctx    *ctx = P->ctx;
+    hipStream_t s   = ctx->stream;
+    const auto &S   = P->spec;
+    const u32   np  = P->p + 1, ncl = np * np * np;
+    const u32   nqC = S.nq ? S.nq : np, nqE = S.nqEnr ? S.nqEnr : 2 * np;
+    // 1-D tables: [classical rule | enriched rule]
+    std::vector<double> tabL, tabD, tabX, tabW, gll, gllw;
+    gllRule(np, gll, gllw);
+    auto addRule = [&](u32 nq) {
+      std::vector<double> x, w;
+      gaussRule(nq, x, w);
+      const u32 off = (u32)tabL.size();
+      for (u32 a = 0; a < np; ++a)
+        for (u32 q = 0; q < nq; ++q)
+          {
+            tabL.push_back(lag(gll, a, x[q]));
+            tabD.push_back(lagD(gll, a, x[q]));
+          }
+      tabX.insert(tabX.end(), x.begin(), x.end());
+      tabW.insert(tabW.end(), w.begin(), w.end());
+      return off;
+    };
+    const u32 offC = addRule(nqC), offE = addRule(nqE);
+    DevBuf dL, dD, dX, dW, dAtoms, dMap;
+    upload(dL, tabL, s);
+    upload(dD, tabD, s);
+    upload(dX, tabX, s);
+    upload(dW, tabW, s);
+    upload(dMap, P->cellMap, s);
+    std::vector<AtomDev> at(S.nAtoms);
+    for (u32 a = 0; a < S.nAtoms; ++a)
+      at[a] = {{S.atomXYZ[3 * a], S.atomXYZ[3 * a + 1], S.atomXYZ[3 * a + 2]}, S.atomZ[a], 0};
+    upload(dAtoms, at, s);
+
+    // group by (n, nq) so each group is one batched GEMM shape
+    std::vector<u32> order(cellList);
+    auto nqOf = [&](u32 c) { return P->cellNdofs[c] > ncl ? nqE : nqC; };
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+      if (P->cellNdofs[a] != P->cellNdofs[b])
+        return P->cellNdofs[a] < P->cellNdofs[b];
+      return nqOf(a) < nqOf(b);
+    });
+    const size_t budget = (size_t)6 << 30; // bytes of tabulation buffers per batch
+    size_t i0 = 0;
+    while (i0 < order.size())
+      {
+        // batch: cells of one shape, within the memory budget
+        const u32 n = P->cellNdofs[order[i0]], nq = nqOf(order[i0]);
+        const u64 nQ = (u64)nq * nq * nq;
+        const size_t per = (size_t)9 * nQ * n * sizeof(double);
+        size_t i1 = i0;
+        while (i1 < order.size() && P->cellNdofs[order[i1]] == n && nqOf(order[i1]) == nq &&
+               (i1 - i0 + 1) * per <= budget)
+          ++i1;
+        if (i1 == i0)
+          ++i1;
+        const u32 cnt = (u32)(i1 - i0);
+        std::vector<TabCell> tc(cnt);
+        for (u32 t = 0; t < cnt; ++t)
+          {
+            const u32 c = order[i0 + t];
+            TabCell   T{};
+            for (int d = 0; d < 3; ++d)
+              {
+                T.lo[d] = P->cellLo[3 * (size_t)c + d];
+                T.h[d]  = P->cellH[3 * (size_t)c + d];
+              }
+            T.n      = n;
+            T.nq     = nq;
+            T.tabOff = nq == nqC && P->cellNdofs[c] <= ncl ? offC : offE;
+            T.mapOff = P->mapOff[c];
+            T.bufOff = (u64)t * nQ * n;
+            tc[t]    = T;
+          }
+        DevBuf dTc;
+        upload(dTc, tc, s);
+        DevBuf &bF  = work(ctx, "setup.F", (size_t)cnt * nQ * n * 8);
+        DevBuf &bFW = work(ctx, "setup.FW", (size_t)cnt * nQ * n * 8);
+        DevBuf &bFV = work(ctx, "setup.FV", (size_t)cnt * nQ * n * 8);
+        DevBuf &bG  = work(ctx, "setup.G", (size_t)cnt * 3 * nQ * n * 8);
+        DevBuf &bGW = work(ctx, "setup.GW", (size_t)cnt * 3 * nQ * n * 8);
+        tabulateK<<<cnt, 256, 0, s>>>(dTc.as<TabCell>(), dMap.as<u32>(), P->nClassical,
+                                      dL.as<double>(), dD.as<double>(), dX.as<double>(),
+                                      dW.as<double>(), (int)np, dAtoms.as<AtomDev>(),
+                                      S.nAtoms, S.vSmear, S.rcut, S.smooth, alpha,
+                                      bF.as<double>(), bFW.as<double>(), bFV.as<double>(),
+                                      bG.as<double>(), bGW.as<double>());
+        checkLaunch("tabulate");
+        // pointer arrays: A (F or G), B (FW / FV / GW), C (H / M / K)
+        std::vector<const double *> pF(cnt), pFW(cnt), pFV(cnt), pG(cnt), pGW(cnt);
+        std::vector<double *>       pH(cnt), pM(cnt), pK(cnt);
+        std::vector<u32>            nv(cnt, n);
+        for (u32 t = 0; t < cnt; ++t)
+          {
+            const u32 c = order[i0 + t];
+            pF[t]  = bF.as<double>() + (u64)t * nQ * n;
+            pFW[t] = bFW.as<double>() + (u64)t * nQ * n;
+            pFV[t] = bFV.as<double>() + (u64)t * nQ * n;
+            pG[t]  = bG.as<double>() + (u64)t * 3 * nQ * n;
+            pGW[t] = bGW.as<double>() + (u64)t * 3 * nQ * n;
+            pH[t]  = P->H.as<double>() + P->matOff[c];
+            pM[t]  = P->M.as<double>() + P->matOff[c];
+            pK[t]  = P->K.as<double>() + P->matOff[c];
+          }
+        DevBuf dpF, dpFW, dpFV, dpG, dpGW, dpH, dpM, dpK, dnv;
+        upload(dpF, pF, s);
+        upload(dpFW, pFW, s);
+        upload(dpFV, pFV, s);
+        upload(dpG, pG, s);
+        upload(dpGW, pGW, s);
+        upload(dpH, pH, s);
+        upload(dpM, pM, s);
+        upload(dpK, pK, s);
+        upload(dnv, nv, s);
+        const double one = 1.0, zero = 0.0;
+        auto gemm = [&](const DevBuf &A, const DevBuf &B, const DevBuf &Cc, rocblas_int kdim) {
+          rocblas_status st = rocblas_dgemm_batched(
+            ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+            (rocblas_int)n, (rocblas_int)n, kdim, &one, A.as<const double *>(), kdim,
+            B.as<const double *>(), kdim, &zero, Cc.as<double *>(), (rocblas_int)n,
+            (rocblas_int)cnt);
+          if (st != rocblas_status_success)
+            throw Error(DFX_E_HIP, "rocblas_dgemm_batched failed");
+        };
+        rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+        gemm(dpF, dpFW, dpM, (rocblas_int)nQ);        // M = F^T W F
+        gemm(dpG, dpGW, dpK, (rocblas_int)(3 * nQ));  // K = G^T W G
+        gemm(dpF, dpFV, dpH, (rocblas_int)nQ);        // H = F^T W V F ...
+        addHalfK<<<cnt, 256, 0, s>>>(dpH.as<double *>(), dpK.as<double *>(), dnv.as<u32>(),
+                                     cnt);             // ... + K / 2
+        checkLaunch("addHalfK");
+        DFX_HIP(hipStreamSynchronize(s));
+        i0 = i1;
+      }
+  }
+
+  // Per-atom L2 projection of the enrichment functions on the classical
+  // nodes interior to the atom's support, with a cell-wise mass operator over
+  // the support cells (non-interior nodes as zero Dirichlet rows) and PCG.
+  void orthogonalize(dfx_problem *P)
+  {
+    dfx_ctx    *ctx = P->ctx;
+    hipStream_t s   = ctx->stream;
+    const u32   np = P->p + 1, ncl = np * np * np;
+    const u32   nA = (u32)P->atomCells.size();
+    P->alpha.alloc((size_t)P->nEnriched * P->nClassical * sizeof(double));
+    DFX_HIP(hipMemsetAsync(P->alpha.p, 0, P->alpha.bytes, s));
+    // global incidence count of each classical node
+    std::vector<u32> inc(P->nClassical, 0);
+    for (u32 c = 0; c < P->nCells; ++c)
+      for (u32 l = 0; l < ncl; ++l)
+        ++inc[P->cellMap[P->mapOff[c] + l]];
+    std::vector<char> isRow(P->nClassical, 0);
+    for (u32 r : P->rows)
+      isRow[r] = 1;
+    DevBuf dRawOff, dNe;
+    upload(dRawOff, P->matOff, s);
+    upload(dNe, P->cellNdofs, s);
+    for (u32 a = 0; a < nA; ++a)
+      {
+        const auto &cells = P->atomCells[a];
+        const u32   nc    = (u32)cells.size();
+        // local numbering of the support's classical nodes
+        std::vector<u32> loc(P->nClassical, UINT32_MAX), glob, lmap((size_t)nc * ncl), cnt;
+        for (u32 i = 0; i < nc; ++i)
+          for (u32 l = 0; l < ncl; ++l)
+            {
+              const u32 g = P->cellMap[P->mapOff[cells[i]] + l];
+              if (loc[g] == UINT32_MAX)
+                {
+                  loc[g] = (u32)glob.size();
+                  glob.push_back(g);
+                  cnt.push_back(0);
+                }
+              ++cnt[loc[g]];
+              lmap[(size_t)i * ncl + l] = loc[g];
+            }
+        const u32 nl = (u32)glob.size();
+        std::vector<u32> interior(nl), crow, csz, coff;
+        for (u32 i = 0; i < nl; ++i)
+          {
+            interior[i] = (cnt[i] == inc[glob[i]] && !isRow[glob[i]]) ? 1u : 0u;
+            if (!interior[i])
+              {
+                crow.push_back(i);
+                csz.push_back(0);
+                coff.push_back(0);
+              }
+          }
+        std::vector<double> cinh(crow.size(), 0.0);
+        // column of this atom's first function in each support cell
+        std::vector<u32> colStart(nc);
+        for (u32 i = 0; i < nc; ++i)
+          {
+            const u32 c = cells[i];
+            u32       j = ncl;
+            while (P->cellMap[P->mapOff[c] + j] != P->nClassical + a * NFUN)
+              ++j;
+            colStart[i] = j;
+          }
+        DevBuf dCells, dCol, blocks, rhsSlots;
+        upload(dCells, cells, s);
+        upload(dCol, colStart, s);
+        blocks.alloc((size_t)nc * ncl * ncl * sizeof(double));
+        rhsSlots.alloc((size_t)nc * ncl * NFUN * sizeof(double));
+        extractClassicalBlock<<<nc, 256, 0, s>>>(P->M.as<double>(), dRawOff.as<u64>(),
+                                                 dNe.as<u32>(), dCells.as<u32>(), ncl,
+                                                 blocks.as<double>());
+        extractMixedBlock<<<nc, 256, 0, s>>>(P->M.as<double>(), dRawOff.as<u64>(),
+                                             dNe.as<u32>(), dCells.as<u32>(), dCol.as<u32>(),
+                                             ncl, rhsSlots.as<double>());
+        checkLaunch("extract");
+        DFX_HIP(hipStreamSynchronize(s));
+        std::vector<double> hBlocks((size_t)nc * ncl * ncl);
+        DFX_HIP(hipMemcpy(hBlocks.data(), blocks.p, blocks.bytes, hipMemcpyDeviceToHost));
+        std::vector<u32> cn(nc, ncl);
+        dfx_constraints *cons = nullptr;
+        dfx_op *op = nullptr, *pc = nullptr;
+        dfx_mv *rs = nullptr, *rhs = nullptr, *x = nullptr;
+        auto cleanup = [&] {
+          if (op) dfx_op_destroy(op);
+          if (pc) dfx_op_destroy(pc);
+          if (rs) dfx_mv_destroy(rs);
+          if (rhs) dfx_mv_destroy(rhs);
+          if (x) dfx_mv_destroy(x);
+          if (cons) dfx_constraints_destroy(cons);
+        };
+        auto chk = [&](int rc, const char *w) {
+          if (rc < 0)
+            {
+              std::string m = std::string(w) + ": " + ctx->err;
+              cleanup();
+              throw Error(rc, m);
+            }
+        };
+        chk(dfx_constraints_create(ctx, nl, (u32)crow.size(), crow.data(), csz.data(),
+                                   coff.data(), nullptr, nullptr, cinh.data(), &cons),
+            "orthogonalize constraints");
+        chk(dfx_op_create_cellwise(ctx, nc, cn.data(), lmap.data(), hBlocks.data(), nl, nl,
+                                   cons, 0, &op),
+            "orthogonalize mass operator");
+        // rhs d = sum_e Q_e^T (mixed block), then zero the non-interior rows
+        chk(dfx_mv_wrap(ctx, rhsSlots.p, nc * ncl, NFUN, DFX_F64, &rs), "wrap");
+        chk(dfx_mv_create(ctx, nl, NFUN, DFX_F64, &rhs), "rhs");
+        chk(dfx_op_scatter_add(op, rs, rhs), "rhs assembly");
+        chk(dfx_constraints_set_constrained_nodes(cons, rhs, 0.0), "rhs rows");
+        // Jacobi: assembled diagonal of the local mass
+        std::vector<double> diag(nl, 0.0);
+        for (u32 i = 0; i < nc; ++i)
+          for (u32 l = 0; l < ncl; ++l)
+            diag[lmap[(size_t)i * ncl + l]] += hBlocks[(size_t)i * ncl * ncl + l + (size_t)ncl * l];
+        for (u32 i = 0; i < nl; ++i)
+          diag[i] = interior[i] ? 1.0 / diag[i] : 1.0;
+        chk(dfx_op_create_diagonal(ctx, nl, diag.data(), 0, nullptr, nullptr, &pc), "pc");
+        chk(dfx_mv_create(ctx, nl, NFUN, DFX_F64, &x), "x");
+        u32 iters = 0;
+        int rc = dfx_cg_solve(ctx, op, pc, rhs, x, 1e-300, 1e-13, 1e10, 10000, &iters, nullptr);
+        if (rc != 0)
+          {
+            cleanup();
+            throw Error(DFX_E_STATE, "enrichment orthogonalization: PCG did not converge");
+          }
+        DevBuf dGlob, dInt;
+        upload(dGlob, glob, s);
+        upload(dInt, interior, s);
+        scatterAlpha<<<256, 256, 0, s>>>(static_cast<double *>(x->d), dGlob.as<u32>(),
+                                         dInt.as<u32>(), nl, a * NFUN, P->nClassical,
+                                         P->alpha.as<double>());
+        checkLaunch("scatterAlpha");
+        DFX_HIP(hipStreamSynchronize(s));
+        cleanup();
+      }
+  }
+
+  // M^ee assembled in cell order from the final cell matrices, and its inverse
+  void enrichedBlock(dfx_problem *P)
+  {
+    const u32 nE = P->nEnriched, nC = P->nClassical;
+    if (!nE)
+      return;
+    const u32 ncl = (P->p + 1) * (P->p + 1) * (P->p + 1);
+    P->enrM.assign((size_t)nE * nE, 0.0);
+    std::vector<double> blk;
+    for (u32 c = 0; c < P->nCells; ++c)
+      {
+        const u32 n = P->cellNdofs[c];
+        if (n == ncl)
+          continue;
+        blk.resize((size_t)n * n);
+        DFX_HIP(hipMemcpy(blk.data(), P->M.as<double>() + P->matOff[c], blk.size() * 8,
+                          hipMemcpyDeviceToHost));
+        for (u32 j = ncl; j < n; ++j)
+          for (u32 i = ncl; i < n; ++i)
+            P->enrM[(P->cellMap[P->mapOff[c] + i] - nC) +
+                    (size_t)nE * (P->cellMap[P->mapOff[c] + j] - nC)] += blk[i + (size_t)n * j];
+      }
+    // inverse through the Cholesky factor: L L^T = M, inv = L^-T L^-1
+    std::vector<double> L(P->enrM);
+    for (u32 j = 0; j < nE; ++j)
+      {
+        double d = L[j + (size_t)nE * j];
+        for (u32 k = 0; k < j; ++k)
+          d -= L[j + (size_t)nE * k] * L[j + (size_t)nE * k];
+        DFX_REQUIRE(d > 0.0, "enriched overlap block is not positive definite");
+        d                        = std::sqrt(d);
+        L[j + (size_t)nE * j]    = d;
+        for (u32 i = j + 1; i < nE; ++i)
+          {
+            double t = L[i + (size_t)nE * j];
+            for (u32 k = 0; k < j; ++k)
+              t -= L[i + (size_t)nE * k] * L[j + (size_t)nE * k];
+            L[i + (size_t)nE * j] = t / d;
+          }
+      }
+    std::vector<double> Li((size_t)nE * nE, 0.0);
+    for (u32 j = 0; j < nE; ++j)
+      {
+        Li[j + (size_t)nE * j] = 1.0 / L[j + (size_t)nE * j];
+        for (u32 i = j + 1; i < nE; ++i)
+          {
+            double t = 0.0;
+            for (u32 k = j; k < i; ++k)
+              t += L[i + (size_t)nE * k] * Li[k + (size_t)nE * j];
+            Li[i + (size_t)nE * j] = -t / L[i + (size_t)nE * i];
+          }
+      }
+    P->enrMinv.assign((size_t)nE * nE, 0.0);
+    for (u32 j = 0; j < nE; ++j)
+      for (u32 i = 0; i < nE; ++i)
+        {
+          double t = 0.0;
+          for (u32 k = std::max(i, j); k < nE; ++k)
+            t += Li[k + (size_t)nE * i] * Li[k + (size_t)nE * j];
+          P->enrMinv[i + (size_t)nE * j] = t;
+        }
+  }
+} // namespace
+
+extern "C"
+{
+  int dfx_problem_build(dfx_ctx *ctx, const dfx_problem_spec *spec, dfx_problem **out)
+  {
+    return guard(ctx, [&] {
+      DFX_REQUIRE(spec && out, "null argument");
+      DFX_REQUIRE(!spec->nAtoms || (spec->atomXYZ && spec->atomZ), "atoms missing");
+      std::unique_ptr<dfx_problem> P(new dfx_problem());
+      P->ctx  = ctx;
+      P->spec = *spec;
+      P->atomXYZ.assign(spec->atomXYZ, spec->atomXYZ + 3 * (size_t)spec->nAtoms);
+      P->atomZ.assign(spec->atomZ, spec->atomZ + spec->nAtoms);
+      P->spec.atomXYZ = P->atomXYZ.data();
+      P->spec.atomZ   = P->atomZ.data();
+      buildHost(P.get());
+      const u64 nm = P->matOff.back();
+      P->H.alloc(nm * 8);
+      P->M.alloc(nm * 8);
+      P->K.alloc(nm * 8);
+      std::vector<u32> all(P->nCells);
+      std::iota(all.begin(), all.end(), 0u);
+      assembleCells(P.get(), all, nullptr);
+      if (P->nEnriched && spec->orthogonalize)
+        {
+          orthogonalize(P.get());
+          std::vector<u32> enr;
+          for (u32 c = 0; c < P->nCells; ++c)
+            if (P->cellNdofs[c] > (P->p + 1) * (P->p + 1) * (P->p + 1))
+              enr.push_back(c);
+          assembleCells(P.get(), enr, P->alpha.as<double>());
+        }
+      enrichedBlock(P.get());
+      ctx->work.erase("setup.F");
+      ctx->work.erase("setup.FW");
+      ctx->work.erase("setup.FV");
+      ctx->work.erase("setup.G");
+      ctx->work.erase("setup.GW");
+      ctxRetain(ctx);
+      *out = P.release();
+      return 0;
+    });
+  }
+
+  int dfx_problem_sizes(const dfx_problem *P, uint64_t *s)
+  {
+    if (!P || !s)
+      return DFX_E_ARG;
+    s[0] = P->nCells;
+    s[1] = P->nDofs;
+    s[2] = P->nClassical;
+    s[3] = P->nEnriched;
+    s[4] = P->cellMap.size();
+    s[5] = P->matOff.back();
+    s[6] = P->rows.size();
+    s[7] = P->cols.size();
+    return 0;
+  }
+
+  int dfx_problem_get(const dfx_problem *P, int which, void *host)
+  {
+    if (!P || !host)
+      return DFX_E_ARG;
+    return guard(P->ctx, [&] {
+      auto cpu = [&](const void *src, size_t bytes) {
+        if (bytes)
+          std::memcpy(host, src, bytes);
+        return 0;
+      };
+      auto dev = [&](const DevBuf &b, size_t bytes) {
+        if (bytes)
+          DFX_HIP(hipMemcpy(host, b.p, bytes, hipMemcpyDeviceToHost));
+        return 0;
+      };
+      const size_t nm = P->matOff.back() * 8;
+      switch (which)
+        {
+          case DFX_PROB_CELL_NDOFS: return cpu(P->cellNdofs.data(), P->cellNdofs.size() * 4);
+          case DFX_PROB_CELL_MAP: return cpu(P->cellMap.data(), P->cellMap.size() * 4);
+          case DFX_PROB_ROWS: return cpu(P->rows.data(), P->rows.size() * 4);
+          case DFX_PROB_ROW_SIZES: return cpu(P->rowSizes.data(), P->rowSizes.size() * 4);
+          case DFX_PROB_ROW_OFFSETS: return cpu(P->rowOffsets.data(), P->rowOffsets.size() * 4);
+          case DFX_PROB_COLS: return cpu(P->cols.data(), P->cols.size() * 4);
+          case DFX_PROB_WEIGHTS: return cpu(P->weights.data(), P->weights.size() * 8);
+          case DFX_PROB_INHOM: return cpu(P->inhom.data(), P->inhom.size() * 8);
+          case DFX_PROB_H: return dev(P->H, nm);
+          case DFX_PROB_M: return dev(P->M, nm);
+          case DFX_PROB_K: return dev(P->K, nm);
+          case DFX_PROB_DINV: return cpu(P->dinv.data(), P->dinv.size() * 8);
+          case DFX_PROB_MGLL: return cpu(P->mgll.data(), P->mgll.size() * 8);
+          case DFX_PROB_ENR_MINV: return cpu(P->enrMinv.data(), P->enrMinv.size() * 8);
+          case DFX_PROB_ENR_M: return cpu(P->enrM.data(), P->enrM.size() * 8);
+          case DFX_PROB_ALPHA:
+            return P->alpha.p ? dev(P->alpha, P->alpha.bytes)
+                              : cpu(nullptr, 0);
+        }
+      throw Error(DFX_E_ARG, "unknown problem array");
+    });
+  }
+
+  int dfx_problem_create_operators(dfx_problem *P, uint32_t flags, dfx_op **H, dfx_op **M,
+                                   dfx_op **K, dfx_op **MInv, dfx_op **MLump,
+                                   dfx_constraints **consOut)
+  {
+    if (!P)
+      return DFX_E_ARG;
+    dfx_ctx *ctx = P->ctx;
+    return guard(ctx, [&] {
+      dfx_constraints *cons = nullptr;
+      int rc = dfx_constraints_create(ctx, P->nDofs, (u32)P->rows.size(), P->rows.data(),
+                                      P->rowSizes.data(), P->rowOffsets.data(),
+                                      P->cols.data(), P->weights.data(), P->inhom.data(),
+                                      &cons);
+      if (rc)
+        throw Error(rc, ctx->err);
+      auto cell = [&](const DevBuf &mats, dfx_op **o) {
+        if (!o)
+          return;
+        std::unique_ptr<dfx_op> op(new dfx_op());
+        buildCellwise(ctx, op.get(), P->nCells, P->cellNdofs.data(), P->cellMap.data(),
+                      P->nDofs, P->nClassical, cons, flags);
+        uploadCellMatrices(op.get(), mats.as<double>(), /*device=*/true);
+        ctxRetain(ctx);
+        *o = op.release();
+      };
+      cell(P->H, H);
+      cell(P->M, M);
+      cell(P->K, K);
+      if (MInv)
+        {
+          rc = dfx_op_create_diagonal(ctx, P->nDofs, P->dinv.data(), P->nEnriched,
+                                      P->nEnriched ? P->enrMinv.data() : nullptr, cons, MInv);
+          if (rc)
+            throw Error(rc, ctx->err);
+        }
+      if (MLump)
+        {
+          rc = dfx_op_create_diagonal(ctx, P->nDofs, P->mgll.data(), P->nEnriched,
+                                      P->nEnriched ? P->enrM.data() : nullptr, cons, MLump);
+          if (rc)
+            throw Error(rc, ctx->err);
+        }
+      if (consOut)
+        *consOut = cons;
+      else
+        dfx_constraints_destroy(cons); // operators keep their own reference
+      return 0;
+    });
+  }
+
+  int dfx_problem_destroy(dfx_problem *P)
+  {
+    if (!P)
+      return DFX_E_ARG;
+    dfx_ctx *ctx = P->ctx;
+    delete P;
+    ctxRelease(ctx);
+    return 0;
+  }
+}

@@ -437,6 +437,46 @@ namespace dfx
       return it != lookup.end() ? it->second : 1250;
     }
 
+    // fermiDirac / fermiDiracDer (src/ksdft/FractionalOccupancyFunction.cpp:9-33)
+    double fermiDirac(double e, double mu, double kb, double T)
+    {
+      const double f = (e - mu) / (kb * T);
+      return f >= 0 ? std::exp(-f) / (1.0 + std::exp(-f)) : 1.0 / (1.0 + std::exp(f));
+    }
+    double fermiDiracDer(double e, double mu, double kb, double T)
+    {
+      const double f = (e - mu) / (kb * T), beta = 1.0 / (kb * T);
+      return f >= 0 ? beta * std::exp(-f) / (1.0 + std::exp(-f)) / (1.0 + std::exp(-f))
+                    : beta * std::exp(f) / (1.0 + std::exp(f)) / (1.0 + std::exp(f));
+    }
+    constexpr double kBoltzmannHartree = 3.166811429e-06; // ksdft/Defaults.cpp:69
+
+    // Fermi level: NewtonRaphsonSolver on sum_i 2 f_i - Ne
+    // (src/linearAlgebra/NewtonRaphsonSolver.t.cpp, FractionalOccupancyFunction.cpp:49-71),
+    // started at eps[ceil(Ne/2) - 1] (KohnShamEigenSolver.t.cpp:396-425)
+    double fermiLevel(const std::vector<double> &eps, double Ne, double T, double tol)
+    {
+      size_t i0 = (size_t)std::ceil(Ne / 2.0);
+      i0        = i0 ? std::min(i0 - 1, eps.size() - 1) : 0;
+      double x  = eps[i0];
+      for (int it = 0; it < 100000; ++it)
+        {
+          double g = -Ne, gp = 0.0;
+          for (double e : eps)
+            {
+              g += 2 * fermiDirac(e, x, kBoltzmannHartree, T);
+              gp += 2 * fermiDiracDer(e, x, kBoltzmannHartree, T);
+            }
+          if (gp == 0.0)
+            break;
+          const double x1 = x - g / gp;
+          if (std::fabs(x1 - x) < tol)
+            return x1;
+          x = x1;
+        }
+      return x;
+    }
+
     void eigenResidualNorms(dfx_ctx *ctx, dfx_op *H, dfx_op *M, View<double> X,
                        const double *eig, double *res)
     {
@@ -628,10 +668,28 @@ extern "C"
           R.residualMs += std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now() - tr).count();
           k::convert<double, double>(ctx->stream, mvView(guess), cv(mvView(waveFns)));
-          bool conv = true;
-          for (u32 j = 0; j < n; ++j)
-            conv = conv && (res[j] <= p->eigenSolveResidualTolerance);
-          if (conv)
+          u32 nOcc = n, nOccConv = 0;
+          if (p->numElectrons > 0)
+            {
+              std::vector<double> eps(energies, energies + n);
+              R.fermiEnergy = fermiLevel(eps, p->numElectrons, p->smearingTemperature,
+                                         p->fermiEnergyTolerance);
+              nOcc = 0;
+              for (u32 j = 0; j < n; ++j)
+                if (fermiDirac(eps[j], R.fermiEnergy, kBoltzmannHartree,
+                               p->smearingTemperature) > p->fracOccupancyTolerance)
+                  {
+                    ++nOcc;
+                    if (res[j] <= p->eigenSolveResidualTolerance)
+                      ++nOccConv;
+                  }
+            }
+          else
+            for (u32 j = 0; j < n; ++j)
+              if (res[j] <= p->eigenSolveResidualTolerance)
+                ++nOccConv;
+          R.numOccupied = nOcc;
+          if (nOccConv == nOcc)
             {
               status = 0;
               ++pass;

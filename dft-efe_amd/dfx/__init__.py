@@ -74,6 +74,7 @@ _sig("dfx_mv_upload", _vp, _vp)
 _sig("dfx_mv_download", _vp, _vp)
 _sig("dfx_mv_device_ptr", _vp, _pp)
 _sig("dfx_mv_set", _vp, _dbl)
+_sig("dfx_mv_fill_random", _vp, C.c_uint64, _dbl, _dbl)
 _sig("dfx_mv_copy", _vp, _vp)
 _sig("dfx_mv_axpby", _vp, _dbl, _vp, _dbl, _vp)
 _sig("dfx_mv_axpby_blocked", _vp, _dbl, _vp, _vp, _dbl, _vp, _vp)
@@ -105,6 +106,13 @@ _sig("dfx_rayleigh_ritz", _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_lanczos_extreme", _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp)
 _sig("dfx_ks_eigensolve", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_cg_solve", _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _u32, _vp, _vp)
+_sig("dfx_op_set_profiling", _vp, C.c_int)
+_sig("dfx_op_get_profile", _vp, _vp, C.c_int)
+_sig("dfx_problem_build", _vp, _vp, _pp)
+_sig("dfx_problem_sizes", _vp, _vp)
+_sig("dfx_problem_get", _vp, C.c_int, _vp)
+_sig("dfx_problem_create_operators", _vp, _u32, _pp, _pp, _pp, _pp, _pp, _pp)
+_sig("dfx_problem_destroy", _vp)
 
 
 def exported_symbols():
@@ -196,6 +204,10 @@ class MultiVector:
 
     def set(self, v):
         self.ctx.check(_lib.dfx_mv_set(self.h, float(v)))
+
+    def fill_random(self, seed, a=-1.0, b=1.0):
+        """SplitMix64 uniform [a, b) fill, on the device."""
+        self.ctx.check(_lib.dfx_mv_fill_random(self.h, C.c_uint64(seed), float(a), float(b)))
 
     def copy_from(self, other):
         self.ctx.check(_lib.dfx_mv_copy(self.h, other.h))
@@ -295,6 +307,11 @@ class _OpInfo(C.Structure):
                 ("matBytesF64", C.c_uint64), ("matBytesF32", C.c_uint64)]
 
 
+class _OpProfile(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("kernelMs", C.c_double), ("flops", C.c_double),
+                ("bytes", C.c_double)]
+
+
 class _Operator:
     def apply(self, X, Y):
         """Y = Op X (X's constrained rows are set by parent->child, as in the
@@ -305,6 +322,19 @@ class _Operator:
         i = _OpInfo()
         _lib.dfx_op_get_info(self.h, C.byref(i))
         return {k: getattr(i, k) for k, _ in _OpInfo._fields_}
+
+    def set_profiling(self, enable=True, precision=None):
+        """Time this operator's cell-apply kernel launches; precision F64 / F32
+        restricts to launches of that value type."""
+        mode = 0 if not enable else (3 if precision is None else (1 if precision == F64 else 2))
+        _lib.dfx_op_set_profiling(self.h, mode)
+
+    def profile(self, reset=True):
+        """launches, summed kernel ms, algorithmic flops and bytes of the
+        profiled cell-apply kernel launches."""
+        p = _OpProfile()
+        self.ctx.check(_lib.dfx_op_get_profile(self.h, C.byref(p), 1 if reset else 0))
+        return {k: getattr(p, k) for k, _ in _OpProfile._fields_}
 
     def close(self):
         if getattr(self, "h", None):
@@ -452,7 +482,9 @@ class _KsParams(C.Structure):
     _fields_ = [("numWantedEigenvalues", C.c_uint32), ("maxChebyshevFilterPass", C.c_uint32),
                 ("polynomialDegree", C.c_uint32), ("eigenSolveResidualTolerance", C.c_double),
                 ("waveFunctionBatchSize", C.c_uint32), ("isResidualChebyshevFilter", C.c_int),
-                ("precision", C.c_int), ("lanczosMaxKrylov", C.c_uint32)]
+                ("precision", C.c_int), ("lanczosMaxKrylov", C.c_uint32),
+                ("numElectrons", C.c_double), ("smearingTemperature", C.c_double),
+                ("fermiEnergyTolerance", C.c_double), ("fracOccupancyTolerance", C.c_double)]
 
 
 class _KsResult(C.Structure):
@@ -460,21 +492,27 @@ class _KsResult(C.Structure):
                 ("lanczosHigh", C.c_double), ("lanczosBeta", C.c_double),
                 ("a", C.c_double), ("b", C.c_double), ("c", C.c_double),
                 ("degree", C.c_uint32), ("filterMs", C.c_double), ("orthoMs", C.c_double),
-                ("rrMs", C.c_double), ("residualMs", C.c_double), ("lanczosMs", C.c_double)]
+                ("rrMs", C.c_double), ("residualMs", C.c_double), ("lanczosMs", C.c_double),
+                ("fermiEnergy", C.c_double), ("numOccupied", C.c_uint32)]
 
 
 class KohnShamEigenSolver:
-    """ksdft::KohnShamEigenSolver (Lanczos bounds, degree lookup, ChFSI passes)."""
+    """ksdft::KohnShamEigenSolver (Lanczos bounds, degree lookup, ChFSI passes,
+    Fermi level and occupied-state residual convergence)."""
 
     def __init__(self, ctx, numWantedEigenvalues, eigenSolveResidualTolerance,
                  maxChebyshevFilterPass, waveFunctionSubspaceGuess, lanczosGuess,
                  polynomialDegree=0, waveFunctionBatchSize=0,
-                 isResidualChebyshevFilter=True, precision=PREC_FP64, lanczosMaxKrylov=20):
+                 isResidualChebyshevFilter=True, precision=PREC_FP64, lanczosMaxKrylov=20,
+                 numElectrons=0.0, smearingTemperature=500.0, fermiEnergyTolerance=1e-10,
+                 fracOccupancyTolerance=1e-3):
         self.ctx = ctx
         self.p = _KsParams(int(numWantedEigenvalues), int(maxChebyshevFilterPass),
                            int(polynomialDegree), float(eigenSolveResidualTolerance),
                            int(waveFunctionBatchSize), 1 if isResidualChebyshevFilter else 0,
-                           int(precision), int(lanczosMaxKrylov))
+                           int(precision), int(lanczosMaxKrylov), float(numElectrons),
+                           float(smearingTemperature), float(fermiEnergyTolerance),
+                           float(fracOccupancyTolerance))
         self.guess = waveFunctionSubspaceGuess
         self.lanczosGuess = lanczosGuess
         self.result = None
@@ -510,3 +548,92 @@ class CGLinearSolver:
         self.iterations = it.value
         self.residualNorms = res
         return rc
+
+
+# ----------------------------------------------------------------- problems
+class _ProblemSpec(C.Structure):
+    _fields_ = [("p", C.c_uint32), ("n", C.c_uint32 * 3), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("nq", C.c_uint32), ("nqEnr", C.c_uint32),
+                ("nAtoms", C.c_uint32), ("atomXYZ", C.c_void_p), ("atomZ", C.c_void_p),
+                ("vSmear", C.c_double), ("enrich", C.c_int), ("rcut", C.c_double),
+                ("smooth", C.c_double), ("orthogonalize", C.c_int)]
+
+
+_PROB_ARRAYS = {  # name: (id, dtype, size key)
+    "cellNdofs": (0, np.uint32, "nCells"), "cellMap": (1, np.uint32, "mapLen"),
+    "rows": (2, np.uint32, "nRows"), "rowSizes": (3, np.uint32, "nRows"),
+    "rowOffsets": (4, np.uint32, "nRows"), "cols": (5, np.uint32, "nEntries"),
+    "weights": (6, np.float64, "nEntries"), "inhom": (7, np.float64, "nRows"),
+    "H": (8, np.float64, "matLen"), "M": (9, np.float64, "matLen"),
+    "K": (10, np.float64, "matLen"), "dinv": (11, np.float64, "nDofs"),
+    "mgll": (12, np.float64, "nDofs"), "enrMinv": (13, np.float64, "nE2"),
+    "enrM": (14, np.float64, "nE2"), "alpha": (15, np.float64, "nEC"),
+}
+
+
+class Problem:
+    """The structured-mesh FE problem (maps, Dirichlet rows, cell matrices)
+    built on the GPU: the stand-in for the reference's deal.II setup."""
+
+    def __init__(self, ctx, p=3, n=(8, 8, 8), lo=(-10., -10., -10.), hi=(10., 10., 10.),
+                 atoms=(), nq=0, nq_enr=0, v_smear=0.5, enrich=False, rcut=4.0,
+                 smooth=2.0, orthogonalize=True):
+        self.ctx = ctx
+        xyz = np.ascontiguousarray([a[:3] for a in atoms], dtype=np.float64).reshape(-1)
+        Z = np.ascontiguousarray([int(a[3]) for a in atoms], dtype=np.int32)
+        self._keep = (xyz, Z)
+        s = _ProblemSpec()
+        s.p = p
+        s.n = (C.c_uint32 * 3)(*[int(v) for v in n])
+        s.lo = (C.c_double * 3)(*[float(v) for v in lo])
+        s.hi = (C.c_double * 3)(*[float(v) for v in hi])
+        s.nq, s.nqEnr, s.nAtoms = nq, nq_enr, len(atoms)
+        s.atomXYZ = xyz.ctypes.data if len(atoms) else None
+        s.atomZ = Z.ctypes.data if len(atoms) else None
+        s.vSmear, s.enrich, s.rcut, s.smooth = v_smear, 1 if enrich else 0, rcut, smooth
+        s.orthogonalize = 1 if orthogonalize else 0
+        h = C.c_void_p()
+        ctx.check(_lib.dfx_problem_build(ctx.h, C.byref(s), C.byref(h)))
+        self.h = h
+        sz = np.zeros(8, dtype=np.uint64)
+        _lib.dfx_problem_sizes(self.h, _ptr(sz))
+        (self.nCells, self.nDofs, self.nClassical, self.nEnriched, self.mapLen,
+         self.matLen, self.nRows, self.nEntries) = [int(v) for v in sz]
+        self.p = p
+
+    def get(self, name):
+        wid, dt, key = _PROB_ARRAYS[name]
+        n = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical}.get(
+            key, getattr(self, key, 0))
+        out = np.zeros(n, dtype=dt)
+        if n:
+            self.ctx.check(_lib.dfx_problem_get(self.h, wid, _ptr(out)))
+        return out
+
+    def operators(self, keep_fp32=True):
+        """(H, M, K, MInv, MLump, constraints) as operator objects."""
+        hs = [C.c_void_p() for _ in range(6)]
+        self.ctx.check(_lib.dfx_problem_create_operators(
+            self.h, OP_KEEP_FP32 if keep_fp32 else 0, *[C.byref(x) for x in hs]))
+        ops = []
+        for k, x in enumerate(hs[:5]):
+            o = CellOperator.__new__(CellOperator) if k < 3 else DiagonalOperator.__new__(DiagonalOperator)
+            o.ctx, o.h, o.nDofs = self.ctx, x, self.nDofs
+            o.nCells, o.sumNe = self.nCells, self.mapLen
+            ops.append(o)
+        cons = Constraints.__new__(Constraints)
+        cons.ctx, cons.h, cons.nDofs = self.ctx, hs[5], self.nDofs
+        for o in ops:
+            o.constraints = cons
+        return tuple(ops) + (cons,)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -98,6 +98,10 @@ int dfx_mv_upload(dfx_mv *mv, const void *host);
 int dfx_mv_download(const dfx_mv *mv, void *host);
 int dfx_mv_device_ptr(dfx_mv *mv, void **ptr);
 int dfx_mv_set(dfx_mv *mv, double value);
+/* entry t = i*B + j set to a + (b - a) u_t, u_t the t-th SplitMix64 output
+ * of `seed` mapped to [0,1) by its top 53 bits (replaces the reference's
+ * time-seeded random initial subspace, src/utils/RandNumGen.cpp:40-43) */
+int dfx_mv_fill_random(dfx_mv *mv, uint64_t seed, double a, double b);
 /* dst = src with value-type conversion; same N and B */
 int dfx_mv_copy(dfx_mv *dst, const dfx_mv *src);
 /* Z = a X + b Y  (blasLapack::axpby, src/linearAlgebra/BlasLapack.t.cpp:228-237) */
@@ -193,6 +197,21 @@ int dfx_op_get_info(const dfx_op *op, dfx_op_info *info);
 /* Download the device-resident cell->dof map in the caller's cell order
  * (bit-exact round trip of what dfx_op_create_cellwise uploaded). */
 int dfx_op_export_map(const dfx_op *op, uint32_t *cellNdofs, uint32_t *cellDofMap);
+/* Per-launch timing of this operator's fused cell-apply kernels: with
+ * profiling on, every kernel launch is bracketed by HIP events on the
+ * context's stream.  get_profile synchronizes, and reports the launches,
+ * their summed kernel time, and the algorithmic flops (2 B sum n_e^2) and
+ * bytes (matrices + map + gathered X + scattered Y) of those launches. */
+typedef struct dfx_op_profile
+{
+  uint64_t launches;
+  double   kernelMs;
+  double   flops;
+  double   bytes;
+} dfx_op_profile;
+/* enable: 0 off; 1 FP64 launches; 2 FP32 launches; 3 both */
+int dfx_op_set_profiling(dfx_op *op, int enable);
+int dfx_op_get_profile(dfx_op *op, dfx_op_profile *out, int reset);
 /* Stand-alone cell gather: Xcell[slot, :] = X[map[slot], :], Xcell is
  * (sum_e n_e) x B  (FECellWiseDataOperations::copyFieldToCellWiseData,
  * src/basis/FECellWiseDataOperations.t.cpp:31-84). */
@@ -253,7 +272,11 @@ int dfx_lanczos_extreme(dfx_ctx *ctx, dfx_op *A, dfx_op *B, dfx_op *BInv,
                         double *highest, double *betaLast);
 
 /* KohnShamEigenSolver::solve (src/ksdft/KohnShamEigenSolver.t.cpp:214-536):
- * Lanczos bounds, degree lookup, ChFSI passes, residual norms. */
+ * Lanczos bounds, degree lookup, ChFSI passes, Fermi level by Newton-Raphson
+ * on Fermi-Dirac occupancies (FractionalOccupancyFunction.cpp:9-71), residual
+ * norms; converged when every state with occupancy > fracOccupancyTolerance
+ * has residual <= eigenSolveResidualTolerance.  numElectrons == 0 counts every
+ * state as occupied. */
 typedef struct dfx_ks_params
 {
   uint32_t numWantedEigenvalues;
@@ -264,6 +287,10 @@ typedef struct dfx_ks_params
   int      isResidualChebyshevFilter;
   int      precision;                 /* DFX_PREC_* */
   uint32_t lanczosMaxKrylov;          /* 0 -> 20 */
+  double   numElectrons;
+  double   smearingTemperature;       /* K */
+  double   fermiEnergyTolerance;
+  double   fracOccupancyTolerance;
 } dfx_ks_params;
 typedef struct dfx_ks_result
 {
@@ -272,6 +299,8 @@ typedef struct dfx_ks_result
   double   a, b, c;                   /* last filter bounds */
   uint32_t degree;
   double   filterMs, orthoMs, rrMs, residualMs, lanczosMs;
+  double   fermiEnergy;
+  uint32_t numOccupied;               /* states with occupancy > tolerance */
 } dfx_ks_result;
 /* residualNorms: host, length numWantedEigenvalues (may be NULL) */
 int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,
@@ -287,6 +316,62 @@ int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,
 int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *b, dfx_mv *x,
                  double absTol, double relTol, double divTol, uint32_t maxIter,
                  uint32_t *iterations, double *residualNorms);
+
+/* ======================================================================
+ * Problem builder: structured hex mesh, FE_Q(p) on GLL points, classical +
+ * Slater-enriched basis, Dirichlet constraints and cell matrices
+ * (H = 1/2 K + V, M, K), built on this GPU.  Stands in for the reference's
+ * deal.II-backed setup (FEBasisManager::reinit, src/basis/FEBasisManager.t.cpp:
+ * 118-260; FEBasisOperations::computeFEMatrices, src/basis/FEBasisOperations.t.cpp:
+ * 41-433; EFE enrichment, src/basis/EnrichmentClassicalInterfaceSpherical.h:76-127).
+ * Numbering: cells lexicographic (x fastest); nodes numbered by first
+ * appearance over cells, cell-local order tensor-lexicographic; enriched dof
+ * of atom a, function k (1s, 2s, 2p_y, 2p_z, 2p_x) is nClassical + 5a + k.
+ * ==================================================================== */
+typedef struct dfx_problem_spec
+{
+  uint32_t      p;
+  uint32_t      n[3];
+  double        lo[3], hi[3];
+  uint32_t      nq;           /* Gauss points per direction, classical cells (0 -> p+1) */
+  uint32_t      nqEnr;        /* Gauss points per direction, enriched cells (0 -> 2(p+1)) */
+  uint32_t      nAtoms;
+  const double *atomXYZ;      /* 3 nAtoms */
+  const int    *atomZ;        /* nAtoms */
+  double        vSmear;       /* V = -sum Z erf(r / vSmear) / r */
+  int           enrich;       /* 1s, 2s, 2p Slater functions per atom */
+  double        rcut, smooth; /* smooth cutoff, support ends at rcut (1 + 1/smooth) */
+  int           orthogonalize;/* L2-orthogonalize enrichment against the classical space */
+} dfx_problem_spec;
+
+#define DFX_PROB_CELL_NDOFS 0  /* uint32 [nCells] */
+#define DFX_PROB_CELL_MAP   1  /* uint32 [sum n_e] */
+#define DFX_PROB_ROWS       2  /* uint32 [nRows] */
+#define DFX_PROB_ROW_SIZES  3  /* uint32 [nRows] */
+#define DFX_PROB_ROW_OFFSETS 4 /* uint32 [nRows] */
+#define DFX_PROB_COLS       5  /* uint32 [nEntries] */
+#define DFX_PROB_WEIGHTS    6  /* double [nEntries] */
+#define DFX_PROB_INHOM      7  /* double [nRows] */
+#define DFX_PROB_H          8  /* double [sum n_e^2] */
+#define DFX_PROB_M          9
+#define DFX_PROB_K          10
+#define DFX_PROB_DINV       11 /* double [nDofs] GLL overlap inverse diagonal */
+#define DFX_PROB_MGLL       12 /* double [nDofs] GLL lumped mass */
+#define DFX_PROB_ENR_MINV   13 /* double [nEnriched^2] (M^ee)^-1, column-major */
+#define DFX_PROB_ENR_M      14 /* double [nEnriched^2] M^ee */
+#define DFX_PROB_ALPHA      15 /* double [nEnriched * nClassical] orthogonalization coefficients */
+
+int dfx_problem_build(dfx_ctx *ctx, const dfx_problem_spec *spec, dfx_problem **out);
+/* sizes[8]: nCells, nDofs, nClassical, nEnriched, sum n_e, sum n_e^2, nRows, nEntries */
+int dfx_problem_sizes(const dfx_problem *prob, uint64_t *sizes);
+int dfx_problem_get(const dfx_problem *prob, int which, void *host);
+/* Operators on the problem's device-resident cell matrices: H (KS), M
+ * (consistent overlap), K (Laplace, for the Poisson CG), MInv (GLL overlap
+ * inverse + enriched block), MLump (Lanczos B).  Any output may be NULL. */
+int dfx_problem_create_operators(dfx_problem *prob, uint32_t flags, dfx_op **H,
+                                 dfx_op **M, dfx_op **K, dfx_op **MInv,
+                                 dfx_op **MLump, dfx_constraints **cons);
+int dfx_problem_destroy(dfx_problem *prob);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,8 @@ extern "C"
   void *orc_build(int p, const int *n, const double *lo, const double *hi,
                   const int *refLo, const int *refHi, int nq, int nqEnr,
                   int nAtoms, const double *atomXYZ, const int *atomZ,
-                  double vSmear, int enrich, double rcut, double smooth)
+                  double vSmear, int enrich, double rcut, double smooth,
+                  int orthogonalize)
   {
     try
       {
@@ -61,6 +62,7 @@ extern "C"
         s.enrich = enrich != 0;
         s.rcut   = rcut;
         s.smooth = smooth;
+        s.orthogonalize = orthogonalize != 0;
         return new Problem(buildProblem(s));
       }
     catch (const std::exception &e)
@@ -204,6 +206,7 @@ extern "C"
         case 8: return cp(P.enrMinv);
         case 9: return cp(P.cellLo);
         case 10: return cp(P.cellH);
+        case 11: return cp(P.alpha);
       }
     return -1;
   }

@@ -273,6 +273,140 @@ namespace oracle
     }
   } // namespace
 
+  // Enrichment orthogonalization (the OrthoEFE construction,
+  // src/basis/EnrichmentClassicalInterfaceSpherical.h:76-97): for each atom,
+  // the classical L2 projection of its enrichment functions onto the nodes
+  // interior to the enrichment support (every incident cell in the support,
+  // unconstrained), M_II alpha = d with d_c = int N_c phi, solved per function
+  // by Jacobi-PCG (the CGLinearSolver iteration, stopped at ||r|| < 1e-13 ||d||).
+  static void orthogonalizeEnrichment(Problem &P,
+                                      const std::vector<std::vector<int>> &cellAtoms,
+                                      int nfun)
+  {
+    const int nl = (P.p + 1) * (P.p + 1) * (P.p + 1);
+    P.alpha.assign((size_t)P.nEnriched * P.nClassical, 0.0);
+    std::vector<u32> inc(P.nClassical, 0);
+    for (u32 e = 0; e < P.nCells; ++e)
+      for (int l = 0; l < nl; ++l)
+        ++inc[P.cellMap[P.cellMapOffset[e] + l]];
+    std::vector<char> constrained(P.nClassical, 0);
+    for (u32 r : P.cons.rows)
+      constrained[r] = 1;
+    const int nAtoms = (int)(P.nEnriched / nfun);
+    for (int ai = 0; ai < nAtoms; ++ai)
+      {
+        std::vector<u32> cells, colStart;
+        for (u32 e = 0; e < P.nCells; ++e)
+          for (size_t q = 0; q < cellAtoms[e].size(); ++q)
+            if (cellAtoms[e][q] == ai)
+              {
+                cells.push_back(e);
+                colStart.push_back((u32)(nl + nfun * q));
+              }
+        std::vector<u32> loc(P.nClassical, UINT32_MAX), glob, cnt;
+        std::vector<u32> lmap(cells.size() * nl);
+        for (size_t i = 0; i < cells.size(); ++i)
+          for (int l = 0; l < nl; ++l)
+            {
+              const u32 g = P.cellMap[P.cellMapOffset[cells[i]] + l];
+              if (loc[g] == UINT32_MAX)
+                {
+                  loc[g] = (u32)glob.size();
+                  glob.push_back(g);
+                  cnt.push_back(0);
+                }
+              ++cnt[loc[g]];
+              lmap[i * nl + l] = loc[g];
+            }
+        const size_t      n = glob.size();
+        std::vector<char> interior(n);
+        for (size_t i = 0; i < n; ++i)
+          interior[i] = cnt[i] == inc[glob[i]] && !constrained[glob[i]];
+        auto Mop = [&](const std::vector<double> &x, std::vector<double> &y) {
+          std::fill(y.begin(), y.end(), 0.0);
+          for (size_t i = 0; i < cells.size(); ++i)
+            {
+              const u32     e  = cells[i];
+              const u32     ne = P.cellNdofs[e];
+              const double *Me = &P.M[P.cellMatOffset[e]];
+              for (int r = 0; r < nl; ++r)
+                {
+                  double acc = 0.0;
+                  for (int c = 0; c < nl; ++c)
+                    acc += Me[r + (size_t)ne * c] * x[lmap[i * nl + c]];
+                  y[lmap[i * nl + r]] += acc;
+                }
+            }
+          for (size_t i = 0; i < n; ++i)
+            if (!interior[i])
+              y[i] = 0.0;
+        };
+        std::vector<double> diag(n, 0.0);
+        for (size_t i = 0; i < cells.size(); ++i)
+          {
+            const u32 e = cells[i], ne = P.cellNdofs[e];
+            for (int l = 0; l < nl; ++l)
+              diag[lmap[i * nl + l]] += P.M[P.cellMatOffset[e] + l + (size_t)ne * l];
+          }
+        for (int k = 0; k < nfun; ++k)
+          {
+            std::vector<double> d(n, 0.0), x(n, 0.0), r(n), z(n), p(n), w(n);
+            for (size_t i = 0; i < cells.size(); ++i)
+              {
+                const u32 e = cells[i], ne = P.cellNdofs[e];
+                for (int l = 0; l < nl; ++l)
+                  d[lmap[i * nl + l]] +=
+                    P.M[P.cellMatOffset[e] + l + (size_t)ne * (colStart[i] + k)];
+              }
+            for (size_t i = 0; i < n; ++i)
+              if (!interior[i])
+                d[i] = 0.0;
+            double bn = 0.0;
+            for (double v : d)
+              bn += v * v;
+            bn = std::sqrt(bn);
+            r  = d;
+            for (size_t i = 0; i < n; ++i)
+              z[i] = interior[i] ? r[i] / diag[i] : r[i];
+            p = z;
+            double zr = 0.0;
+            for (size_t i = 0; i < n; ++i)
+              zr += z[i] * r[i];
+            for (int it = 0; it < 10000; ++it)
+              {
+                double rn = 0.0;
+                for (double v : r)
+                  rn += v * v;
+                if (std::sqrt(rn) < 1e-13 * bn)
+                  break;
+                Mop(p, w);
+                double pw = 0.0;
+                for (size_t i = 0; i < n; ++i)
+                  pw += p[i] * w[i];
+                const double al = zr / pw;
+                for (size_t i = 0; i < n; ++i)
+                  {
+                    x[i] += al * p[i];
+                    r[i] -= al * w[i];
+                  }
+                for (size_t i = 0; i < n; ++i)
+                  z[i] = interior[i] ? r[i] / diag[i] : r[i];
+                double zrn = 0.0;
+                for (size_t i = 0; i < n; ++i)
+                  zrn += z[i] * r[i];
+                const double be = zrn / zr;
+                zr              = zrn;
+                for (size_t i = 0; i < n; ++i)
+                  p[i] = z[i] + be * p[i];
+              }
+            const size_t g = (size_t)ai * nfun + k;
+            for (size_t i = 0; i < n; ++i)
+              if (interior[i])
+                P.alpha[g * P.nClassical + glob[i]] = x[i];
+          }
+      }
+  }
+
   Problem buildProblem(const MeshSpec &S)
   {
     Problem P;
@@ -566,8 +700,11 @@ namespace oracle
     gaussLegendre(nqC, qxC, qwC);
     gaussLegendre(nqE, qxE, qwE);
 
+    // cell matrices of the cells in `todo`; with `alpha`, the enriched
+    // functions are phi_g - sum_l alpha[g][map(l)] N_l (orthogonalized basis)
+    auto computeCells = [&](const std::vector<u32> &todo, const std::vector<double> *alpha) {
     std::vector<double> f, gx, gy, gz, V;
-    for (u32 e = 0; e < P.nCells; ++e)
+    for (u32 e : todo)
       {
         const u32   ne  = P.cellNdofs[e];
         const bool  enr = ne > (u32)nl;
@@ -620,6 +757,18 @@ namespace oracle
                     double    val, grad[3];
                     enrichmentValueGrad(S.atoms[gid / nfun], gid % nfun, S.rcut,
                                         S.smooth, X, val, grad);
+                    if (alpha)
+                      for (int lc = 0; lc < nl; ++lc)
+                        {
+                          const double a = (*alpha)[(size_t)gid * P.nClassical +
+                                                    P.cellMap[P.cellMapOffset[e] + lc]];
+                          if (a == 0.0)
+                            continue;
+                          val -= a * f[(size_t)lc * nQ + q];
+                          grad[0] -= a * gx[(size_t)lc * nQ + q];
+                          grad[1] -= a * gy[(size_t)lc * nQ + q];
+                          grad[2] -= a * gz[(size_t)lc * nQ + q];
+                        }
                     f[(size_t)l * nQ + q]  = val;
                     gx[(size_t)l * nQ + q] = grad[0];
                     gy[(size_t)l * nQ + q] = grad[1];
@@ -647,6 +796,20 @@ namespace oracle
               Me[i + (size_t)ne * j] = m;
               He[i + (size_t)ne * j] = 0.5 * k + v;
             }
+      }
+    };
+    std::vector<u32> allCells(P.nCells);
+    for (u32 e = 0; e < P.nCells; ++e)
+      allCells[e] = e;
+    computeCells(allCells, nullptr);
+    if (P.nEnriched && S.orthogonalize)
+      {
+        orthogonalizeEnrichment(P, cellAtoms, nfun);
+        std::vector<u32> enrCells;
+        for (u32 e = 0; e < P.nCells; ++e)
+          if (P.cellNdofs[e] > (u32)nl)
+            enrCells.push_back(e);
+        computeCells(enrCells, &P.alpha);
       }
 
     // ------------------------------------------------ GLL-lumped inverse

@@ -61,6 +61,7 @@ namespace oracle
     bool   enrich  = false;      // attach 1s,2s,2p Slater enrichment per atom
     double rcut    = 4.0;        // smooth cutoff start
     double smooth  = 2.0;        // support ends at rcut (1 + 1/smooth)
+    bool   orthogonalize = false;// L2-orthogonalize enrichment against the classical space
   };
 
   // Flattened constraints, same arrays as
@@ -85,6 +86,7 @@ namespace oracle
     std::vector<double> mgll;                     // assembled GLL mass diag (raw)
     std::vector<double> dinv;                     // GLL overlap inverse diagonal
     std::vector<double> enrMinv;                  // nEnriched^2 dense (M^ee)^-1, col-major
+    std::vector<double> alpha;                    // nEnriched x nClassical projection coefficients
     std::vector<int>    isHanging, isDirichlet;   // per classical node
   };
 

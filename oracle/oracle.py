@@ -65,6 +65,43 @@ def cheby_degree(upper):
     return 1250
 
 
+K_BOLTZMANN_HARTREE = 3.166811429e-06  # src/ksdft/Defaults.cpp:69
+
+
+def fermi_dirac(e, mu, kb, T):
+    """src/ksdft/FractionalOccupancyFunction.cpp:9-18"""
+    f = (e - mu) / (kb * T)
+    return np.exp(-f) / (1.0 + np.exp(-f)) if f >= 0 else 1.0 / (1.0 + np.exp(f))
+
+
+def fermi_dirac_der(e, mu, kb, T):
+    """src/ksdft/FractionalOccupancyFunction.cpp:20-33"""
+    f = (e - mu) / (kb * T)
+    beta = 1.0 / (kb * T)
+    if f >= 0:
+        return beta * np.exp(-f) / (1.0 + np.exp(-f)) / (1.0 + np.exp(-f))
+    return beta * np.exp(f) / (1.0 + np.exp(f)) / (1.0 + np.exp(f))
+
+
+def fermi_level(eig, ne, T, tol):
+    """NewtonRaphsonSolver on sum_i 2 f_i - Ne (src/linearAlgebra/
+    NewtonRaphsonSolver.t.cpp, FractionalOccupancyFunction.cpp:49-71), started at
+    eig[ceil(Ne/2) - 1] (src/ksdft/KohnShamEigenSolver.t.cpp:396-425)."""
+    i0 = int(np.ceil(ne / 2.0))
+    i0 = min(i0 - 1, len(eig) - 1) if i0 else 0
+    x = float(eig[i0])
+    for _ in range(100000):
+        g = -ne + sum(2 * fermi_dirac(e, x, K_BOLTZMANN_HARTREE, T) for e in eig)
+        gp = sum(2 * fermi_dirac_der(e, x, K_BOLTZMANN_HARTREE, T) for e in eig)
+        if gp == 0.0:
+            break
+        x1 = x - g / gp
+        if abs(x1 - x) < tol:
+            return x1
+        x = x1
+    return x
+
+
 def rng_uniform(seed, a, b, n):
     out = np.empty(n, dtype=np.float64)
     lib().orc_rng_uniform(C.c_uint64(seed), C.c_double(a), C.c_double(b),
@@ -131,7 +168,7 @@ class Problem:
     @classmethod
     def build(cls, p=3, n=(8, 8, 8), lo=(-10., -10., -10.), hi=(10., 10., 10.),
               ref_lo=(0, 0, 0), ref_hi=(0, 0, 0), nq=0, nq_enr=0, atoms=(),
-              v_smear=0.5, enrich=False, rcut=4.0, smooth=2.0):
+              v_smear=0.5, enrich=False, rcut=4.0, smooth=2.0, orthogonalize=False):
         n = np.array(n, dtype=np.int32)
         lo = np.array(lo, dtype=np.float64); hi = np.array(hi, dtype=np.float64)
         rl = np.array(ref_lo, dtype=np.int32); rh = np.array(ref_hi, dtype=np.int32)
@@ -143,7 +180,7 @@ class Problem:
                             C.c_int(nq), C.c_int(nq_enr), C.c_int(len(atoms)),
                             _p(xyz), _p(Z), C.c_double(v_smear),
                             C.c_int(1 if enrich else 0), C.c_double(rcut),
-                            C.c_double(smooth))
+                            C.c_double(smooth), C.c_int(1 if orthogonalize else 0))
         P = cls(h)
         P.p = p
         return P
@@ -202,6 +239,9 @@ class Problem:
         d["mgll"] = self._f64(6, self.nDofs)
         d["enrMinv"] = self._f64(8, self.nEnriched * self.nEnriched)
         return d
+
+    def alpha(self):
+        return self._f64(11, self.nEnriched * self.nClassical).reshape(self.nEnriched, -1)
 
     def node_xyz(self):
         return self._f64(7, 3 * self.nClassical).reshape(-1, 3)
@@ -315,13 +355,15 @@ class Problem:
         return rc, x, it.value
 
     def ks_eigensolve(self, X0, lanczos_guess, degree=0, tol=1e-6, max_pass=10,
-                      residual=True, batch=0, max_krylov=20):
+                      residual=True, batch=0, max_krylov=20, num_electrons=0.0,
+                      temperature=500.0, fermi_tol=1e-10, occ_tol=1e-3):
         """KohnShamEigenSolver::solve (src/ksdft/KohnShamEigenSolver.t.cpp:214-536)
         composed from the oracle's pieces: Lanczos bounds and the upper-bound
         heuristic (:264-297), degree lookup (:36-46), one degree-1 plain pass
         before residual-filter passes (src/ksdft/KohnShamDFT.t.cpp:462-487),
-        then ChFSI passes until every residual norm is within tol (all states
-        counted as occupied)."""
+        then ChFSI passes until every occupied state (Fermi-Dirac occupancy
+        above occ_tol at the Newton-Raphson Fermi level, :396-460) has residual
+        norm within tol.  num_electrons == 0 counts every state as occupied."""
         n = X0.shape[1]
         N = self.nDofs
         lo, hi, beta = self.lanczos(lanczos_guess, max_krylov)
@@ -337,7 +379,7 @@ class Problem:
         if residual:
             rc, eig, X, G = self.chfsi(a, b, c, 1, G, eig, residual=False, batch=batch)
             G = X.copy()
-        passes, res, rc = 0, None, 6
+        passes, res, rc, mu, nocc = 0, None, 6, 0.0, n
         for it in range(max_pass):
             rc, eig, X, G = self.chfsi(a, b, c, m, G, eig, residual=residual, batch=batch)
             res = self.residual_norms(X, eig)
@@ -346,12 +388,19 @@ class Problem:
             if rc != 0:
                 rc = 7
                 break
-            if np.all(res <= tol):
+            if num_electrons > 0:
+                mu = fermi_level(eig, num_electrons, temperature, fermi_tol)
+                occ = np.array([fermi_dirac(e, mu, K_BOLTZMANN_HARTREE, temperature)
+                                for e in eig]) > occ_tol
+            else:
+                occ = np.ones(n, dtype=bool)
+            nocc = int(occ.sum())
+            if np.all(res[occ] <= tol):
                 rc = 0
                 break
             a, b = eig[0], eig[-1]
             rc = 6
-        return dict(rc=rc, eig=eig, X=X, passes=passes, res=res,
+        return dict(rc=rc, eig=eig, X=X, passes=passes, res=res, fermi=mu, nocc=nocc,
                     bounds=(lo, hi, beta), a=a, b=b, c=c, degree=m)
 
     def time_apply(self, kind, X, reps=3, threads=1):

@@ -27,6 +27,15 @@ def enriched(p=2, n=4, L=24.0, nq_enr=6):
                            enrich=True, nq_enr=nq_enr)
 
 
+def enriched_orth(p=2, n=6, L=24.0):
+    """Two atoms with orthogonalized enrichment on a mesh where the supports
+    have interior nodes (the KS driver test case)."""
+    h = L / 2
+    atoms = [(-1.5, 0.5, 0.2, 1), (2.0, -1.0, 0.7, 6)]
+    return O.Problem.build(p=p, n=(n, n, n), lo=(-h,) * 3, hi=(h,) * 3, atoms=atoms,
+                           enrich=True, orthogonalize=True)
+
+
 def guess(P, nvec, seed=1):
     X = O.rng_uniform(seed, -1.0, 1.0, P.nDofs * nvec).reshape(P.nDofs, nvec)
     return P.p2c(X)
