@@ -39,7 +39,11 @@ namespace dfx
   {
     hipStream_t s = op->ctx->stream;
     k::p2c<T>(s, op->cons, X);
-    k::fill<T>(s, Y, 0.0);
+    // every classical row is stored by the first cell that reaches it and every
+    // enriched row by the enriched reduction, so Y is cleared only when some
+    // dof lies in no cell
+    if (op->fillY || (op->nDofs > op->nClassical && !op->nEnrSlots))
+      k::fill<T>(s, Y, 0.0);
     T *yE = nullptr;
     if (op->nEnrSlots)
       {

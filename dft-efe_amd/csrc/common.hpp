@@ -226,6 +226,8 @@ struct dfx_op
   std::vector<dfx::Launch> launches64, launches32;
   dfx::u64 fragLen64 = 0, fragLen32 = 0;
   dfx::DevBuf map, meta, frag64, frag32;
+  dfx::DevBuf first;     // per launch position, 8 words: first-touch bits of classical rows
+  bool fillY  = false;   // some classical dof lies in no cell: Y needs a zero fill
   bool hasF32 = false;
   // enriched partial reduction
   dfx::u32 nEnrSlots = 0;

@@ -59,11 +59,15 @@ namespace dfx
         __device__ static inline int row(int l, int v) { return 4 * (l >> 4) + v; }
       };
 
+      // first[i / 32] bit i % 32: this cell is the first (in launch order) to
+      // touch its classical row i, which is then stored instead of added, so Y
+      // needs no zero fill and the first touch of every dof skips the read.
       template <class T, int NT, int MTW>
       __global__ __launch_bounds__(256) void cellApplyK(
-        const CellMeta *__restrict__ meta, u32 begin, const u32 *__restrict__ map,
-        const T *__restrict__ frag, const T *__restrict__ x, u32 ldx,
-        T *__restrict__ y, u32 ldy, T *__restrict__ yE, u32 B)
+        const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
+        const u32 *__restrict__ map, const T *__restrict__ frag,
+        const T *__restrict__ x, u32 ldx, T *__restrict__ y, u32 ldy,
+        T *__restrict__ yE, u32 B)
       {
         using M                = Mfma<T>;
         using acc_t            = typename M::acc_t;
@@ -71,10 +75,16 @@ namespace dfx
         constexpr int G        = M::G;
         constexpr int KB       = 4 * G;
         constexpr int BC       = 16 * NT;
+        constexpr int VC       = BC / G;              // 16-byte vectors per chunk row
+        constexpr int KMAX     = 64 * MTW;            // largest padded K of this class
+        constexpr int GATHER_IT = (KMAX * VC + 255) / 256;
         extern __shared__ __align__(16) unsigned char smemRaw[];
-        T *sX = reinterpret_cast<T *>(smemRaw);
+        T   *sX    = reinterpret_cast<T *>(smemRaw);
+        __shared__ u32 sMap[256];
+        __shared__ u32 sFirst[8];
 
-        const CellMeta cm   = meta[begin + blockIdx.x];
+        const u32      pos  = begin + blockIdx.x;
+        const CellMeta cm   = meta[pos];
         const int      n    = (int)cm.n;
         const int      nc   = (int)cm.nc;
         const int      mt   = (n + 15) >> 4;
@@ -82,21 +92,53 @@ namespace dfx
         const int      nkb  = kpad / KB;
         const int      w    = threadIdx.x >> 6;
         const int      l    = threadIdx.x & 63;
-        const u32 *cmap     = map + cm.mapOff;
         const T   *cfrag    = frag + (sizeof(T) == 8 ? cm.fragOff64 : cm.fragOff32);
+        if (threadIdx.x < n)
+          sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
+        if (threadIdx.x < 8)
+          sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
+        __syncthreads();
+        const bool vecOk = (ldx % G == 0) && (B % G == 0) &&
+                           ((reinterpret_cast<uintptr_t>(x) & 15) == 0);
 
         for (u32 c0 = 0; c0 < B; c0 += BC)
           {
             const int bw = min(BC, (int)(B - c0));
-            // ---- gather X_e (this column chunk) into LDS, K-grouped by G
-            for (int t = threadIdx.x; t < kpad * BC; t += 256)
+            // ---- gather X_e (this column chunk) into LDS, K-grouped by G:
+            // independent 16-byte loads, all issued before any is consumed
+            if (vecOk)
               {
-                const int kk = t / BC, c = t - kk * BC;
-                T         v  = (T)0;
-                if (kk < n && c < bw)
-                  v = x[(u64)cmap[kk] * ldx + c0 + c];
-                sX[((kk / G) * BC + c) * G + (kk % G)] = v;
+                vec_t v[GATHER_IT];
+#pragma unroll
+                for (int it = 0; it < GATHER_IT; ++it)
+                  {
+                    const int t  = threadIdx.x + it * 256;
+                    const int kk = t / VC, c = (t - kk * VC) * G;
+                    v[it]        = vec_t{};
+                    if (kk < n && c < bw)
+                      v[it] = *reinterpret_cast<const vec_t *>(
+                        x + (u64)sMap[kk] * ldx + c0 + c);
+                  }
+#pragma unroll
+                for (int it = 0; it < GATHER_IT; ++it)
+                  {
+                    const int t  = threadIdx.x + it * 256;
+                    const int kk = t / VC, c = (t - kk * VC) * G;
+                    if (kk < kpad)
+#pragma unroll
+                      for (int g = 0; g < G; ++g)
+                        sX[((kk / G) * BC + c + g) * G + (kk % G)] = v[it][g];
+                  }
               }
+            else
+              for (int t = threadIdx.x; t < kpad * BC; t += 256)
+                {
+                  const int kk = t / BC, c = t - kk * BC;
+                  T         v  = (T)0;
+                  if (kk < n && c < bw)
+                    v = x[(u64)sMap[kk] * ldx + c0 + c];
+                  sX[((kk / G) * BC + c) * G + (kk % G)] = v;
+                }
             __syncthreads();
 
             acc_t acc[MTW][NT];
@@ -146,54 +188,85 @@ namespace dfx
                   a[t] = an[t];
               }
 
-            // ---- scatter: classical rows RMW into y, enriched rows into yE
+            // ---- scatter: classical rows into y (store on first touch, else
+            // read-modify-write; a tile's reads are all issued before its
+            // stores), enriched rows into yE
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
               {
-                const int mtile = w + 4 * t;
-                if (mtile >= mt)
+                if (w + 4 * t >= mt)
                   continue;
+                T old[NT][4];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                  {
-                    const int c = nt * 16 + (l & 15);
-                    if (c >= bw)
-                      continue;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                      {
-                        const int row = mtile * 16 + M::row(l, v);
-                        if (row >= n)
-                          continue;
-                        const T val = acc[t][nt][v];
-                        if (row < nc)
-                          {
-                            T *dst = y + (u64)cmap[row] * ldy + c0 + c;
-                            *dst += val;
-                          }
-                        else
-                          yE[(u64)(cm.enrOff + (u32)(row - nc)) * B + c0 + c] = val;
-                      }
-                  }
+                  for (int v = 0; v < 4; ++v)
+                    {
+                      const int row = (w + 4 * t) * 16 + M::row(l, v);
+                      const int c   = nt * 16 + (l & 15);
+                      old[nt][v]    = (T)0;
+                      if (c < bw && row < nc && !((sFirst[row >> 5] >> (row & 31)) & 1u))
+                        old[nt][v] = y[(u64)sMap[row] * ldy + c0 + c];
+                    }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                  for (int v = 0; v < 4; ++v)
+                    {
+                      const int row = (w + 4 * t) * 16 + M::row(l, v);
+                      const int c   = nt * 16 + (l & 15);
+                      if (c >= bw || row >= n)
+                        continue;
+                      if (row < nc)
+                        y[(u64)sMap[row] * ldy + c0 + c] = old[nt][v] + acc[t][nt][v];
+                      else
+                        yE[(u64)(cm.enrOff + (u32)(row - nc)) * B + c0 + c] = acc[t][nt][v];
+                    }
               }
             __syncthreads();
           }
       }
 
-      // y[nC + g, j] = sum_{slots of g, cell order} yE[slot, j]
+      // y[nC + g, j] = sum over the cells of g of yE[slot, j].  One block per
+      // (enriched dof g, 16 columns); 16 slot lanes take every 16th slot and
+      // keep 8 loads in flight; the 16 lane sums are combined in a fixed
+      // order, so the result does not depend on timing.
       template <class T>
-      __global__ void enrReduceK(const u32 *off, const u32 *slots, u32 nE, u32 nC,
-                                 const T *yE, u32 B, T *y, u32 ldy)
+      __global__ __launch_bounds__(256) void enrReduceK(const u32 *__restrict__ off,
+                                                        const u32 *__restrict__ slots,
+                                                        u32 nC, const T *__restrict__ yE,
+                                                        u32 B, T *__restrict__ y, u32 ldy)
       {
-        const u64 n = (u64)nE * B;
-        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
-             t += (u64)gridDim.x * blockDim.x)
+        constexpr int CB = 16, SL = 16, U = 8;
+        __shared__ T part[SL][CB];
+        const u32 g  = blockIdx.x;
+        const int cj = threadIdx.x % CB, sl = threadIdx.x / CB;
+        const u32 j  = blockIdx.y * CB + cj;
+        const u32 e0 = off[g], e1 = off[g + 1];
+        T         acc = 0;
+        if (j < B)
+          for (u32 e = e0 + sl; e < e1; e += SL * U)
+            {
+              T v[U];
+#pragma unroll
+              for (int u = 0; u < U; ++u)
+                {
+                  const u32 ee = e + u * SL;
+                  v[u]         = ee < e1 ? yE[(u64)slots[ee] * B + j] : (T)0;
+                }
+#pragma unroll
+              for (int u = 0; u < U; ++u)
+                acc += v[u];
+            }
+        part[sl][cj] = acc;
+        __syncthreads();
+        if (sl == 0 && j < B)
           {
-            const u32 g = (u32)(t / B), j = (u32)(t - (u64)g * B);
-            T         acc = 0;
-            for (u32 e = off[g]; e < off[g + 1]; ++e)
-              acc += yE[(u64)slots[e] * B + j];
-            y[(u64)(nC + g) * ldy + j] = acc;
+            T s = 0;
+#pragma unroll
+            for (int q = 0; q < SL; ++q)
+              s += part[q][cj];
+            y[(u64)(nC + g) * ldy + j] = s;
           }
       }
 
@@ -232,9 +305,10 @@ namespace dfx
         constexpr int G  = Mfma<T>::G;
         constexpr int BC = 16 * NT;
         (void)G;
+        // dynamic tile plus the static map / first-touch words (about 1 KiB)
         const size_t lds = (size_t)L.kpadMax * BC * sizeof(T);
-        DFX_REQUIRE(lds <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
-        static size_t attrSet = 64 * 1024;
+        DFX_REQUIRE(lds + 2048 <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
+        static size_t attrSet = 60 * 1024;
         if (lds > attrSet)
           {
             DFX_HIP(hipFuncSetAttribute(
@@ -243,8 +317,8 @@ namespace dfx
             attrSet = lds;
           }
         cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
-          op->meta.as<CellMeta>(), L.begin, op->map.as<u32>(), frag, x.p, x.ld, y.p,
-          y.ld, yE, x.B);
+          op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
+          x.p, x.ld, y.p, y.ld, yE, x.B);
         checkLaunch("cellApply");
       }
 
@@ -263,6 +337,23 @@ namespace dfx
       }
     } // namespace
 
+    // cap on the column-chunk width from DFX_CELL_NT_F64 / DFX_CELL_NT_F32
+    // (1, 2, 4 or 8; 0 or unset = built-in choice), read once
+    static int ntOverride(bool f64)
+    {
+      static const int v[2] = {
+        [] {
+          const char *e = getenv("DFX_CELL_NT_F32");
+          return e ? atoi(e) : 0;
+        }(),
+        [] {
+          const char *e = getenv("DFX_CELL_NT_F64");
+          return e ? atoi(e) : 0;
+        }()};
+      const int x = v[f64 ? 1 : 0];
+      return (x == 1 || x == 2 || x == 4 || x == 8) ? x : 0;
+    }
+
     // column-chunk width (in 16-column tiles) for a launch
     template <class T>
     static int chooseNT(u32 B, u32 kpadMax)
@@ -273,6 +364,8 @@ namespace dfx
         cap = kpadMax <= 128 ? 4 : 2;
       else
         cap = kpadMax <= 128 ? 8 : 4;
+      if (const int o = ntOverride(sizeof(T) == 8))
+        cap = o;
       if (nt > cap)
         nt = cap;
       if (nt == 3)
@@ -303,11 +396,10 @@ namespace dfx
       const u32 nE = op->nDofs - op->nClassical;
       if (!nE || !y.B)
         return;
-      const u64 n = (u64)nE * y.B;
-      u64       g = (n + 255) / 256;
-      enrReduceK<T><<<(unsigned)(g > 65536 ? 65536 : g), 256, 0, s>>>(
-        op->enrSlotOffsets.as<u32>(), op->enrSlots.as<u32>(), nE, op->nClassical, yE,
-        y.B, y.p, y.ld);
+      DFX_REQUIRE((y.B + 15) / 16 <= 65535, "too many columns for the enriched reduction");
+      enrReduceK<T><<<dim3(nE, (y.B + 15) / 16), 256, 0, s>>>(
+        op->enrSlotOffsets.as<u32>(), op->enrSlots.as<u32>(), op->nClassical, yE, y.B,
+        y.p, y.ld);
       checkLaunch("enrichedReduce");
     }
 

@@ -100,6 +100,16 @@ namespace dfx
           }
         nc[e] = cnt;
       }
+    {
+      // a dof may appear once per cell (two lanes would update one row)
+      std::vector<u32> last(nDofs, 0xffffffffu);
+      for (u32 e = 0; e < nCells; ++e)
+        for (u32 s = op->hMapOffset[e]; s < op->hMapOffset[e + 1]; ++s)
+          {
+            DFX_REQUIRE(last[map[s]] != e, "a dof appears twice in one cell's map");
+            last[map[s]] = e;
+          }
+    }
 
     // ---- greedy colouring over shared classical dofs (cell order)
     std::vector<u32> dofCellOff(nClassical + 1, 0), dofCells;
@@ -200,6 +210,27 @@ namespace dfx
         q = r;
       }
 
+    // first-touch bits in launch order: the first cell to reach a classical
+    // dof stores its contribution, later cells add to it
+    std::vector<u32> hFirst((size_t)nCells * 8, 0u);
+    {
+      std::vector<unsigned char> seen(nClassical, 0);
+      for (u32 q = 0; q < nCells; ++q)
+        {
+          const u32 e = op->hOrder[q];
+          for (u32 i = 0; i < nc[e]; ++i)
+            {
+              const u32 d = map[op->hMapOffset[e] + i];
+              if (!seen[d])
+                {
+                  seen[d] = 1;
+                  hFirst[(size_t)q * 8 + i / 32] |= 1u << (i % 32);
+                }
+            }
+        }
+      op->fillY = std::find(seen.begin(), seen.end(), 0) != seen.end();
+    }
+
     // enriched dof -> slots (cell order)
     const u32 nE = nDofs - nClassical;
     std::vector<u32> eOff(nE + 1, 0), eSlots(op->nEnrSlots);
@@ -230,6 +261,7 @@ namespace dfx
     hipStream_t st = ctx->stream;
     upload(op->map, map, sumNe, st);
     upload(op->meta, op->hMeta, st);
+    upload(op->first, hFirst, st);
     upload(op->enrSlotOffsets, eOff, st);
     upload(op->enrSlots, eSlots, st);
     upload(op->dofSlotOffsets, sOff, st);
