@@ -18,6 +18,20 @@
 // A_e^T[mt*16 + l%16][kb*4G + G*(l/16) + s] = A_e(k, j) (column-major input).
 #include "kernels.hpp"
 
+// Build-time tuning of the cell kernel (defaults measured on MI355X):
+// A-operand prefetch depth in K blocks, whether the read-modify-write loads of
+// Y are issued before the MFMA loop, and the minimum waves per SIMD the
+// register allocation must allow.
+#ifndef DFX_CELL_DEPTH
+#define DFX_CELL_DEPTH 1
+#endif
+#ifndef DFX_CELL_EARLY
+#define DFX_CELL_EARLY 1
+#endif
+#ifndef DFX_CELL_MINW
+#define DFX_CELL_MINW 2
+#endif
+
 namespace dfx
 {
   namespace k
@@ -59,27 +73,280 @@ namespace dfx
         __device__ static inline int row(int l, int v) { return 4 * (l >> 4) + v; }
       };
 
+      // ---- raw buffer access (32-bit byte offsets, hardware range check: a
+      // load at an offset past the buffer returns 0 and such a store is
+      // dropped, so masked lanes need no branch)
+      typedef __amdgpu_buffer_rsrc_t rsrc_t;
+      typedef unsigned u2_t __attribute__((ext_vector_type(2)));
+      typedef unsigned u4_t __attribute__((ext_vector_type(4)));
+      constexpr u32 OFF_NONE = 0x80000000u; // past every buffer (all < 2^31 bytes)
+
+      __device__ inline rsrc_t rsrc(const void *p, u32 bytes)
+      {
+        // third dword 0x00020000: raw (unswizzled) 32-bit data, gfx9 family
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                                 (int)bytes, 0x00020000);
+      }
+
+      template <class T>
+      struct Buf;
+      template <>
+      struct Buf<float>
+      {
+        __device__ static f4_t load4(rsrc_t r, u32 off)
+        {
+          return __builtin_bit_cast(f4_t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        }
+        __device__ static float load(rsrc_t r, u32 off)
+        {
+          return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+        }
+        __device__ static void store(float v, rsrc_t r, u32 off)
+        {
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+        }
+      };
+      template <>
+      struct Buf<double>
+      {
+        __device__ static d2_t load4(rsrc_t r, u32 off)
+        {
+          return __builtin_bit_cast(d2_t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        }
+        __device__ static double load(rsrc_t r, u32 off)
+        {
+          return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+        }
+        __device__ static void store(double v, rsrc_t r, u32 off)
+        {
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2_t, v), r, off, 0, 0);
+        }
+      };
+
+      // The cell kernel on raw buffers (x, y and yE each under 2 GiB): the same
+      // data flow as cellApplyK below with no per-lane branches, since every
+      // masked load or store goes to OFF_NONE.  The LDS tile is KMAX rows
+      // (the class maximum) so the gather writes every row it loads.
+      template <class T, int NT, int MTW>
+      __global__ __launch_bounds__(256, DFX_CELL_MINW) void cellApplyBufK(
+        const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
+        const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 xBytes,
+        u32 ldx, T *y, u32 yBytes, u32 ldy, T *yE, u32 yEBytes, u32 B)
+      {
+        using M                 = Mfma<T>;
+        using BF                = Buf<T>;
+        using acc_t             = typename M::acc_t;
+        using vec_t             = typename M::vec_t;
+        constexpr int G         = M::G;
+        constexpr int KB        = 4 * G;
+        constexpr int BC        = 16 * NT;
+        constexpr int VC        = BC / G;
+        constexpr int KMAX      = 64 * MTW;
+        constexpr int D         = (KMAX / KB) < DFX_CELL_DEPTH ? (KMAX / KB) : DFX_CELL_DEPTH;
+        constexpr int GATHER_IT = KMAX * VC / 256;
+        constexpr bool EARLY    = DFX_CELL_EARLY && MTW * NT * 4 * (int)sizeof(T) <= 128;
+        constexpr u32 S         = sizeof(T);
+        static_assert(KMAX * VC % 256 == 0, "gather must cover the tile exactly");
+        extern __shared__ __align__(16) unsigned char smemRaw[];
+        T *sX = reinterpret_cast<T *>(smemRaw);
+        __shared__ u32 sMap[256];
+        __shared__ u32 sFirst[8];
+
+        const u32      pos  = begin + blockIdx.x;
+        const CellMeta cm   = meta[pos];
+        const int      n    = (int)cm.n;
+        const int      nc   = (int)cm.nc;
+        const int      mt   = (n + 15) >> 4;
+        const int      nkb  = (n + KB - 1) / KB;
+        const int      w    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+        const int      l    = threadIdx.x & 63;
+        const rsrc_t   rA   = rsrc(frag + (S == 8 ? cm.fragOff64 : cm.fragOff32),
+                                   (u32)(mt * nkb * 64 * G) * S);
+        const rsrc_t   rX   = rsrc(x, xBytes);
+        const rsrc_t   rY   = rsrc(y, yBytes);
+        const rsrc_t   rE   = rsrc(yE, yEBytes);
+
+        auto loadA = [&](int kb, vec_t *dst) {
+#pragma unroll
+          for (int t = 0; t < MTW; ++t)
+            {
+              const int mtile = w + 4 * t;
+              const u32 off   = (mtile < mt && kb < nkb)
+                                  ? ((u32)(mtile * nkb + kb) * 64 + l) * (G * S)
+                                  : OFF_NONE;
+              dst[t] = BF::load4(rA, off);
+            }
+        };
+        vec_t ring[D][MTW];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          loadA(d, ring[d]);
+
+        if (threadIdx.x < n)
+          sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
+        if (threadIdx.x < 8)
+          sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
+        __syncthreads();
+
+        const int cl = l & 15;
+        auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(l, v); };
+
+        for (u32 c0 = 0; c0 < B; c0 += BC)
+          {
+            const int bw = min(BC, (int)(B - c0));
+            // ---- gather (columns are whole 16-byte vectors: B % G == 0)
+            vec_t v[GATHER_IT];
+#pragma unroll
+            for (int i = 0; i < GATHER_IT; ++i)
+              {
+                const int t  = threadIdx.x + i * 256;
+                const int kk = t / VC, c = (t - kk * VC) * G;
+                const u32 m  = sMap[kk]; // kk < KMAX <= 256: always in the array
+                v[i] = BF::load4(rX, ((kk < n) & (c < bw)) ? (m * ldx + c0 + c) * S : OFF_NONE);
+              }
+#pragma unroll
+            for (int i = 0; i < GATHER_IT; ++i)
+              {
+                const int t  = threadIdx.x + i * 256;
+                const int kk = t / VC, c = (t - kk * VC) * G;
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                  sX[((kk / G) * BC + c + g) * G + (kk % G)] = v[i][g];
+              }
+
+            // ---- old values of the classical rows touched before
+            T    old[EARLY ? MTW : 1][EARLY ? NT : 1][4];
+            auto loadOld = [&](int t, T (*o)[4]) {
+#pragma unroll
+              for (int vv = 0; vv < 4; ++vv)
+                {
+                  // row < 64 MTW <= 256: sMap / sFirst reads stay in the arrays
+                  const int  row  = rowOf(t, vv);
+                  const u32  fb   = sFirst[row >> 5], m = sMap[row];
+                  const bool rmw  = (row < nc) & !((fb >> (row & 31)) & 1u);
+                  const u32  base = (m * ldy + c0 + cl) * S;
+#pragma unroll
+                  for (int nt = 0; nt < NT; ++nt)
+                    o[nt][vv] = BF::load(rY, (rmw & (nt * 16 + cl < bw)) ? base + nt * 16 * S
+                                                                        : OFF_NONE);
+                }
+            };
+            if constexpr (EARLY)
+              {
+#pragma unroll
+                for (int t = 0; t < MTW; ++t)
+                  if (w + 4 * t < mt)
+                    loadOld(t, old[t]);
+              }
+            __syncthreads();
+
+            acc_t acc[MTW][NT];
+#pragma unroll
+            for (int t = 0; t < MTW; ++t)
+#pragma unroll
+              for (int nt = 0; nt < NT; ++nt)
+                acc[t][nt] = acc_t{0, 0, 0, 0};
+
+            for (int kb = 0; kb < nkb; ++kb)
+              {
+                vec_t b[NT];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                  b[nt] = *reinterpret_cast<const vec_t *>(
+                    sX + (((kb * 4 + (l >> 4)) * BC) + nt * 16 + cl) * G);
+                vec_t a[MTW];
+#pragma unroll
+                for (int t = 0; t < MTW; ++t)
+                  a[t] = ring[0][t];
+#pragma unroll
+                for (int d = 0; d + 1 < D; ++d)
+#pragma unroll
+                  for (int t = 0; t < MTW; ++t)
+                    ring[d][t] = ring[d + 1][t];
+                loadA(kb + D, ring[D - 1]);
+#pragma unroll
+                for (int s = 0; s < G; ++s)
+#pragma unroll
+                  for (int t = 0; t < MTW; ++t)
+                    if (w + 4 * t < mt)
+#pragma unroll
+                      for (int nt = 0; nt < NT; ++nt)
+                        acc[t][nt] = M::op(a[t][s], b[nt][s], acc[t][nt]);
+              }
+            if (c0 + BC < B)
+#pragma unroll
+              for (int d = 0; d < D; ++d)
+                loadA(d, ring[d]);
+
+            // ---- scatter: classical rows to y (stored on first touch, else
+            // added to the old value), enriched rows to yE
+#pragma unroll
+            for (int t = 0; t < MTW; ++t)
+              {
+                if (w + 4 * t >= mt)
+                  continue;
+                T  oldLate[EARLY ? 1 : NT][4];
+                T(*o)[4] = EARLY ? old[EARLY ? t : 0] : oldLate;
+                if constexpr (!EARLY)
+                  loadOld(t, oldLate);
+                const bool enr = (w + 4 * t) * 16 + 15 >= nc; // wave-uniform
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv)
+                  {
+                    const int row = rowOf(t, vv);
+                    const u32 yb  = (sMap[row] * ldy + c0 + cl) * S;
+                    const u32 eb  = ((cm.enrOff + (u32)(row - nc)) * B + c0 + cl) * S;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                      {
+                        const bool colOk = nt * 16 + cl < bw;
+                        BF::store(o[nt][vv] + acc[t][nt][vv], rY,
+                                  ((row < nc) & colOk) ? yb + nt * 16 * S : OFF_NONE);
+                        if (enr)
+                          BF::store(acc[t][nt][vv], rE,
+                                    ((row >= nc) & (row < n) & colOk) ? eb + nt * 16 * S
+                                                                      : OFF_NONE);
+                      }
+                  }
+              }
+            __syncthreads();
+          }
+      }
+
+      // One block (4 waves) per cell.  Wave w owns the M tiles w, w + 4, ...
+      // (MTW of them) and all NT 16-column tiles of a column chunk.
+      //
+      // Latency hiding inside the block: the first D K blocks of the cell's A
+      // fragments are requested right after the cell's metadata, before the
+      // map and the gather; during the MFMA loop A runs D K blocks ahead; the
+      // old values of the classical rows of Y (read-modify-write) are
+      // requested before the MFMA loop when they fit in registers (EARLY).
+      //
       // first[i / 32] bit i % 32: this cell is the first (in launch order) to
       // touch its classical row i, which is then stored instead of added, so Y
       // needs no zero fill and the first touch of every dof skips the read.
       template <class T, int NT, int MTW>
-      __global__ __launch_bounds__(256) void cellApplyK(
+      __global__ __launch_bounds__(256, DFX_CELL_MINW) void cellApplyK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
         const u32 *__restrict__ map, const T *__restrict__ frag,
         const T *__restrict__ x, u32 ldx, T *__restrict__ y, u32 ldy,
         T *__restrict__ yE, u32 B)
       {
-        using M                = Mfma<T>;
-        using acc_t            = typename M::acc_t;
-        using vec_t            = typename M::vec_t;
-        constexpr int G        = M::G;
-        constexpr int KB       = 4 * G;
-        constexpr int BC       = 16 * NT;
-        constexpr int VC       = BC / G;              // 16-byte vectors per chunk row
-        constexpr int KMAX     = 64 * MTW;            // largest padded K of this class
+        using M                 = Mfma<T>;
+        using acc_t             = typename M::acc_t;
+        using vec_t             = typename M::vec_t;
+        constexpr int G         = M::G;
+        constexpr int KB        = 4 * G;
+        constexpr int BC        = 16 * NT;
+        constexpr int VC        = BC / G;             // 16-byte vectors per chunk row
+        constexpr int KMAX      = 64 * MTW;           // largest padded K of this class
+        constexpr int NKB       = KMAX / KB;          // largest K-block count
+        constexpr int D = NKB < DFX_CELL_DEPTH ? NKB : DFX_CELL_DEPTH; // A prefetch depth
         constexpr int GATHER_IT = (KMAX * VC + 255) / 256;
+        constexpr int GATHER_BATCH = GATHER_IT < 8 ? GATHER_IT : 8; // loads in flight
+        constexpr bool EARLY    = DFX_CELL_EARLY && MTW * NT * 4 * (int)sizeof(T) <= 128;
         extern __shared__ __align__(16) unsigned char smemRaw[];
-        T   *sX    = reinterpret_cast<T *>(smemRaw);
+        T *sX = reinterpret_cast<T *>(smemRaw);
         __shared__ u32 sMap[256];
         __shared__ u32 sFirst[8];
 
@@ -90,9 +357,26 @@ namespace dfx
         const int      mt   = (n + 15) >> 4;
         const int      kpad = ((n + KB - 1) / KB) * KB;
         const int      nkb  = kpad / KB;
-        const int      w    = threadIdx.x >> 6;
+        const int      w    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
         const int      l    = threadIdx.x & 63;
-        const T   *cfrag    = frag + (sizeof(T) == 8 ? cm.fragOff64 : cm.fragOff32);
+        const T       *cfrag = frag + (sizeof(T) == 8 ? cm.fragOff64 : cm.fragOff32);
+
+        auto loadA = [&](int kb, vec_t *dst) {
+#pragma unroll
+          for (int t = 0; t < MTW; ++t)
+            {
+              const int mtile = w + 4 * t;
+              dst[t]          = vec_t{};
+              if (mtile < mt && kb < nkb)
+                dst[t] = *reinterpret_cast<const vec_t *>(
+                  cfrag + ((u64)(mtile * nkb + kb) * 64 + l) * G);
+            }
+        };
+        vec_t ring[D][MTW];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          loadA(d, ring[d]);
+
         if (threadIdx.x < n)
           sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
         if (threadIdx.x < 8)
@@ -108,26 +392,30 @@ namespace dfx
             // independent 16-byte loads, all issued before any is consumed
             if (vecOk)
               {
-                vec_t v[GATHER_IT];
 #pragma unroll
-                for (int it = 0; it < GATHER_IT; ++it)
+                for (int i0 = 0; i0 < GATHER_IT; i0 += GATHER_BATCH)
                   {
-                    const int t  = threadIdx.x + it * 256;
-                    const int kk = t / VC, c = (t - kk * VC) * G;
-                    v[it]        = vec_t{};
-                    if (kk < n && c < bw)
-                      v[it] = *reinterpret_cast<const vec_t *>(
-                        x + (u64)sMap[kk] * ldx + c0 + c);
-                  }
+                    vec_t v[GATHER_BATCH];
 #pragma unroll
-                for (int it = 0; it < GATHER_IT; ++it)
-                  {
-                    const int t  = threadIdx.x + it * 256;
-                    const int kk = t / VC, c = (t - kk * VC) * G;
-                    if (kk < kpad)
+                    for (int i = 0; i < GATHER_BATCH; ++i)
+                      {
+                        const int t  = threadIdx.x + (i0 + i) * 256;
+                        const int kk = t / VC, c = (t - kk * VC) * G;
+                        v[i]         = vec_t{};
+                        if (i0 + i < GATHER_IT && kk < n && c < bw)
+                          v[i] = *reinterpret_cast<const vec_t *>(
+                            x + (u64)sMap[kk] * ldx + c0 + c);
+                      }
 #pragma unroll
-                      for (int g = 0; g < G; ++g)
-                        sX[((kk / G) * BC + c + g) * G + (kk % G)] = v[it][g];
+                    for (int i = 0; i < GATHER_BATCH; ++i)
+                      {
+                        const int t  = threadIdx.x + (i0 + i) * 256;
+                        const int kk = t / VC, c = (t - kk * VC) * G;
+                        if (i0 + i < GATHER_IT && kk < kpad)
+#pragma unroll
+                          for (int g = 0; g < G; ++g)
+                            sX[((kk / G) * BC + c + g) * G + (kk % G)] = v[i][g];
+                      }
                   }
               }
             else
@@ -139,6 +427,31 @@ namespace dfx
                     v = x[(u64)sMap[kk] * ldx + c0 + c];
                   sX[((kk / G) * BC + c) * G + (kk % G)] = v;
                 }
+
+            // ---- old values of the classical rows (read-modify-write).  Row
+            // (t, v) of this lane is dof sMap[row]; its NT columns are 16 apart.
+            T   old[EARLY ? MTW : 1][EARLY ? NT : 1][4];
+            const int cl = (l & 15);
+            auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(l, v); };
+            auto loadOld = [&](int t, T (*o)[4]) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                {
+                  const int row  = rowOf(t, v);
+                  const bool rmw = w + 4 * t < mt && row < nc &&
+                                   !((sFirst[row >> 5] >> (row & 31)) & 1u);
+                  const T *src = y + (u64)sMap[rmw ? row : 0] * ldy + c0 + cl;
+#pragma unroll
+                  for (int nt = 0; nt < NT; ++nt)
+                    o[nt][v] = (rmw && nt * 16 + cl < bw) ? src[nt * 16] : (T)0;
+                }
+            };
+            if constexpr (EARLY)
+              {
+#pragma unroll
+                for (int t = 0; t < MTW; ++t)
+                  loadOld(t, old[t]);
+              }
             __syncthreads();
 
             acc_t acc[MTW][NT];
@@ -148,33 +461,25 @@ namespace dfx
               for (int nt = 0; nt < NT; ++nt)
                 acc[t][nt] = acc_t{0, 0, 0, 0};
 
-            vec_t a[MTW], an[MTW];
-#pragma unroll
-            for (int t = 0; t < MTW; ++t)
-              {
-                const int mtile = w + 4 * t;
-                a[t]            = vec_t{};
-                if (mtile < mt)
-                  a[t] = *reinterpret_cast<const vec_t *>(
-                    cfrag + ((u64)(mtile * nkb) * 64 + l) * G);
-              }
+            // ring[0] holds K block kb; ring[d] block kb + d (a rotating
+            // register window, so every index is static)
             for (int kb = 0; kb < nkb; ++kb)
               {
-                // prefetch the next K block of A
-#pragma unroll
-                for (int t = 0; t < MTW; ++t)
-                  {
-                    const int mtile = w + 4 * t;
-                    an[t]           = vec_t{};
-                    if (mtile < mt && kb + 1 < nkb)
-                      an[t] = *reinterpret_cast<const vec_t *>(
-                        cfrag + ((u64)(mtile * nkb + kb + 1) * 64 + l) * G);
-                  }
                 vec_t b[NT];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
                     sX + (((kb * 4 + (l >> 4)) * BC) + nt * 16 + (l & 15)) * G);
+                vec_t a[MTW];
+#pragma unroll
+                for (int t = 0; t < MTW; ++t)
+                  a[t] = ring[0][t];
+#pragma unroll
+                for (int d = 0; d + 1 < D; ++d)
+#pragma unroll
+                  for (int t = 0; t < MTW; ++t)
+                    ring[d][t] = ring[d + 1][t];
+                loadA(kb + D, ring[D - 1]);
 #pragma unroll
                 for (int s = 0; s < G; ++s)
 #pragma unroll
@@ -183,45 +488,37 @@ namespace dfx
 #pragma unroll
                       for (int nt = 0; nt < NT; ++nt)
                         acc[t][nt] = M::op(a[t][s], b[nt][s], acc[t][nt]);
-#pragma unroll
-                for (int t = 0; t < MTW; ++t)
-                  a[t] = an[t];
               }
+            // the next column chunk starts again at K block 0
+            if (c0 + BC < B)
+#pragma unroll
+              for (int d = 0; d < D; ++d)
+                loadA(d, ring[d]);
 
             // ---- scatter: classical rows into y (store on first touch, else
-            // read-modify-write; a tile's reads are all issued before its
-            // stores), enriched rows into yE
+            // add to the old value), enriched rows into yE
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
               {
                 if (w + 4 * t >= mt)
                   continue;
-                T old[NT][4];
+                T  oldLate[EARLY ? 1 : NT][4];
+                T(*o)[4] = EARLY ? old[EARLY ? t : 0] : oldLate;
+                if constexpr (!EARLY)
+                  loadOld(t, oldLate);
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
+                for (int v = 0; v < 4; ++v)
+                  {
+                    const int row = rowOf(t, v);
+                    if (row >= n)
+                      continue;
+                    T *dst = row < nc ? y + (u64)sMap[row] * ldy + c0 + cl
+                                      : yE + (u64)(cm.enrOff + (u32)(row - nc)) * B + c0 + cl;
 #pragma unroll
-                  for (int v = 0; v < 4; ++v)
-                    {
-                      const int row = (w + 4 * t) * 16 + M::row(l, v);
-                      const int c   = nt * 16 + (l & 15);
-                      old[nt][v]    = (T)0;
-                      if (c < bw && row < nc && !((sFirst[row >> 5] >> (row & 31)) & 1u))
-                        old[nt][v] = y[(u64)sMap[row] * ldy + c0 + c];
-                    }
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-                  for (int v = 0; v < 4; ++v)
-                    {
-                      const int row = (w + 4 * t) * 16 + M::row(l, v);
-                      const int c   = nt * 16 + (l & 15);
-                      if (c >= bw || row >= n)
-                        continue;
-                      if (row < nc)
-                        y[(u64)sMap[row] * ldy + c0 + c] = old[nt][v] + acc[t][nt][v];
-                      else
-                        yE[(u64)(cm.enrOff + (u32)(row - nc)) * B + c0 + c] = acc[t][nt][v];
-                    }
+                    for (int nt = 0; nt < NT; ++nt)
+                      if (nt * 16 + cl < bw)
+                        dst[nt * 16] = (row < nc ? o[nt][v] : (T)0) + acc[t][nt][v];
+                  }
               }
             __syncthreads();
           }
@@ -304,21 +601,32 @@ namespace dfx
       {
         constexpr int G  = Mfma<T>::G;
         constexpr int BC = 16 * NT;
-        (void)G;
+        const u64     S  = sizeof(T);
+        const u64 xBytes = (u64)x.N * x.ld * S, yBytes = (u64)y.N * y.ld * S;
+        const u64 eBytes = (u64)op->nEnrSlots * x.B * S;
+        const bool buf   = xBytes < (1ull << 31) && yBytes < (1ull << 31) &&
+                         eBytes < (1ull << 31) && x.B % G == 0 && x.ld % G == 0 &&
+                         (reinterpret_cast<uintptr_t>(x.p) & 15) == 0;
         // dynamic tile plus the static map / first-touch words (about 1 KiB)
-        const size_t lds = (size_t)L.kpadMax * BC * sizeof(T);
+        const size_t lds = (size_t)(buf ? 64 * MTW : L.kpadMax) * BC * S;
         DFX_REQUIRE(lds + 2048 <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
-        static size_t attrSet = 60 * 1024;
-        if (lds > attrSet)
+        const void *fn = buf ? reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW>)
+                             : reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>);
+        static size_t attrSet[2] = {60 * 1024, 60 * 1024};
+        if (lds > attrSet[buf])
           {
-            DFX_HIP(hipFuncSetAttribute(
-              reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>),
-              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            attrSet = lds;
+            DFX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+            attrSet[buf] = lds;
           }
-        cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
-          op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
-          x.p, x.ld, y.p, y.ld, yE, x.B);
+        if (buf)
+          cellApplyBufK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
+            op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
+            x.p, (u32)xBytes, x.ld, y.p, (u32)yBytes, y.ld, yE, (u32)eBytes, x.B);
+        else
+          cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
+            op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
+            x.p, x.ld, y.p, y.ld, yE, x.B);
         checkLaunch("cellApply");
       }
 

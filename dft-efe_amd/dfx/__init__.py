@@ -20,7 +20,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdfx.so")
+# DFX_LIBRARY names another build of libdfx.so (e.g. one compiled with other
+# tuning macros); the default is the in-tree build next to this file.
+LIB_PATH = os.environ.get("DFX_LIBRARY") or os.path.join(_HERE, "libdfx.so")
 
 F64, F32 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1

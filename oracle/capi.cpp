@@ -345,70 +345,4 @@ extern "C"
                      absTol, relTol, divTol, maxIter, iters);
     });
   }
-
-  // CPU baseline: the reference's per-cell apply (gather, per-cell GEMM,
-  // scatter, constraints) run `reps` times over the problem, with T threads
-  // over contiguous cell slabs (thread-private Y + reduction, the stand-in
-  // for MPI domain decomposition + accumulateAddLocallyOwned).  Returns the
-  // median seconds per apply.
-  double orc_time_apply(void *h, int kind, double *X, double *Y, uint32_t B,
-                        int reps, int threads)
-  {
-    Problem &P = *static_cast<Problem *>(h);
-    std::vector<double> t;
-    const double *mats = kind == OP_M ? P.M.data() : (kind == OP_K ? P.K.data() : P.H.data());
-    for (int r = 0; r < reps + 1; ++r)
-      {
-        auto t0 = std::chrono::steady_clock::now();
-        if (threads <= 1)
-          cellApply(P, mats, X, Y, B);
-        else
-          {
-            distributeParentToChild(P.cons, X, B);
-            const size_t n = (size_t)P.nDofs * B;
-            std::vector<std::vector<double>> part(threads, std::vector<double>(n, 0.0));
-            std::vector<std::thread> th;
-            for (int ti = 0; ti < threads; ++ti)
-              th.emplace_back([&, ti] {
-                const uint32_t e0 = (uint32_t)((uint64_t)P.nCells * ti / threads);
-                const uint32_t e1 = (uint32_t)((uint64_t)P.nCells * (ti + 1) / threads);
-                std::vector<double> xc, yc;
-                double *Yp = part[ti].data();
-                for (uint32_t e = e0; e < e1; ++e)
-                  {
-                    const uint32_t ne = P.cellNdofs[e];
-                    xc.resize((size_t)ne * B);
-                    yc.resize((size_t)ne * B);
-                    for (uint32_t i = 0; i < ne; ++i)
-                      std::memcpy(&xc[(size_t)i * B],
-                                  X + (size_t)P.cellMap[P.cellMapOffset[e] + i] * B,
-                                  sizeof(double) * B);
-                    gemm('N', 'N', (int)B, (int)ne, (int)ne, 1.0, xc.data(), (int)B,
-                         mats + P.cellMatOffset[e], (int)ne, 0.0, yc.data(), (int)B);
-                    for (uint32_t i = 0; i < ne; ++i)
-                      {
-                        double *dst = Yp + (size_t)P.cellMap[P.cellMapOffset[e] + i] * B;
-                        for (uint32_t b = 0; b < B; ++b)
-                          dst[b] += yc[(size_t)i * B + b];
-                      }
-                  }
-              });
-            for (auto &x : th)
-              x.join();
-            for (size_t i = 0; i < n; ++i)
-              {
-                double s = 0.0;
-                for (int ti = 0; ti < threads; ++ti)
-                  s += part[ti][i];
-                Y[i] = s;
-              }
-            distributeChildToParent(P.cons, Y, B);
-          }
-        auto t1 = std::chrono::steady_clock::now();
-        if (r > 0)
-          t.push_back(std::chrono::duration<double>(t1 - t0).count());
-      }
-    std::sort(t.begin(), t.end());
-    return t.empty() ? 0.0 : t[t.size() / 2];
-  }
 }

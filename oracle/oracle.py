@@ -404,8 +404,12 @@ class Problem:
                     bounds=(lo, hi, beta), a=a, b=b, c=c, degree=m)
 
     def time_apply(self, kind, X, reps=3, threads=1):
+        """Median seconds of the threaded CPU apply: (total, cell part, O(N B)
+        vector part, Y)."""
         X = np.array(X, dtype=np.float64, order="C")
         Y = np.zeros_like(X)
+        tc, tv = C.c_double(0), C.c_double(0)
         t = lib().orc_time_apply(self.h, C.c_int(kind), _p(X), _p(Y),
-                                 C.c_uint32(X.shape[1]), C.c_int(reps), C.c_int(threads))
-        return t, Y
+                                 C.c_uint32(X.shape[1]), C.c_int(reps), C.c_int(threads),
+                                 C.byref(tc), C.byref(tv))
+        return t, tc.value, tv.value, Y
