@@ -99,6 +99,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--convergence-passes", type=int, default=25)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy 5 times (the reference kernel for "
+                         "scaling HBM byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -110,6 +113,18 @@ def main():
     import numpy as np
     import dfx
     from dfx import configs
+
+    if args.calibrate_copy:
+        # 2^22 rows x 32 FP64 = 1 GiB read + 1 GiB written per copy
+        ctx = dfx.Context(0)
+        A = dfx.MultiVector(ctx, 1 << 22, 32)
+        Bv = dfx.MultiVector(ctx, 1 << 22, 32)
+        A.set(1.0)
+        for _ in range(5):
+            Bv.copy_from(A)
+        ctx.synchronize()
+        log("copy calibration: 5 copies of 1073741824 bytes (convertK<double, double>)")
+        return
 
     cfg = configs.config(args.config)
     ctx = dfx.Context(0)
@@ -241,8 +256,17 @@ def main():
         log(f"CPU baseline {base['value']:.4g} passes/s on {base['cores']} threads")
 
     per_launch_flops = prof32["flops"] / max(prof32["launches"], 1)
+    per_launch_bytes = prof32["bytes"] / max(prof32["launches"], 1)
     per_launch_ms = prof32["kernelMs"] / max(prof32["launches"], 1)
-    achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12 if per_launch_ms else 0.0
+    # the roofline that bounds the kernel is the one with the longer ideal time
+    t_mfma = per_launch_flops / (PEAK_FP32_MFMA * 1e12)
+    t_hbm = per_launch_bytes / (PEAK_HBM * 1e9)
+    if t_hbm >= t_mfma:
+        bound, peak, unit = "hbm", PEAK_HBM, "GB/s"
+        achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms else 0.0
+    else:
+        bound, peak, unit = "mfma", PEAK_FP32_MFMA, "TFLOP/s"
+        achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12 if per_launch_ms else 0.0
     traffic = traffic_from_profiles("fp32")
     result = {
         "metric": "ChFSI pass time + cell-apply TFLOP/s, mixed vs FP64 @1e-6 Ha; p99 serve",
@@ -271,13 +295,19 @@ def main():
         "cell_apply_fp64_frac_of_peak": tflops(prof64) / PEAK_FP64_MFMA,
         "convergence": conv,
         "setup_s": setup_s,
-        "roofline": {"bound": "mfma", "kernel": "cellApplyK<float> (fused gather/MFMA/scatter)",
-                     "achieved": achieved, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA, "traffic": traffic,
+        "roofline": {"bound": bound,
+                     "kernel": "cellApplyBufK<float> (fused gather / FP32 MFMA / scatter)",
+                     "achieved": achieved, "peak": peak, "unit": unit,
+                     "frac": achieved / peak, "traffic": traffic,
                      "launches": prof32["launches"],
                      "avg_launch_ms": per_launch_ms,
                      "algorithmic_flops_per_launch": per_launch_flops,
-                     "algorithmic_bytes_per_launch": prof32["bytes"] / max(prof32["launches"], 1)},
+                     "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "achieved_tflops": (per_launch_flops / (per_launch_ms * 1e-3) / 1e12
+                                         if per_launch_ms else 0.0),
+                     "frac_of_fp32_mfma_peak": (per_launch_flops / (per_launch_ms * 1e-3)
+                                                / 1e12 / PEAK_FP32_MFMA
+                                                if per_launch_ms else 0.0)},
         "cpu_baseline": base,
     }
     print(json.dumps(result), flush=True)

@@ -69,10 +69,11 @@ namespace dfx
             DFX_HIP(hipEventRecord(r.b, s));
             const double B = X.B, sz = sizeof(T);
             r.flops = 2.0 * B * (double)l.sumN2;
-            // matrices once, map, gathered X, RMW of classical rows of Y,
-            // stored enriched partials
+            // cell matrices once, the map, gathered X rows, classical rows of
+            // Y written (and read unless first touched), enriched partials
             r.bytes = sz * (double)l.sumN2 + 4.0 * (double)l.sumN + sz * B * (double)l.sumN +
-                      2.0 * sz * B * (double)l.sumNc + sz * B * (double)(l.sumN - l.sumNc);
+                      sz * B * (2.0 * (double)l.sumNc - (double)l.sumFirst) +
+                      sz * B * (double)(l.sumN - l.sumNc);
             op->prof.push_back(r);
           }
       }

@@ -188,6 +188,8 @@ namespace dfx
     u32 enrOff;  // first row of this cell's enriched partials in the yE buffer
     u64 fragOff64; // offset (elements) of the cell's FP64 fragment-ordered matrix
     u64 fragOff32; // offset (elements) of the cell's FP32 fragment-ordered matrix
+    u32 rowLo;     // smallest classical dof of the cell (buffer base row)
+    u32 pad_;
   };
 
   // one kernel launch: cells [begin, end) of the launch order, all with the
@@ -200,6 +202,7 @@ namespace dfx
     u64 sumN2;  // sum of n_e^2 over the launch's cells
     u64 sumN;   // sum of n_e
     u64 sumNc;  // sum of classical rows
+    u64 sumFirst = 0; // classical rows stored on first touch (not read)
   };
 
   // one profiled kernel launch
@@ -218,6 +221,7 @@ struct dfx_op
   dfx_constraints *cons = nullptr;
   // ----- cell-wise
   dfx::u32 nCells = 0, nColors = 0, maxN = 0;
+  dfx::u32 maxRowSpan = 0; // max over cells of (largest - smallest classical dof + 1)
   dfx::u64 sumNe = 0, sumNe2 = 0;
   std::vector<dfx::u32> hCellNdofs, hMapOffset; // caller order
   std::vector<dfx::u64> hMatOffset;             // raw (caller) layout offsets

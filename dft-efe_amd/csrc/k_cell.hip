@@ -123,15 +123,18 @@ namespace dfx
         }
       };
 
-      // The cell kernel on raw buffers (x, y and yE each under 2 GiB): the same
-      // data flow as cellApplyK below with no per-lane branches, since every
-      // masked load or store goes to OFF_NONE.  The LDS tile is KMAX rows
-      // (the class maximum) so the gather writes every row it loads.
+      // The cell kernel on raw buffers: the same data flow as cellApplyK below
+      // with no per-lane branches, since every masked load or store goes to
+      // OFF_NONE.  Buffers are based at the cell's smallest classical row
+      // (rowLo) for x and y and at the first enriched row (nC) for enriched x
+      // rows, so offsets stay below 2 GiB when the cells' classical row spans
+      // and the enriched block do (checked by the launcher).  The LDS tile is
+      // KMAX rows (the class maximum) so the gather writes every row it loads.
       template <class T, int NT, int MTW>
       __global__ __launch_bounds__(256, DFX_CELL_MINW) void cellApplyBufK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
-        const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 xBytes,
-        u32 ldx, T *y, u32 yBytes, u32 ldy, T *yE, u32 yEBytes, u32 B)
+        const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
+        u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC)
       {
         using M                 = Mfma<T>;
         using BF                = Buf<T>;
@@ -162,9 +165,12 @@ namespace dfx
         const int      l    = threadIdx.x & 63;
         const rsrc_t   rA   = rsrc(frag + (S == 8 ? cm.fragOff64 : cm.fragOff32),
                                    (u32)(mt * nkb * 64 * G) * S);
-        const rsrc_t   rX   = rsrc(x, xBytes);
-        const rsrc_t   rY   = rsrc(y, yBytes);
-        const rsrc_t   rE   = rsrc(yE, yEBytes);
+        const u32      lo   = cm.rowLo;
+        auto cap = [](u64 b) { return (u32)(b < 0x7fffffffull ? b : 0x7fffffffull); };
+        const rsrc_t rX  = rsrc(x + (u64)lo * ldx, cap((u64)(nRowsX - lo) * ldx * S));
+        const rsrc_t rXE = rsrc(x + (u64)nC * ldx, cap((u64)(nRowsX - nC) * ldx * S));
+        const rsrc_t rY  = rsrc(y + (u64)lo * ldy, cap((u64)(nRowsY - lo) * ldy * S));
+        const rsrc_t rE  = rsrc(yE, yEBytes);
 
         auto loadA = [&](int kb, vec_t *dst) {
 #pragma unroll
@@ -202,7 +208,15 @@ namespace dfx
                 const int t  = threadIdx.x + i * 256;
                 const int kk = t / VC, c = (t - kk * VC) * G;
                 const u32 m  = sMap[kk]; // kk < KMAX <= 256: always in the array
-                v[i] = BF::load4(rX, ((kk < n) & (c < bw)) ? (m * ldx + c0 + c) * S : OFF_NONE);
+                v[i] = BF::load4(rX, ((kk < nc) & (c < bw)) ? ((m - lo) * ldx + c0 + c) * S
+                                                            : OFF_NONE);
+                if (nc < n) // enriched cell (block-uniform)
+                  {
+                    const vec_t ve = BF::load4(
+                      rXE, ((kk >= nc) & (kk < n) & (c < bw)) ? ((m - nC) * ldx + c0 + c) * S
+                                                             : OFF_NONE);
+                    v[i] = kk >= nc ? ve : v[i];
+                  }
               }
 #pragma unroll
             for (int i = 0; i < GATHER_IT; ++i)
@@ -224,7 +238,7 @@ namespace dfx
                   const int  row  = rowOf(t, vv);
                   const u32  fb   = sFirst[row >> 5], m = sMap[row];
                   const bool rmw  = (row < nc) & !((fb >> (row & 31)) & 1u);
-                  const u32  base = (m * ldy + c0 + cl) * S;
+                  const u32  base = ((m - lo) * ldy + c0 + cl) * S;
 #pragma unroll
                   for (int nt = 0; nt < NT; ++nt)
                     o[nt][vv] = BF::load(rY, (rmw & (nt * 16 + cl < bw)) ? base + nt * 16 * S
@@ -294,7 +308,7 @@ namespace dfx
                 for (int vv = 0; vv < 4; ++vv)
                   {
                     const int row = rowOf(t, vv);
-                    const u32 yb  = (sMap[row] * ldy + c0 + cl) * S;
+                    const u32 yb  = ((sMap[row] - lo) * ldy + c0 + cl) * S;
                     const u32 eb  = ((cm.enrOff + (u32)(row - nc)) * B + c0 + cl) * S;
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt)
@@ -602,10 +616,12 @@ namespace dfx
         constexpr int G  = Mfma<T>::G;
         constexpr int BC = 16 * NT;
         const u64     S  = sizeof(T);
-        const u64 xBytes = (u64)x.N * x.ld * S, yBytes = (u64)y.N * y.ld * S;
+        const u64 nE     = op->nDofs - op->nClassical;
+        const u64 span   = op->maxRowSpan;
         const u64 eBytes = (u64)op->nEnrSlots * x.B * S;
-        const bool buf   = xBytes < (1ull << 31) && yBytes < (1ull << 31) &&
-                         eBytes < (1ull << 31) && x.B % G == 0 && x.ld % G == 0 &&
+        const bool buf   = span * x.ld * S < (1ull << 31) && span * y.ld * S < (1ull << 31) &&
+                         nE * x.ld * S < (1ull << 31) && eBytes < (1ull << 31) &&
+                         x.B % G == 0 && x.ld % G == 0 &&
                          (reinterpret_cast<uintptr_t>(x.p) & 15) == 0;
         // dynamic tile plus the static map / first-touch words (about 1 KiB)
         const size_t lds = (size_t)(buf ? 64 * MTW : L.kpadMax) * BC * S;
@@ -622,7 +638,7 @@ namespace dfx
         if (buf)
           cellApplyBufK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
-            x.p, (u32)xBytes, x.ld, y.p, (u32)yBytes, y.ld, yE, (u32)eBytes, x.B);
+            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical);
         else
           cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,

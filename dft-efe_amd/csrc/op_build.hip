@@ -178,6 +178,15 @@ namespace dfx
         m.enrOff    = enrOff[e];
         m.fragOff64 = f64;
         m.fragOff32 = f32;
+        u32 lo = 0xffffffffu, hi = 0;
+        for (u32 i = 0; i < nc[e]; ++i)
+          {
+            lo = std::min(lo, map[op->hMapOffset[e] + i]);
+            hi = std::max(hi, map[op->hMapOffset[e] + i]);
+          }
+        m.rowLo         = nc[e] ? lo : 0;
+        m.pad_          = 0;
+        op->maxRowSpan  = std::max(op->maxRowSpan, nc[e] ? hi - lo + 1 : 0u);
         const u64 mpad = 16 * (u64)((n + 15) / 16);
         f64 += mpad * (8 * (u64)((n + 7) / 8));
         f32 += mpad * (16 * (u64)((n + 15) / 16));
@@ -229,6 +238,13 @@ namespace dfx
             }
         }
       op->fillY = std::find(seen.begin(), seen.end(), 0) != seen.end();
+      for (auto *L : {&op->launches64, &op->launches32})
+        for (Launch &l : *L)
+          {
+            l.sumFirst = 0;
+            for (u64 w = (u64)l.begin * 8; w < (u64)l.end * 8; ++w)
+              l.sumFirst += (u64)__builtin_popcount(hFirst[w]);
+          }
     }
 
     // enriched dof -> slots (cell order)

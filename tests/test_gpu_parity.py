@@ -111,16 +111,32 @@ def test_cell_apply_fp64(ctx, probs, name, B):
 
 
 @pytest.mark.parametrize("name", ["c1", "hang3", "enr"])
-def test_cell_apply_fp32(ctx, probs, name):
+@pytest.mark.parametrize("B", [24, 6])
+def test_cell_apply_fp32(ctx, probs, name, B):
     P = probs[name]
     ops = Ops(ctx, P)
-    B = 24
     X = guess(P, B, seed=9)
     Xg = mv(ctx, X.astype(np.float32), dfx.F32)
     Yg = dfx.MultiVector(ctx, P.nDofs, B, dfx.F32)
     ops.H.apply(Xg, Yg)
     ref = P.apply(O.OP_H, X.astype(np.float32).astype(np.float64))
     assert rel(Yg.to_numpy().astype(np.float64), ref) < TOL_APPLY32
+
+
+@pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
+def test_cell_apply_kernel_paths_agree_bitwise(ctx, probs, dtype):
+    """The buffer-addressed kernel (column count a multiple of the 16-byte
+    vector) and the pointer kernel (any other count) give the same bits per
+    column: columns are independent and both sum in the same K order."""
+    P = probs["enr"]
+    ops = Ops(ctx, P)
+    npd = np.float64 if dtype == dfx.F64 else np.float32
+    X = guess(P, 4, seed=31).astype(npd)
+    Y4 = dfx.MultiVector(ctx, P.nDofs, 4, dtype)
+    ops.H.apply(mv(ctx, X, dtype), Y4)
+    Y1 = dfx.MultiVector(ctx, P.nDofs, 1, dtype)
+    ops.H.apply(mv(ctx, np.ascontiguousarray(X[:, :1]), dtype), Y1)
+    assert np.array_equal(Y4.to_numpy()[:, :1], Y1.to_numpy())
 
 
 def test_cell_apply_deterministic(ctx, probs):
