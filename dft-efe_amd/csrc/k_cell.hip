@@ -32,6 +32,7 @@
 #define DFX_CELL_MINW 2
 #endif
 
+
 namespace dfx
 {
   namespace k
@@ -146,10 +147,17 @@ namespace dfx
         constexpr int VC        = BC / G;
         constexpr int KMAX      = 64 * MTW;
         constexpr int D         = (KMAX / KB) < DFX_CELL_DEPTH ? (KMAX / KB) : DFX_CELL_DEPTH;
-        constexpr int GATHER_IT = KMAX * VC / 256;
+        constexpr int UNITS     = (KMAX / G) * VC; // G x G gather units of the tile
+        constexpr int UPT       = (UNITS + 255) / 256;
+        static_assert(UNITS % 64 == 0, "gather units must fill whole waves");
+        // LDS column position: within each 16-column segment the column is
+        // rotated by the segment index, so that both the gather's stores
+        // (columns G apart) and the MFMA's B reads (16 consecutive columns)
+        // fall in 16 distinct 16-byte bank groups
+        auto swz = [](int col) { return (col & ~15) | ((col + (col >> 4)) & 15); };
         constexpr bool EARLY    = DFX_CELL_EARLY && MTW * NT * 4 * (int)sizeof(T) <= 128;
         constexpr u32 S         = sizeof(T);
-        static_assert(KMAX * VC % 256 == 0, "gather must cover the tile exactly");
+        constexpr int  NRING    = D;
         extern __shared__ __align__(16) unsigned char smemRaw[];
         T *sX = reinterpret_cast<T *>(smemRaw);
         __shared__ u32 sMap[256];
@@ -168,6 +176,8 @@ namespace dfx
         const u32      lo   = cm.rowLo;
         auto cap = [](u64 b) { return (u32)(b < 0x7fffffffull ? b : 0x7fffffffull); };
         const rsrc_t rX  = rsrc(x + (u64)lo * ldx, cap((u64)(nRowsX - lo) * ldx * S));
+        // enriched rows (>= nC) reachable through rX as well: one load per row
+        const bool   oneX = nc == n || (u64)(nRowsX - lo) * ldx * S < 0x7fffffffull;
         const rsrc_t rXE = rsrc(x + (u64)nC * ldx, cap((u64)(nRowsX - nC) * ldx * S));
         const rsrc_t rY  = rsrc(y + (u64)lo * ldy, cap((u64)(nRowsY - lo) * ldy * S));
         const rsrc_t rE  = rsrc(yE, yEBytes);
@@ -183,9 +193,9 @@ namespace dfx
               dst[t] = BF::load4(rA, off);
             }
         };
-        vec_t ring[D][MTW];
+        vec_t ring[NRING][MTW];
 #pragma unroll
-        for (int d = 0; d < D; ++d)
+        for (int d = 0; d < NRING; ++d)
           loadA(d, ring[d]);
 
         if (threadIdx.x < n)
@@ -200,32 +210,53 @@ namespace dfx
         for (u32 c0 = 0; c0 < B; c0 += BC)
           {
             const int bw = min(BC, (int)(B - c0));
-            // ---- gather (columns are whole 16-byte vectors: B % G == 0)
-            vec_t v[GATHER_IT];
+            // ---- gather (columns are whole 16-byte vectors: B % G == 0).  A
+            // unit is G rows x G columns: G 16-byte loads, transposed in
+            // registers into G 16-byte LDS stores (one per column, holding G
+            // consecutive K values), at the swizzled column position swz().
+            vec_t v[UPT][G];
 #pragma unroll
-            for (int i = 0; i < GATHER_IT; ++i)
+            for (int u = 0; u < UPT; ++u)
               {
-                const int t  = threadIdx.x + i * 256;
-                const int kk = t / VC, c = (t - kk * VC) * G;
-                const u32 m  = sMap[kk]; // kk < KMAX <= 256: always in the array
-                v[i] = BF::load4(rX, ((kk < nc) & (c < bw)) ? ((m - lo) * ldx + c0 + c) * S
-                                                            : OFF_NONE);
-                if (nc < n) // enriched cell (block-uniform)
+                const int  t    = threadIdx.x + u * 256;
+                const int  q    = t / VC, c = (t - q * VC) * G; // K group, first column
+                const bool live = t < UNITS;                     // wave-uniform
+#pragma unroll
+                for (int r = 0; r < G; ++r)
                   {
-                    const vec_t ve = BF::load4(
-                      rXE, ((kk >= nc) & (kk < n) & (c < bw)) ? ((m - nC) * ldx + c0 + c) * S
-                                                             : OFF_NONE);
-                    v[i] = kk >= nc ? ve : v[i];
+                    const int kk = q * G + r;
+                    const u32 m  = sMap[live ? kk : 0]; // kk < KMAX <= 256
+                    if (oneX) // every row of the cell through rX (block-uniform)
+                      v[u][r] = BF::load4(rX, (live & (kk < n) & (c < bw))
+                                                ? ((m - lo) * ldx + c0 + c) * S
+                                                : OFF_NONE);
+                    else
+                      {
+                        const vec_t vc = BF::load4(rX, (live & (kk < nc) & (c < bw))
+                                                         ? ((m - lo) * ldx + c0 + c) * S
+                                                         : OFF_NONE);
+                        const vec_t ve = BF::load4(rXE, (live & (kk >= nc) & (kk < n) & (c < bw))
+                                                          ? ((m - nC) * ldx + c0 + c) * S
+                                                          : OFF_NONE);
+                        v[u][r] = kk >= nc ? ve : vc;
+                      }
                   }
               }
 #pragma unroll
-            for (int i = 0; i < GATHER_IT; ++i)
+            for (int u = 0; u < UPT; ++u)
               {
-                const int t  = threadIdx.x + i * 256;
-                const int kk = t / VC, c = (t - kk * VC) * G;
+                const int t = threadIdx.x + u * 256;
+                const int q = t / VC, c = (t - q * VC) * G;
+                if (t < UNITS)
 #pragma unroll
-                for (int g = 0; g < G; ++g)
-                  sX[((kk / G) * BC + c + g) * G + (kk % G)] = v[i][g];
+                  for (int g = 0; g < G; ++g)
+                    {
+                      vec_t col;
+#pragma unroll
+                      for (int r = 0; r < G; ++r)
+                        col[r] = v[u][r][g];
+                      *reinterpret_cast<vec_t *>(sX + (q * BC + swz(c + g)) * G) = col;
+                    }
               }
 
             // ---- old values of the classical rows touched before
@@ -261,23 +292,24 @@ namespace dfx
               for (int nt = 0; nt < NT; ++nt)
                 acc[t][nt] = acc_t{0, 0, 0, 0};
 
+            // ring[0] holds K block kb, ring[d] block kb + d
             for (int kb = 0; kb < nkb; ++kb)
               {
                 vec_t b[NT];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
-                    sX + (((kb * 4 + (l >> 4)) * BC) + nt * 16 + cl) * G);
+                    sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
                 vec_t a[MTW];
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
                   a[t] = ring[0][t];
 #pragma unroll
-                for (int d = 0; d + 1 < D; ++d)
+                for (int d = 0; d + 1 < NRING; ++d)
 #pragma unroll
                   for (int t = 0; t < MTW; ++t)
                     ring[d][t] = ring[d + 1][t];
-                loadA(kb + D, ring[D - 1]);
+                loadA(kb + NRING, ring[NRING - 1]);
 #pragma unroll
                 for (int s = 0; s < G; ++s)
 #pragma unroll
@@ -287,9 +319,10 @@ namespace dfx
                       for (int nt = 0; nt < NT; ++nt)
                         acc[t][nt] = M::op(a[t][s], b[nt][s], acc[t][nt]);
               }
+            // the next column chunk starts again at K block 0
             if (c0 + BC < B)
 #pragma unroll
-              for (int d = 0; d < D; ++d)
+              for (int d = 0; d < NRING; ++d)
                 loadA(d, ring[d]);
 
             // ---- scatter: classical rows to y (stored on first touch, else
