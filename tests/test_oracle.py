@@ -341,3 +341,114 @@ def test_cpu_baseline_timing_loop_computes_the_apply():
         t, tc, tv, Y = P.time_apply(O.OP_H, X.copy(), reps=1, threads=threads)
         assert t > 0 and tc > 0 and tv >= 0
         assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+# ------------------------------------------------------------ KS driver pieces
+def test_fermi_level_counts_electrons():
+    """At the Newton-Raphson Fermi level sum_i 2 f_i = Ne; with a gap much
+    larger than kT it lies between HOMO and LUMO, and a symmetric spectrum
+    with a half-filled degenerate pair puts it exactly at the pair."""
+    eig = np.array([-3.0, -2.0, -1.0, -0.5, 0.2, 0.9])
+    T = 500.0
+    kT = O.K_BOLTZMANN_HARTREE * T
+    for ne in (2, 4, 6, 8):
+        ef = O.fermi_level(eig, ne, T, 1e-12)
+        occ = sum(2 * O.fermi_dirac(e, ef, O.K_BOLTZMANN_HARTREE, T) for e in eig)
+        assert abs(occ - ne) < 1e-9
+        homo, lumo = eig[ne // 2 - 1], eig[ne // 2]
+        assert homo + 20 * kT < ef < lumo - 20 * kT
+    sym = np.array([-2.0, -1.0, -1.0, 0.0])  # 4 electrons: 2 in -2, 2 shared by the pair
+    ef = O.fermi_level(sym, 4, T, 1e-12)
+    assert abs(ef - (-1.0)) < 1e-9
+
+
+def test_fermi_dirac_limits_and_derivative():
+    kb, T, mu = O.K_BOLTZMANN_HARTREE, 300.0, 0.1
+    assert O.fermi_dirac(mu, mu, kb, T) == 0.5
+    assert O.fermi_dirac(mu - 1.0, mu, kb, T) == 1.0
+    assert O.fermi_dirac(mu + 1.0, mu, kb, T) < 1e-300
+    for e in (0.099, 0.1, 0.1005):
+        h = 1e-7
+        fd = (O.fermi_dirac(e, mu + h, kb, T) - O.fermi_dirac(e, mu - h, kb, T)) / (2 * h)
+        assert abs(fd - O.fermi_dirac_der(e, mu, kb, T)) < 1e-5 * max(1.0, abs(fd))
+
+
+def test_cheby_degree_lookup_boundaries():
+    """lower_bound over the table keys: a key equal to a table entry takes
+    that entry, one past it the next; beyond the last entry 1250."""
+    assert O.cheby_degree(0) == 6 and O.cheby_degree(10) == 6 and O.cheby_degree(11) == 9
+    assert O.cheby_degree(750) == 30 and O.cheby_degree(751) == 39
+    assert O.cheby_degree(500000) == 1000 and O.cheby_degree(500001) == 1250
+    assert O.cheby_degree(10.9) == 6  # truncation to the integer key
+
+
+def test_residual_norms_on_diagonal_operator():
+    """||H x - eps M x|| for a unit vector e_i of a diagonal pencil is |eps m_i - lam_i|."""
+    lam = np.linspace(-1.0, 4.0, 12)
+    mdiag = np.linspace(1.0, 2.0, 12)
+    N = len(lam)
+    P = O.Problem.from_arrays(np.ones(N), np.arange(N), N, N, H=lam, M=mdiag,
+                              dinv=1.0 / mdiag, mgll=mdiag, p=1)
+    X = np.zeros((N, 3)); X[2, 0] = 1.0; X[5, 1] = 1.0; X[7, 2] = 1.0
+    eps = np.array([lam[2], lam[5] + 0.25, lam[7] - 1.0])
+    res = P.residual_norms(X, eps)
+    want = np.abs(eps * mdiag[[2, 5, 7]] - lam[[2, 5, 7]])
+    assert np.allclose(res, want, rtol=1e-14, atol=1e-15)
+
+
+def test_orthogonalized_enrichment_is_orthogonal_on_interior_nodes():
+    """After the projection M_II alpha = d, int N_c phi~_g = 0 for every
+    classical node c interior to atom g's support (all of c's cells enriched
+    by g, c unconstrained): checked on the assembled M^ce."""
+    P = O.Problem.build(p=2, n=(6, 6, 6), lo=(-12.,) * 3, hi=(12.,) * 3,
+                        atoms=[(-1.5, 0.5, 0.2, 1), (2.0, -1.0, 0.7, 6)],
+                        enrich=True, orthogonalize=True)
+    d = P.arrays()
+    nd, cmap, M = d["cellNdofs"], d["cellMap"], d["M"]
+    nC, nE = P.nClassical, P.nEnriched
+    Mce = np.zeros((nC, nE))
+    inc = np.zeros(nC, dtype=np.int64)
+    ncl = (P.p + 1) ** 3
+    cnt = {}
+    off = moff = 0
+    for e, n in enumerate(nd):
+        ids = cmap[off:off + n]
+        Me = M[moff:moff + n * n].reshape(n, n, order="F")
+        inc[ids[:ncl]] += 1
+        for jj in range(ncl, n):
+            g = ids[jj] - nC
+            Mce[ids[:ncl], g] += Me[:ncl, jj]
+            for c in ids[:ncl]:
+                cnt[(g // 5, c)] = cnt.get((g // 5, c), 0) + 1
+        off += n
+        moff += n * n
+    rows = set(d["rows"].tolist())
+    worst, scale, checked = 0.0, np.abs(Mce).max(), 0
+    for (a, c), k in cnt.items():
+        if k // 5 == inc[c] and c not in rows:
+            worst = max(worst, np.abs(Mce[c, 5 * a:5 * a + 5]).max())
+            checked += 1
+    assert checked > 10 and worst < 1e-10 * scale
+    # and the projection did something: without it the same entries are O(1)
+    Q = O.Problem.build(p=2, n=(6, 6, 6), lo=(-12.,) * 3, hi=(12.,) * 3,
+                        atoms=[(-1.5, 0.5, 0.2, 1), (2.0, -1.0, 0.7, 6)],
+                        enrich=True, orthogonalize=False)
+    assert np.abs(Q.arrays()["M"] - M).max() > 1e-3 * scale
+
+
+def test_ks_eigensolve_v0_converges_to_tensor_closed_form():
+    """The KS driver (Lanczos bounds, warm-up pass, residual-filter passes,
+    Fermi level, occupancy-based convergence) on V = 0: the occupied states
+    converge to sums of 1-D generalized eigenvalues."""
+    p, n, L = 3, 3, 20.0
+    K1, M1 = _one_d_gep(p, n, L)
+    l1 = sl.eigh(0.5 * K1, M1, eigvals_only=True)
+    all3 = np.sort((l1[:, None, None] + l1[None, :, None] + l1[None, None, :]).ravel())
+    P = O.Problem.build(p=p, n=(n, n, n), lo=(-10.,) * 3, hi=(10.,) * 3, atoms=[])
+    X0 = guess(P, 10, seed=1)
+    lg = P.p2c(O.rng_uniform(2, -1, 1, P.nDofs).reshape(-1, 1))[:, 0]
+    r = P.ks_eigensolve(X0, lg, degree=25, tol=1e-8, max_pass=40, num_electrons=2)
+    assert r["rc"] == 0
+    occ = r["nocc"]
+    assert occ >= 1 and np.all(r["res"][:occ] <= 1e-8)
+    assert abs(r["eig"][0] - all3[0]) < 1e-10
