@@ -34,11 +34,15 @@ namespace dfx
 
   // Y = C^T (sum_e Q_e^T A_e Q_e) C X, KohnShamOperatorContextFE::apply
   // (src/ksdft/KohnShamOperatorContextFE.t.cpp:1326-1415)
+  // dsc / xe: see applyInvThenCellwise (classical rows gathered as dsc o X,
+  // enriched rows from xe; X already constrained)
   template <class T>
-  void applyCellwise(dfx_op *op, View<T> X, View<T> Y)
+  void applyCellwise(dfx_op *op, View<T> X, View<T> Y, const double *dsc = nullptr,
+                     const T *xe = nullptr)
   {
     hipStream_t s = op->ctx->stream;
-    k::p2c<T>(s, op->cons, X);
+    if (!dsc)
+      k::p2c<T>(s, op->cons, X);
     // every classical row is stored by the first cell that reaches it and every
     // enriched row by the enriched reduction, so Y is cleared only when some
     // dof lies in no cell
@@ -63,7 +67,7 @@ namespace dfx
             DFX_HIP(hipEventCreate(&r.b));
             DFX_HIP(hipEventRecord(r.a, s));
           }
-        k::cellApply<T>(s, op, l, frag, Xc, Y, yE);
+        k::cellApply<T>(s, op, l, frag, Xc, Y, yE, dsc, xe);
         if (prof)
           {
             DFX_HIP(hipEventRecord(r.b, s));
@@ -95,6 +99,45 @@ namespace dfx
       k::enrichedBlock<T>(s, Y, op->enrBlock.as<double>(), Xc, nC, op->nEnriched);
     k::c2p<T>(s, op->cons, Y);
   }
+
+  // Y = A (Dinv X) with Dinv a diagonal (+ enriched block) operator and A a
+  // cell-wise one sharing homogeneous Dirichlet constraints (no parents, zero
+  // values, classical rows only): the diagonal is applied inside the cell
+  // gather instead of as a pass of its own, and only the enriched rows of
+  // Dinv X are formed (in S1).  Every value equals the unfused sequence
+  // Dinv.apply(X, S1); A.apply(S1, Y) bit for bit (the same products in the
+  // same precision; constrained rows are zero either way) and X receives the
+  // same parent->child update.  Returns false (nothing done) when the
+  // operators or the multivectors do not qualify.
+  template <class T>
+  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y)
+  {
+    if (Dinv->kind != OP_DIAGONAL || A->kind != OP_CELLWISE || Dinv->cons != A->cons)
+      return false;
+    if (Dinv->nDofs != A->nDofs || Dinv->nEnriched != A->nDofs - A->nClassical)
+      return false;
+    if (A->cons && (!A->cons->homDirichlet ||
+                    (!A->cons->hRows.empty() && A->cons->hRows.back() >= A->nClassical)))
+      return false;
+    if ((sizeof(T) == 4 && !A->hasF32) || X.N != A->nDofs || Y.N != A->nDofs ||
+        S1.N != A->nDofs || S1.ld != X.ld || X.B != Y.B || S1.B != X.B)
+      return false;
+    if (!k::cellBufferPath<T>(A, View<const T>{X.p, X.N, X.B, X.ld},
+                              View<const T>{Y.p, Y.N, Y.B, Y.ld}) ||
+        (reinterpret_cast<uintptr_t>(S1.p) & 15) != 0)
+      return false;
+    hipStream_t s = A->ctx->stream;
+    k::p2c<T>(s, Dinv->cons, X);
+    if (Dinv->nEnriched)
+      k::enrichedBlock<T>(s, S1, Dinv->enrBlock.as<double>(),
+                          View<const T>{X.p, X.N, X.B, X.ld}, A->nClassical, Dinv->nEnriched);
+    applyCellwise<T>(A, X, Y, Dinv->diag.as<double>(), S1.p);
+    return true;
+  }
+  template bool applyInvThenCellwise<double>(dfx_op *, dfx_op *, View<double>, View<double>,
+                                             View<double>);
+  template bool applyInvThenCellwise<float>(dfx_op *, dfx_op *, View<float>, View<float>,
+                                            View<float>);
 
   template <class T>
   void applyOp(dfx_op *op, View<T> X, View<T> Y)
@@ -434,6 +477,8 @@ extern "C"
       std::vector<char> isRow(nDofs, 0);
       c->hRows.assign(rows, rows + nRows);
       c->hInhom.assign(inhom, inhom + nRows);
+      c->homDirichlet = nEnt == 0 && std::all_of(c->hInhom.begin(), c->hInhom.end(),
+                                                 [](double v) { return v == 0.0; });
       c->hRowOffsets.resize(nRows + 1);
       c->hCols.resize(nEnt);
       c->hWeights.resize(nEnt);

@@ -65,6 +65,8 @@ namespace dfx
   View<T> view(dfx_mv *m);
   template <class T>
   void applyOp(dfx_op *op, View<T> X, View<T> Y);
+  template <class T>
+  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y);
 
   // column dots of two device multivectors -> host (FP64), deterministic
   void colDotsHost(dfx_ctx *ctx, const dfx_mv *X, const dfx_mv *Y, double *out);

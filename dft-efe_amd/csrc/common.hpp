@@ -163,6 +163,7 @@ struct dfx_constraints
   std::atomic<int> refs{1}; // creator + every operator built on it
   dfx_ctx *ctx = nullptr;
   dfx::u32 nDofs = 0, nRows = 0, nEntries = 0, nParents = 0;
+  bool homDirichlet = false; // every row is x = 0 (no parents, zero inhomogeneity)
   std::vector<dfx::u32> hRows, hRowOffsets, hCols, hParents, hParentOffsets,
     hParentRows, hParentEntries;
   std::vector<double> hWeights, hInhom;

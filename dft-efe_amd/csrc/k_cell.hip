@@ -31,6 +31,12 @@
 #ifndef DFX_CELL_MINW
 #define DFX_CELL_MINW 2
 #endif
+#ifndef DFX_CELL_AREG_MAX
+#define DFX_CELL_AREG_MAX 128
+#endif
+#ifndef DFX_CELL_DEPTH1
+#define DFX_CELL_DEPTH1 1
+#endif
 
 
 namespace dfx
@@ -131,11 +137,12 @@ namespace dfx
       // rows, so offsets stay below 2 GiB when the cells' classical row spans
       // and the enriched block do (checked by the launcher).  The LDS tile is
       // KMAX rows (the class maximum) so the gather writes every row it loads.
-      template <class T, int NT, int MTW>
+      template <class T, int NT, int MTW, bool DSC>
       __global__ __launch_bounds__(256, DFX_CELL_MINW) void cellApplyBufK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
         const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
-        u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC)
+        u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC,
+        const double *__restrict__ dsc, const T *xe)
       {
         using M                 = Mfma<T>;
         using BF                = Buf<T>;
@@ -146,7 +153,8 @@ namespace dfx
         constexpr int BC        = 16 * NT;
         constexpr int VC        = BC / G;
         constexpr int KMAX      = 64 * MTW;
-        constexpr int D         = (KMAX / KB) < DFX_CELL_DEPTH ? (KMAX / KB) : DFX_CELL_DEPTH;
+        constexpr int DEPTH     = NT == 1 ? DFX_CELL_DEPTH1 : DFX_CELL_DEPTH;
+        constexpr int D         = (KMAX / KB) < DEPTH ? (KMAX / KB) : DEPTH;
         constexpr int UNITS     = (KMAX / G) * VC; // G x G gather units of the tile
         constexpr int UPT       = (UNITS + 255) / 256;
         static_assert(UNITS % 64 == 0, "gather units must fill whole waves");
@@ -157,11 +165,18 @@ namespace dfx
         auto swz = [](int col) { return (col & ~15) | ((col + (col >> 4)) & 15); };
         constexpr bool EARLY    = DFX_CELL_EARLY && MTW * NT * 4 * (int)sizeof(T) <= 128;
         constexpr u32 S         = sizeof(T);
-        constexpr int  NRING    = D;
+        constexpr int  NKBMAX   = KMAX / KB;
+        // narrow chunks (NT == 1: few columns, little work to hide latency
+        // behind) hold the wave's whole A operand in registers, requested
+        // at the start; wider chunks use a D-deep register ring
+        constexpr bool AREG     = NT == 1 && MTW * NKBMAX * 4 <= // 4 VGPRs per vector
+                                                 (S == 4 ? DFX_CELL_AREG_MAX : DFX_CELL_AREG_MAX / 2);
+        constexpr int  NRING    = AREG ? NKBMAX : D;
         extern __shared__ __align__(16) unsigned char smemRaw[];
         T *sX = reinterpret_cast<T *>(smemRaw);
         __shared__ u32 sMap[256];
         __shared__ u32 sFirst[8];
+        __shared__ __align__(16) T sDsc[DSC ? 256 : 4];
 
         const u32      pos  = begin + blockIdx.x;
         const CellMeta cm   = meta[pos];
@@ -177,8 +192,8 @@ namespace dfx
         auto cap = [](u64 b) { return (u32)(b < 0x7fffffffull ? b : 0x7fffffffull); };
         const rsrc_t rX  = rsrc(x + (u64)lo * ldx, cap((u64)(nRowsX - lo) * ldx * S));
         // enriched rows (>= nC) reachable through rX as well: one load per row
-        const bool   oneX = nc == n || (u64)(nRowsX - lo) * ldx * S < 0x7fffffffull;
-        const rsrc_t rXE = rsrc(x + (u64)nC * ldx, cap((u64)(nRowsX - nC) * ldx * S));
+        const bool   oneX = nc == n || (xe == x && (u64)(nRowsX - lo) * ldx * S < 0x7fffffffull);
+        const rsrc_t rXE = rsrc(xe + (u64)nC * ldx, cap((u64)(nRowsX - nC) * ldx * S));
         const rsrc_t rY  = rsrc(y + (u64)lo * ldy, cap((u64)(nRowsY - lo) * ldy * S));
         const rsrc_t rE  = rsrc(yE, yEBytes);
 
@@ -200,6 +215,12 @@ namespace dfx
 
         if (threadIdx.x < n)
           sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
+        // M^-1 folded in: the MFMA's B operand (X_e) is scaled row by row,
+        // diag on the classical rows and 1 elsewhere (exact)
+        if constexpr (DSC)
+          sDsc[threadIdx.x] = threadIdx.x < (unsigned)nc
+                                ? (T)dsc[map[cm.mapOff + threadIdx.x]]
+                                : (T)1;
         if (threadIdx.x < 8)
           sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
         __syncthreads();
@@ -242,6 +263,7 @@ namespace dfx
                       }
                   }
               }
+
 #pragma unroll
             for (int u = 0; u < UPT; ++u)
               {
@@ -292,14 +314,52 @@ namespace dfx
               for (int nt = 0; nt < NT; ++nt)
                 acc[t][nt] = acc_t{0, 0, 0, 0};
 
-            // ring[0] holds K block kb, ring[d] block kb + d
-            for (int kb = 0; kb < nkb; ++kb)
+            if constexpr (AREG)
+              {
+#pragma unroll
+                for (int kb = 0; kb < NKBMAX; ++kb)
+                  if (kb < nkb) // block-uniform
+                    {
+                      vec_t b[NT];
+#pragma unroll
+                      for (int nt = 0; nt < NT; ++nt)
+                        b[nt] = *reinterpret_cast<const vec_t *>(
+                          sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
+                      if constexpr (DSC)
+                        {
+                          const vec_t sc =
+                            *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
+#pragma unroll
+                          for (int nt = 0; nt < NT; ++nt)
+                            b[nt] = sc * b[nt];
+                        }
+#pragma unroll
+                      for (int s = 0; s < G; ++s)
+#pragma unroll
+                        for (int t = 0; t < MTW; ++t)
+                          if (w + 4 * t < mt)
+#pragma unroll
+                            for (int nt = 0; nt < NT; ++nt)
+                              acc[t][nt] = M::op(ring[kb][t][s], b[nt][s], acc[t][nt]);
+                    }
+              }
+            else
+              // ring[0] holds K block kb, ring[d] block kb + d
+              for (int kb = 0; kb < nkb; ++kb)
               {
                 vec_t b[NT];
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
                     sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
+                if constexpr (DSC)
+                  {
+                    const vec_t sc =
+                      *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                      b[nt] = sc * b[nt];
+                  }
                 vec_t a[MTW];
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
@@ -642,36 +702,65 @@ namespace dfx
           }
       }
 
+    } // namespace
+
+    template <class T>
+    bool cellBufferPath(const dfx_op *op, View<const T> x, View<const T> y)
+    {
+      constexpr int G  = Mfma<T>::G;
+      const u64     S  = sizeof(T);
+      const u64 nE     = op->nDofs - op->nClassical;
+      const u64 span   = op->maxRowSpan;
+      const u64 eBytes = (u64)op->nEnrSlots * x.B * S;
+      return span * x.ld * S < (1ull << 31) && span * y.ld * S < (1ull << 31) &&
+             nE * x.ld * S < (1ull << 31) && eBytes < (1ull << 31) && x.B % G == 0 &&
+             x.ld % G == 0 && (reinterpret_cast<uintptr_t>(x.p) & 15) == 0;
+    }
+    template bool cellBufferPath<double>(const dfx_op *, View<const double>,
+                                         View<const double>);
+    template bool cellBufferPath<float>(const dfx_op *, View<const float>, View<const float>);
+
+    namespace
+    {
       template <class T, int NT, int MTW>
       void launchCell(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                      View<const T> x, View<T> y, T *yE)
+                      View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe)
       {
-        constexpr int G  = Mfma<T>::G;
         constexpr int BC = 16 * NT;
         const u64     S  = sizeof(T);
-        const u64 nE     = op->nDofs - op->nClassical;
-        const u64 span   = op->maxRowSpan;
         const u64 eBytes = (u64)op->nEnrSlots * x.B * S;
-        const bool buf   = span * x.ld * S < (1ull << 31) && span * y.ld * S < (1ull << 31) &&
-                         nE * x.ld * S < (1ull << 31) && eBytes < (1ull << 31) &&
-                         x.B % G == 0 && x.ld % G == 0 &&
-                         (reinterpret_cast<uintptr_t>(x.p) & 15) == 0;
+        const bool buf   = cellBufferPath<T>(op, x, View<const T>{y.p, y.N, y.B, y.ld});
         // dynamic tile plus the static map / first-touch words (about 1 KiB)
         const size_t lds = (size_t)(buf ? 64 * MTW : L.kpadMax) * BC * S;
         DFX_REQUIRE(lds + 2048 <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
-        const void *fn = buf ? reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW>)
-                             : reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>);
-        static size_t attrSet[2] = {60 * 1024, 60 * 1024};
-        if (lds > attrSet[buf])
+        const int   which = !buf ? 0 : (dsc ? 2 : 1);
+        const void *fn    = which == 0 ? reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>)
+                          : which == 1
+                            ? reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false>)
+                            : reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true>);
+        // raised once per kernel to the largest tile seen (several host
+        // threads may launch concurrently: serving slots)
+        static std::atomic<size_t> attrSet[3] = {60 * 1024, 60 * 1024, 60 * 1024};
+        size_t cur = attrSet[which].load();
+        if (lds > cur)
           {
             DFX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds));
-            attrSet[buf] = lds;
+            while (lds > cur && !attrSet[which].compare_exchange_weak(cur, lds))
+              {
+              }
           }
-        if (buf)
-          cellApplyBufK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
+        DFX_REQUIRE(buf || !dsc, "folded diagonal needs the buffer-addressed cell kernel");
+        if (which == 2)
+          cellApplyBufK<T, NT, MTW, true><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
-            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical);
+            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical, dsc,
+            xe ? xe : x.p);
+        else if (which == 1)
+          cellApplyBufK<T, NT, MTW, false><<<L.end - L.begin, 256, lds, s>>>(
+            op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
+            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical, nullptr,
+            x.p);
         else
           cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
@@ -681,14 +770,15 @@ namespace dfx
 
       template <class T, int NT>
       void launchCellMtw(hipStream_t s, const dfx_op *op, const Launch &L,
-                         const T *frag, View<const T> x, View<T> y, T *yE)
+                         const T *frag, View<const T> x, View<T> y, T *yE,
+                         const double *dsc, const T *xe)
       {
         switch (L.mtw)
           {
-            case 1: return launchCell<T, NT, 1>(s, op, L, frag, x, y, yE);
-            case 2: return launchCell<T, NT, 2>(s, op, L, frag, x, y, yE);
-            case 3: return launchCell<T, NT, 3>(s, op, L, frag, x, y, yE);
-            case 4: return launchCell<T, NT, 4>(s, op, L, frag, x, y, yE);
+            case 1: return launchCell<T, NT, 1>(s, op, L, frag, x, y, yE, dsc, xe);
+            case 2: return launchCell<T, NT, 2>(s, op, L, frag, x, y, yE, dsc, xe);
+            case 3: return launchCell<T, NT, 3>(s, op, L, frag, x, y, yE, dsc, xe);
+            case 4: return launchCell<T, NT, 4>(s, op, L, frag, x, y, yE, dsc, xe);
           }
         throw Error(DFX_E_UNSUPPORTED, "cell size above 256 dofs");
       }
@@ -734,16 +824,16 @@ namespace dfx
 
     template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                   View<const T> x, View<T> y, T *yE)
+                   View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe)
     {
       if (L.end <= L.begin || !x.B)
         return;
       switch (chooseNT<T>(x.B, L.kpadMax))
         {
-          case 1: return launchCellMtw<T, 1>(s, op, L, frag, x, y, yE);
-          case 2: return launchCellMtw<T, 2>(s, op, L, frag, x, y, yE);
-          case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE);
-          case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE);
+          case 1: return launchCellMtw<T, 1>(s, op, L, frag, x, y, yE, dsc, xe);
+          case 2: return launchCellMtw<T, 2>(s, op, L, frag, x, y, yE, dsc, xe);
+          case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE, dsc, xe);
+          case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE, dsc, xe);
         }
     }
 
@@ -777,10 +867,10 @@ namespace dfx
 
     template void cellApply<double>(hipStream_t, const dfx_op *, const Launch &,
                                     const double *, View<const double>, View<double>,
-                                    double *);
+                                    double *, const double *, const double *);
     template void cellApply<float>(hipStream_t, const dfx_op *, const Launch &,
-                                   const float *, View<const float>, View<float>,
-                                   float *);
+                                   const float *, View<const float>, View<float>, float *,
+                                   const double *, const float *);
     template void enrichedReduce<double>(hipStream_t, const dfx_op *, const double *,
                                          View<double>);
     template void enrichedReduce<float>(hipStream_t, const dfx_op *, const float *,

@@ -73,9 +73,15 @@ namespace dfx
 
     // ------------------------------------------------------- cell apply
     // one colour / size-class launch of the fused gather -> MFMA -> scatter
+    // dsc (optional): the launch gathers diag o X for the classical rows
+    // (M^-1 folded into the gather) and the enriched rows from xe
     template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                   View<const T> x, View<T> y, T *yE);
+                   View<const T> x, View<T> y, T *yE, const double *dsc = nullptr,
+                   const T *xe = nullptr);
+    // whether launches on (x, y) take the buffer-addressed kernel
+    template <class T>
+    bool cellBufferPath(const dfx_op *op, View<const T> x, View<const T> y);
     // y[nC + g, j] = sum over the enriched dof's slots of yE[slot, j]
     template <class T>
     void enrichedReduce(hipStream_t s, const dfx_op *op, const T *yE, View<T> y);

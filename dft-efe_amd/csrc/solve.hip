@@ -212,8 +212,11 @@ namespace dfx
       k::fill<T>(s, rs, 0.0);
       for (u32 deg = 2; deg <= m; ++deg)
         {
-          applyOp<T>(BInv, rn, s1);
-          applyOp<T>(A, s1, s2);
+          if (!applyInvThenCellwise<T>(BInv, A, rn, s1, s2))
+            {
+              applyOp<T>(BInv, rn, s1);
+              applyOp<T>(A, s1, s2);
+            }
           k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb, a1[deg],
                         a2[deg], c0);
           std::swap(rn, rs);

@@ -115,6 +115,8 @@ _sig("dfx_problem_sizes", _vp, _vp)
 _sig("dfx_problem_get", _vp, C.c_int, _vp)
 _sig("dfx_problem_create_operators", _vp, _u32, _pp, _pp, _pp, _pp, _pp, _pp)
 _sig("dfx_problem_destroy", _vp)
+_sig("dfx_request_footprint", _vp, _vp)
+_sig("dfx_serve", _vp, _u32, _vp, _vp, _vp)
 
 
 def exported_symbols():
@@ -639,3 +641,83 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- serving
+class _Request(C.Structure):
+    _fields_ = [("spec", _ProblemSpec), ("numStates", C.c_uint32), ("tolerance", C.c_double),
+                ("numElectrons", C.c_double), ("maxPasses", C.c_uint32),
+                ("guessSeed", C.c_uint64), ("arrivalSeconds", C.c_double)]
+
+
+class _RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("passes", C.c_uint32), ("admitSeconds", C.c_double),
+                ("doneSeconds", C.c_double), ("footprintBytes", C.c_uint64),
+                ("slot", C.c_uint32)]
+
+
+class _ServeParams(C.Structure):
+    _fields_ = [("device", C.c_int), ("maxSlots", C.c_uint32),
+                ("memoryBudgetBytes", C.c_uint64), ("precision", C.c_int)]
+
+
+def _request_struct(r, keep):
+    """r: dict with p, n, lo, hi, atoms [(x, y, z, Z)], states, tol, electrons,
+    max_passes, seed, arrival (and optional enrich / rcut / smooth / v_smear)."""
+    atoms = r["atoms"]
+    xyz = np.ascontiguousarray([a[:3] for a in atoms], dtype=np.float64).reshape(-1)
+    Z = np.ascontiguousarray([int(a[3]) for a in atoms], dtype=np.int32)
+    keep.append((xyz, Z))
+    q = _Request()
+    s = q.spec
+    s.p = int(r["p"])
+    s.n = (C.c_uint32 * 3)(*[int(v) for v in r["n"]])
+    s.lo = (C.c_double * 3)(*[float(v) for v in r["lo"]])
+    s.hi = (C.c_double * 3)(*[float(v) for v in r["hi"]])
+    s.nq, s.nqEnr, s.nAtoms = 0, 0, len(atoms)
+    s.atomXYZ = xyz.ctypes.data if len(atoms) else None
+    s.atomZ = Z.ctypes.data if len(atoms) else None
+    s.vSmear = float(r.get("v_smear", 0.5))
+    s.enrich = 1 if r.get("enrich", True) else 0
+    s.rcut, s.smooth = float(r.get("rcut", 4.0)), float(r.get("smooth", 2.0))
+    s.orthogonalize = 1 if r.get("orthogonalize", True) else 0
+    q.numStates = int(r["states"])
+    q.tolerance = float(r.get("tol", 1e-6))
+    q.numElectrons = float(r["electrons"])
+    q.maxPasses = int(r.get("max_passes", 50))
+    q.guessSeed = int(r.get("seed", 1))
+    q.arrivalSeconds = float(r.get("arrival", 0.0))
+    return q
+
+
+def request_footprint(r):
+    """Estimated peak device bytes of one request (what admission counts)."""
+    keep = []
+    q = _request_struct(r, keep)
+    out = C.c_uint64(0)
+    rc = _lib.dfx_request_footprint(C.byref(q), C.byref(out))
+    if rc != 0:
+        raise DfxError(rc, "dfx_request_footprint")
+    return int(out.value)
+
+
+def serve(requests, max_slots=4, memory_budget=200 << 30, precision=PREC_MIXED, device=0):
+    """Run a stream of independent Kohn-Sham eigenproblems with continuous
+    batching (dfx_serve).  Returns (energies per request, result dicts)."""
+    keep = []
+    arr = (_Request * len(requests))(*[_request_struct(r, keep) for r in requests])
+    res = (_RequestResult * len(requests))()
+    total = sum(int(r["states"]) for r in requests)
+    en = np.zeros(max(total, 1))
+    p = _ServeParams(int(device), int(max_slots), int(memory_budget), int(precision))
+    rc = _lib.dfx_serve(C.byref(p), len(requests), arr, _ptr(en), res)
+    if rc < 0:
+        raise DfxError(rc, "dfx_serve failed (a request hit a library error)")
+    out, off = [], 0
+    energies = []
+    for r, x in zip(requests, res):
+        k = int(r["states"])
+        energies.append(en[off:off + k].copy())
+        off += k
+        out.append({f: getattr(x, f) for f, _ in _RequestResult._fields_})
+    return energies, out

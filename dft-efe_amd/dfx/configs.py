@@ -87,6 +87,31 @@ def config(name):
     raise KeyError(name)
 
 
+def serving_stream(seed=5, n_requests=256, rate=4.0, p=3):
+    """The C5 request stream: Poisson arrivals at `rate` per second; each
+    request has 1-8 atoms (Z in {1, 6}, minimum separation 2 bohr) in a
+    16-28 bohr cube, a 6^3-14^3 mesh at order p, 5-40 wanted states (raised to
+    hold 0.6 Ne + 2 when the atoms bring more electrons) and a 1e-6 residual
+    tolerance.  All draws come from SplitMix64(seed)."""
+    s = Stream(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += -np.log(1.0 - s.uniform()) / rate
+        na = 1 + min(int(s.uniform() * 8), 7)
+        L = 16.0 + 12.0 * s.uniform()
+        nc = 6 + min(int(s.uniform() * 9), 8)
+        k = 5 + min(int(s.uniform() * 36), 35)
+        atoms = place_atoms(seed * 1000003 + i, na, -L / 2, L / 2, (1, 6), 2.0,
+                            margin=min(6.0, L / 4))
+        ne = sum(a[3] for a in atoms)
+        k = min(40, max(k, int(np.ceil(0.6 * ne)) + 2))
+        out.append(dict(p=p, n=(nc, nc, nc), lo=(-L / 2,) * 3, hi=(L / 2,) * 3, atoms=atoms,
+                        enrich=True, states=k, electrons=float(ne), tol=1e-6, max_passes=100,
+                        seed=seed * 7919 + i, arrival=t))
+    return out
+
+
 def poisson_rhs_charges(seed, n_centres=4, lo=-10.0, hi=10.0):
     """(centres, charges) of one Poisson right-hand side (config c2)."""
     s = Stream(seed)

@@ -373,6 +373,50 @@ int dfx_problem_create_operators(dfx_problem *prob, uint32_t flags, dfx_op **H,
                                  dfx_op **MLump, dfx_constraints **cons);
 int dfx_problem_destroy(dfx_problem *prob);
 
+/* ======================================================================
+ * Serving: a stream of independent Kohn-Sham eigenproblems on one GPU with
+ * continuous batching (no counterpart in the reference; each request is the
+ * problem build plus KohnShamEigenSolver::solve,
+ * src/ksdft/KohnShamEigenSolver.t.cpp:214-536).  Requests are admitted in
+ * arrival order when they have arrived, a slot is free and their estimated
+ * device footprint fits in memoryBudgetBytes; each admitted request runs on
+ * its own context / HIP stream, so the work of all admitted requests
+ * overlaps on the device.  Results do not depend on the mix.
+ * ==================================================================== */
+typedef struct dfx_request
+{
+  dfx_problem_spec spec;       /* mesh, atoms, enrichment (caller-owned arrays) */
+  uint32_t numStates;
+  double   tolerance;          /* residual tolerance of the occupied states */
+  double   numElectrons;
+  uint32_t maxPasses;
+  uint64_t guessSeed;          /* SplitMix64 seeds of the subspace (seed) and Lanczos (seed+1) */
+  double   arrivalSeconds;     /* from the start of the stream */
+} dfx_request;
+typedef struct dfx_request_result
+{
+  int      status;             /* 0, an EigenSolverErrorCode (> 0) or DFX_E_* */
+  uint32_t passes;
+  double   admitSeconds;       /* from the start of the stream */
+  double   doneSeconds;
+  uint64_t footprintBytes;
+  uint32_t slot;
+} dfx_request_result;
+typedef struct dfx_serve_params
+{
+  int      device;
+  uint32_t maxSlots;           /* requests in flight (one host thread + stream each) */
+  uint64_t memoryBudgetBytes;  /* admission limit on the summed footprints */
+  int      precision;          /* DFX_PREC_* */
+} dfx_serve_params;
+/* estimated peak device bytes of one request (what admission counts) */
+int dfx_request_footprint(const dfx_request *r, uint64_t *bytes);
+/* runs the whole stream; energies: host, sum of numStates doubles in request
+ * order; results: one per request.  Returns the first library error (< 0) of
+ * any request, else 0 (per-request solver codes are in results). */
+int dfx_serve(const dfx_serve_params *p, uint32_t nRequests, const dfx_request *reqs,
+              double *energies, dfx_request_result *results);
+
 #ifdef __cplusplus
 }
 #endif

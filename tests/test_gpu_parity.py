@@ -191,6 +191,36 @@ def test_chebyshev_filter_fp64(ctx, probs, name, residual):
     assert ang.max() < TOL_ANGLE64
 
 
+@pytest.mark.parametrize("name", ["c1", "enr"])
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+def test_residual_filter_folded_inverse_is_bit_identical(ctx, probs, name, prec):
+    """The residual filter folds M^-1 into the H apply's gather when both share
+    homogeneous Dirichlet constraints.  With M^-1 built on a second, identical
+    constraints object the folding does not apply: both runs give the same bits."""
+    P = probs[name]
+    ops = Ops(ctx, P)
+    d = ops.d
+    cons2 = dfx.Constraints(ctx, P.nDofs, d["rows"], d["rowSizes"], d["rowOffsets"],
+                            d["cols"], d["weights"], d["inhom"])
+    nE = P.nEnriched
+    enr = d["enrMinv"] if nE else None
+    Minv2 = dfx.DiagonalOperator(ctx, d["dinv"], nE, enr, cons2)
+    nv = 8
+    X = guess(P, nv, seed=27)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    outs = []
+    for minv in (ops.Minv, Minv2):
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, 10, 1e-14, G, True, 0, prec)
+        eig = np.linspace(a, b, nv)
+        assert s.solve(ops.H, eig, V, True, ops.M, minv) == 0
+        outs.append((eig.copy(), V.to_numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_chebyshev_filter_blocks_equal_fused(ctx, probs):
     """Filtering block by block (eigenVectorBatchSize) gives the same bits as
     filtering all columns at once: columns are independent in every kernel."""

@@ -1,0 +1,50 @@
+"""Serving (continuous batching of independent KS eigenproblems): each result
+is the same alone and in the mix, and admission honours the memory budget."""
+import numpy as np
+import pytest
+
+import dfx
+from dfx import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(n=6, seed=11):
+    s = configs.Stream(seed)
+    out = []
+    for i in range(n):
+        na = 1 + (i % 2)
+        atoms = configs.place_atoms(seed * 100 + i, na, -8.0, 8.0, (1, 6), 2.0, margin=4.0)
+        out.append(dict(p=3, n=(4 + i % 3,) * 3, lo=(-8.0,) * 3, hi=(8.0,) * 3, atoms=atoms,
+                        enrich=True, states=10, electrons=float(sum(a[3] for a in atoms)),
+                        tol=1e-6, max_passes=100, seed=100 + i, arrival=0.02 * i))
+    return out
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    reqs = small_stream()
+    alone_e, alone_r = [], []
+    for r in reqs:  # one request at a time, nothing else on the device
+        e, res = dfx.serve([dict(r, arrival=0.0)], max_slots=1)
+        alone_e.append(e[0]); alone_r.append(res[0])
+    mix_e, mix_r = dfx.serve(reqs, max_slots=4)
+    for a, ra, m, rm in zip(alone_e, alone_r, mix_e, mix_r):
+        assert ra["status"] == 0 and rm["status"] == 0
+        assert ra["passes"] == rm["passes"]
+        assert np.array_equal(a, m)          # bit for bit
+    # the mix really overlapped: some request was admitted before another finished
+    spans = sorted((x["admitSeconds"], x["doneSeconds"]) for x in mix_r)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for r, x in zip(reqs, mix_r):
+        assert x["admitSeconds"] >= r["arrival"] - 1e-3
+
+
+def test_admission_respects_the_memory_budget():
+    reqs = small_stream(n=4, seed=12)
+    fps = [dfx.request_footprint(r) for r in reqs]
+    budget = max(fps) + min(fps) // 2   # never two requests at once
+    _, res = dfx.serve([dict(r, arrival=0.0) for r in reqs], max_slots=4,
+                       memory_budget=budget)
+    spans = sorted((x["admitSeconds"], x["doneSeconds"]) for x in res)
+    assert all(x["status"] == 0 for x in res)
+    assert all(spans[i + 1][0] >= spans[i][1] - 1e-6 for i in range(len(spans) - 1))
