@@ -159,6 +159,10 @@ namespace dfx
       (void)hipStreamSynchronize(ctx->stream);
     if (ctx->blas)
       rocblas_destroy_handle(ctx->blas);
+    for (auto &g : ctx->graphs)
+      if (g.second.exec)
+        (void)hipGraphExecDestroy(g.second.exec);
+    ctx->graphs.clear();
     ctx->scratchSmall.release();
     ctx->reducePartials.release();
     ctx->work.clear();

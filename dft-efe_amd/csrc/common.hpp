@@ -146,6 +146,14 @@ struct dfx_ctx
   dfx::DevBuf    scratchSmall; // host<->device small transfers (reductions)
   dfx::DevBuf    reducePartials;
   std::map<std::string, dfx::DevBuf> work; // named solver scratch
+  // captured degree loops of the Chebyshev filter, by name; `key` holds the
+  // buffers, sizes and scalars the graph was captured with
+  struct Graph
+  {
+    hipGraphExec_t        exec = nullptr;
+    std::vector<uint64_t> key;
+  };
+  std::map<std::string, Graph> graphs;
 };
 
 struct dfx_mv

@@ -488,7 +488,8 @@ class _KsParams(C.Structure):
                 ("waveFunctionBatchSize", C.c_uint32), ("isResidualChebyshevFilter", C.c_int),
                 ("precision", C.c_int), ("lanczosMaxKrylov", C.c_uint32),
                 ("numElectrons", C.c_double), ("smearingTemperature", C.c_double),
-                ("fermiEnergyTolerance", C.c_double), ("fracOccupancyTolerance", C.c_double)]
+                ("fermiEnergyTolerance", C.c_double), ("fracOccupancyTolerance", C.c_double),
+                ("useGraph", C.c_int)]
 
 
 class _KsResult(C.Structure):
@@ -509,14 +510,14 @@ class KohnShamEigenSolver:
                  polynomialDegree=0, waveFunctionBatchSize=0,
                  isResidualChebyshevFilter=True, precision=PREC_FP64, lanczosMaxKrylov=20,
                  numElectrons=0.0, smearingTemperature=500.0, fermiEnergyTolerance=1e-10,
-                 fracOccupancyTolerance=1e-3):
+                 fracOccupancyTolerance=1e-3, useGraph=False):
         self.ctx = ctx
         self.p = _KsParams(int(numWantedEigenvalues), int(maxChebyshevFilterPass),
                            int(polynomialDegree), float(eigenSolveResidualTolerance),
                            int(waveFunctionBatchSize), 1 if isResidualChebyshevFilter else 0,
                            int(precision), int(lanczosMaxKrylov), float(numElectrons),
                            float(smearingTemperature), float(fermiEnergyTolerance),
-                           float(fracOccupancyTolerance))
+                           float(fracOccupancyTolerance), 1 if useGraph else 0)
         self.guess = waveFunctionSubspaceGuess
         self.lanczosGuess = lanczosGuess
         self.result = None

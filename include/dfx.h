@@ -291,6 +291,7 @@ typedef struct dfx_ks_params
   double   smearingTemperature;       /* K */
   double   fermiEnergyTolerance;
   double   fracOccupancyTolerance;
+  int      useGraph;                  /* capture the filter's degree loop in a hipGraph */
 } dfx_ks_params;
 typedef struct dfx_ks_result
 {

@@ -221,6 +221,38 @@ def test_residual_filter_folded_inverse_is_bit_identical(ctx, probs, name, prec)
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+def test_filter_graph_replays_match_eager(ctx, probs, prec):
+    """The residual filter's degree loop captured in a hipGraph and replayed
+    (once per block, three blocks, two passes with other work in between)
+    gives the same bits as the eager launches."""
+    P = probs["enr"]
+    ops = Ops(ctx, P)
+    nv = 12
+    X = guess(P, nv, seed=41)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    outs = []
+    for use_graph in (False, True):
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, 9, 1e-14, G, True, 4, prec,
+                                             useGraph=use_graph)
+        eig = np.linspace(a, b, nv)
+        res = []
+        for _ in range(2):
+            assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+            res.append((eig.copy(), V.to_numpy()))
+            W = dfx.MultiVector(ctx, P.nDofs, nv)      # eager work between replays
+            ops.H.apply(mv(ctx, X), W)
+            G.copy_from(V)
+            s.reinit(eig[0], eig[-1], c, 9, 1e-14, G)
+        outs.append(res)
+    for (e0, v0), (e1, v1) in zip(outs[0], outs[1]):
+        assert np.array_equal(e0, e1) and np.array_equal(v0, v1)
+
+
 def test_chebyshev_filter_blocks_equal_fused(ctx, probs):
     """Filtering block by block (eigenVectorBatchSize) gives the same bits as
     filtering all columns at once: columns are independent in every kernel."""

@@ -40,9 +40,10 @@ def test_results_are_the_same_alone_and_in_the_mix():
 
 
 def test_admission_respects_the_memory_budget():
-    reqs = small_stream(n=4, seed=12)
+    reqs = [dict(r, n=(5, 5, 5)) for r in small_stream(n=4, seed=12)]
     fps = [dfx.request_footprint(r) for r in reqs]
-    budget = max(fps) + min(fps) // 2   # never two requests at once
+    budget = max(fps) + min(fps) // 2   # room for one request, never two
+    assert budget < 2 * min(fps)
     _, res = dfx.serve([dict(r, arrival=0.0) for r in reqs], max_slots=4,
                        memory_budget=budget)
     spans = sorted((x["admitSeconds"], x["doneSeconds"]) for x in res)
