@@ -77,7 +77,7 @@ namespace dfx
             // Y written (and read unless first touched), enriched partials
             r.bytes = sz * (double)l.sumN2 + 4.0 * (double)l.sumN + sz * B * (double)l.sumN +
                       sz * B * (2.0 * (double)l.sumNc - (double)l.sumFirst) +
-                      sz * B * (double)(l.sumN - l.sumNc);
+                      sz * B * (double)(l.sumN - l.sumNc) + (dsc ? 8.0 * (double)l.sumNc : 0.0);
             op->prof.push_back(r);
           }
       }
@@ -127,11 +127,20 @@ namespace dfx
         (reinterpret_cast<uintptr_t>(S1.p) & 15) != 0)
       return false;
     hipStream_t s = A->ctx->stream;
+    if (A->foldedUid != Dinv->uid)
+      {
+        // d[map[slot]] for every slot of the cell->dof map (classical slots
+        // only are read), so the kernel loads it beside the map
+        A->foldedDiag.ensure((size_t)A->sumNe * sizeof(double));
+        k::gatherDiag(s, Dinv->diag.as<double>(), A->map.as<u32>(), A->sumNe,
+                      A->nClassical, A->foldedDiag.as<double>());
+        A->foldedUid = Dinv->uid;
+      }
     k::p2c<T>(s, Dinv->cons, X);
     if (Dinv->nEnriched)
       k::enrichedBlock<T>(s, S1, Dinv->enrBlock.as<double>(),
                           View<const T>{X.p, X.N, X.B, X.ld}, A->nClassical, Dinv->nEnriched);
-    applyCellwise<T>(A, X, Y, Dinv->diag.as<double>(), S1.p);
+    applyCellwise<T>(A, X, Y, A->foldedDiag.as<double>(), S1.p);
     return true;
   }
   template bool applyInvThenCellwise<double>(dfx_op *, dfx_op *, View<double>, View<double>,

@@ -226,6 +226,17 @@ struct dfx_op
 {
   dfx_ctx     *ctx  = nullptr;
   int          kind = dfx::OP_CELLWISE;
+  // unique per operator object for the life of the process (cache keys)
+  uint64_t     uid  = nextUid();
+  static uint64_t nextUid()
+  {
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+  }
+  // diagonal of a diagonal operator gathered per cell slot (d[map[s]]) for
+  // folding it into this cell-wise operator's gather; built for foldedUid
+  dfx::DevBuf foldedDiag;
+  uint64_t    foldedUid = 0;
   dfx::u32     nDofs = 0, nClassical = 0;
   dfx_constraints *cons = nullptr;
   // ----- cell-wise

@@ -217,10 +217,9 @@ namespace dfx
           sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
         // M^-1 folded in: the MFMA's B operand (X_e) is scaled row by row,
         // diag on the classical rows and 1 elsewhere (exact)
-        if constexpr (DSC)
-          sDsc[threadIdx.x] = threadIdx.x < (unsigned)nc
-                                ? (T)dsc[map[cm.mapOff + threadIdx.x]]
-                                : (T)1;
+        if constexpr (DSC) // dsc holds d[map[slot]] per map slot
+          sDsc[threadIdx.x] = threadIdx.x < (unsigned)nc ? (T)dsc[cm.mapOff + threadIdx.x]
+                                                         : (T)1;
         if (threadIdx.x < 8)
           sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
         __syncthreads();
@@ -835,6 +834,29 @@ namespace dfx
           case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE, dsc, xe);
           case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE, dsc, xe);
         }
+    }
+
+    namespace
+    {
+      __global__ void gatherDiagK(const double *d, const u32 *map, u64 n, u32 nC, double *out)
+      {
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            const u32 m = map[t];
+            out[t]      = m < nC ? d[m] : 1.0;
+          }
+      }
+    } // namespace
+
+    void gatherDiag(hipStream_t s, const double *d, const u32 *map, u64 n, u32 nC,
+                    double *out)
+    {
+      if (!n)
+        return;
+      const u64 g = (n + 255) / 256;
+      gatherDiagK<<<(unsigned)(g > 65536 ? 65536 : g), 256, 0, s>>>(d, map, n, nC, out);
+      checkLaunch("gatherDiag");
     }
 
     template <class T>

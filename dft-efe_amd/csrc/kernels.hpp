@@ -79,6 +79,9 @@ namespace dfx
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
                    View<const T> x, View<T> y, T *yE, const double *dsc = nullptr,
                    const T *xe = nullptr);
+    // out[s] = d[map[s]] (map[s] < nC) else 1, for every slot of a cell->dof map
+    void gatherDiag(hipStream_t s, const double *d, const u32 *map, u64 n, u32 nC,
+                    double *out);
     // whether launches on (x, y) take the buffer-addressed kernel
     template <class T>
     bool cellBufferPath(const dfx_op *op, View<const T> x, View<const T> y);
