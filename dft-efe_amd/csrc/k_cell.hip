@@ -31,6 +31,18 @@
 #ifndef DFX_CELL_MINW
 #define DFX_CELL_MINW 2
 #endif
+#ifndef DFX_CELL_MINW_SMALL
+#define DFX_CELL_MINW_SMALL 4
+#endif
+#ifndef DFX_CELL_EARLY_SMALL
+#define DFX_CELL_EARLY_SMALL 0
+#endif
+#ifndef DFX_CELL_AREG_WIDE
+#define DFX_CELL_AREG_WIDE 0
+#endif
+#ifndef DFX_CELL_ORDER
+#define DFX_CELL_ORDER 0
+#endif
 #ifndef DFX_CELL_AREG_MAX
 #define DFX_CELL_AREG_MAX 128
 #endif
@@ -137,8 +149,19 @@ namespace dfx
       // rows, so offsets stay below 2 GiB when the cells' classical row spans
       // and the enriched block do (checked by the launcher).  The LDS tile is
       // KMAX rows (the class maximum) so the gather writes every row it loads.
+      // FP32 launches of at most two M tiles per wave (cells of <= 128 dofs)
+      // fit in 128 VGPRs when the read-modify-write loads are issued per tile
+      // in the epilogue: four workgroups per CU instead of three
+      template <class T, int NT, int MTW>
+      struct CellPolicy
+      {
+        static constexpr bool small = sizeof(T) == 4 && MTW <= 2 && NT > 1 && NT <= 4;
+        static constexpr int  minw  = small ? DFX_CELL_MINW_SMALL : DFX_CELL_MINW;
+        static constexpr bool early = small ? DFX_CELL_EARLY_SMALL : DFX_CELL_EARLY;
+      };
+
       template <class T, int NT, int MTW, bool DSC>
-      __global__ __launch_bounds__(256, DFX_CELL_MINW) void cellApplyBufK(
+      __global__ __launch_bounds__(256, (CellPolicy<T, NT, MTW>::minw)) void cellApplyBufK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
         const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
         u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC,
@@ -163,14 +186,15 @@ namespace dfx
         // (columns G apart) and the MFMA's B reads (16 consecutive columns)
         // fall in 16 distinct 16-byte bank groups
         auto swz = [](int col) { return (col & ~15) | ((col + (col >> 4)) & 15); };
-        constexpr bool EARLY    = DFX_CELL_EARLY && MTW * NT * 4 * (int)sizeof(T) <= 128;
+        constexpr bool EARLY    = CellPolicy<T, NT, MTW>::early && MTW * NT * 4 * (int)sizeof(T) <= 128;
         constexpr u32 S         = sizeof(T);
         constexpr int  NKBMAX   = KMAX / KB;
         // narrow chunks (NT == 1: few columns, little work to hide latency
         // behind) hold the wave's whole A operand in registers, requested
         // at the start; wider chunks use a D-deep register ring
-        constexpr bool AREG     = NT == 1 && MTW * NKBMAX * 4 <= // 4 VGPRs per vector
-                                                 (S == 4 ? DFX_CELL_AREG_MAX : DFX_CELL_AREG_MAX / 2);
+        constexpr bool AREG     = (NT == 1 && MTW * NKBMAX * 4 <= // 4 VGPRs per vector
+                                                  (S == 4 ? DFX_CELL_AREG_MAX : DFX_CELL_AREG_MAX / 2)) ||
+                              (NT > 1 && MTW * NKBMAX * 4 <= DFX_CELL_AREG_WIDE);
         constexpr int  NRING    = AREG ? NKBMAX : D;
         extern __shared__ __align__(16) unsigned char smemRaw[];
         T *sX = reinterpret_cast<T *>(smemRaw);
@@ -209,9 +233,10 @@ namespace dfx
             }
         };
         vec_t ring[NRING][MTW];
+        if constexpr (!DFX_CELL_ORDER)
 #pragma unroll
-        for (int d = 0; d < NRING; ++d)
-          loadA(d, ring[d]);
+          for (int d = 0; d < NRING; ++d)
+            loadA(d, ring[d]);
 
         if (threadIdx.x < n)
           sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
@@ -261,6 +286,15 @@ namespace dfx
                         v[u][r] = kk >= nc ? ve : vc;
                       }
                   }
+              }
+            if constexpr (DFX_CELL_ORDER)
+              {
+                // the A operand is requested after the gather: loads complete
+                // in order, so the gather's LDS stores wait for the gather only
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int d = 0; d < NRING; ++d)
+                  loadA(d, ring[d]);
               }
 
 #pragma unroll
@@ -379,7 +413,7 @@ namespace dfx
                         acc[t][nt] = M::op(a[t][s], b[nt][s], acc[t][nt]);
               }
             // the next column chunk starts again at K block 0
-            if (c0 + BC < B)
+            if (!DFX_CELL_ORDER && c0 + BC < B)
 #pragma unroll
               for (int d = 0; d < NRING; ++d)
                 loadA(d, ring[d]);
