@@ -41,7 +41,7 @@ def traffic_from_profiles(kind):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals[f"traffic_bytes_per_launch_{kind}"])
+        return float(vals[f"traffic_bytes_per_colour_step_{kind}"])
     except (KeyError, ValueError):
         return None
 
@@ -345,7 +345,9 @@ def main():
         "setup_s": setup_s,
         "serving": serving,
         "roofline": {"bound": bound,
-                     "kernel": "cellApplyBufK<float> (fused gather / FP32 MFMA / scatter)",
+                     "kernel": ("cellApplyBufK<float> (fused gather / FP32 MFMA / scatter); "
+                                "one launch unit = one colour step, its size-class "
+                                "launches running side by side"),
                      "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic,
                      "launches": prof32["launches"],

@@ -57,28 +57,59 @@ namespace dfx
     const T *frag = fragOf<T>(op);
     const auto &L = sizeof(T) == 8 ? op->launches64 : op->launches32;
     View<const T> Xc{X.p, X.N, X.B, X.ld};
-    for (const Launch &l : L)
+    // the size classes of one colour touch disjoint dofs: the second one of a
+    // colour runs on the context's second stream, beside the first, so that
+    // the tail of one launch overlaps the other
+    static const bool forkEnv = [] {
+      const char *e = getenv("DFX_CELL_FORK");
+      return !e || atoi(e) != 0;
+    }();
+    const bool fork = forkEnv && op->ctx->forkClasses;
+    const bool prof = op->profiling && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
+    ProfRec    r{};
+    for (size_t li = 0; li < L.size(); ++li)
       {
-        ProfRec    r{};
-        const bool prof = op->profiling && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
-        if (prof)
+        const Launch &l     = L[li];
+        const bool    first = li == 0 || L[li - 1].color != l.color;
+        const bool    last  = li + 1 == L.size() || L[li + 1].color != l.color;
+        const bool    side  = fork && !first;
+        hipStream_t   ls    = side ? op->ctx->stream2 : s;
+        // one profile record per colour step (its launches, side by side),
+        // bracketed on the main stream
+        if (prof && first)
           {
+            r = ProfRec{};
             DFX_HIP(hipEventCreate(&r.a));
             DFX_HIP(hipEventCreate(&r.b));
             DFX_HIP(hipEventRecord(r.a, s));
           }
-        k::cellApply<T>(s, op, l, frag, Xc, Y, yE, dsc, xe);
+        if (fork && first && !last)
+          {
+            // the next launch (same colour) will go to stream2 after this point
+            DFX_HIP(hipEventRecord(op->ctx->forkEv, s));
+            DFX_HIP(hipStreamWaitEvent(op->ctx->stream2, op->ctx->forkEv, 0));
+          }
+        k::cellApply<T>(ls, op, l, frag, Xc, Y, yE, dsc, xe);
+        if (side)
+          {
+            DFX_HIP(hipEventRecord(op->ctx->joinEv, ls));
+            DFX_HIP(hipStreamWaitEvent(s, op->ctx->joinEv, 0));
+          }
         if (prof)
           {
-            DFX_HIP(hipEventRecord(r.b, s));
             const double B = X.B, sz = sizeof(T);
-            r.flops = 2.0 * B * (double)l.sumN2;
+            r.flops += 2.0 * B * (double)l.sumN2;
             // cell matrices once, the map, gathered X rows, classical rows of
             // Y written (and read unless first touched), enriched partials
-            r.bytes = sz * (double)l.sumN2 + 4.0 * (double)l.sumN + sz * B * (double)l.sumN +
-                      sz * B * (2.0 * (double)l.sumNc - (double)l.sumFirst) +
-                      sz * B * (double)(l.sumN - l.sumNc) + (dsc ? 8.0 * (double)l.sumNc : 0.0);
-            op->prof.push_back(r);
+            r.bytes += sz * (double)l.sumN2 + 4.0 * (double)l.sumN + sz * B * (double)l.sumN +
+                       sz * B * (2.0 * (double)l.sumNc - (double)l.sumFirst) +
+                       sz * B * (double)(l.sumN - l.sumNc) +
+                       (dsc ? 8.0 * (double)l.sumNc : 0.0);
+            if (last)
+              {
+                DFX_HIP(hipEventRecord(r.b, s));
+                op->prof.push_back(r);
+              }
           }
       }
     if (op->nEnrSlots)
@@ -175,6 +206,15 @@ namespace dfx
     ctx->scratchSmall.release();
     ctx->reducePartials.release();
     ctx->work.clear();
+    if (ctx->stream2)
+      {
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamDestroy(ctx->stream2);
+      }
+    if (ctx->forkEv)
+      (void)hipEventDestroy(ctx->forkEv);
+    if (ctx->joinEv)
+      (void)hipEventDestroy(ctx->joinEv);
     if (ctx->stream)
       (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -212,6 +252,9 @@ extern "C"
       DFX_HIP(hipSetDevice(device));
       ctx->device = device;
       DFX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      DFX_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+      DFX_HIP(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming));
+      DFX_HIP(hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming));
       if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
         throw Error(DFX_E_HIP, "rocblas_create_handle failed");
       rocblas_set_stream(ctx->blas, ctx->stream);

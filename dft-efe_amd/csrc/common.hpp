@@ -141,6 +141,11 @@ struct dfx_ctx
   std::atomic<int> refs{1};
   int            device = 0;
   hipStream_t    stream = nullptr;
+  // second stream for the size classes of one colour, which run side by side
+  // (forked from and joined back into `stream` with the two events)
+  hipStream_t    stream2 = nullptr;
+  hipEvent_t     forkEv = nullptr, joinEv = nullptr;
+  bool           forkClasses = true; // false: every launch on `stream` (serving slots)
   rocblas_handle blas   = nullptr;
   std::string    err;
   dfx::DevBuf    scratchSmall; // host<->device small transfers (reductions)
@@ -212,6 +217,7 @@ namespace dfx
     u64 sumN;   // sum of n_e
     u64 sumNc;  // sum of classical rows
     u64 sumFirst = 0; // classical rows stored on first touch (not read)
+    u32 color    = 0; // launches of one colour touch disjoint classical dofs
   };
 
   // one profiled kernel launch

@@ -214,8 +214,10 @@ namespace dfx
             sc += nc[op->hOrder[r]];
             ++r;
           }
-        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64, s2, s1, sc});
-        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32, s2, s1, sc});
+        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64, s2, s1, sc, 0,
+                                  (u32)color[e0]});
+        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32, s2, s1, sc, 0,
+                                  (u32)color[e0]});
         q = r;
       }
 

@@ -103,6 +103,9 @@ namespace dfx
       int      rc  = dfx_ctx_create(device, &ctx);
       if (rc != 0)
         return rc;
+      // slots already run side by side: one stream per request keeps the
+      // process within the device's hardware queues
+      ctx->forkClasses = false;
       dfx_problem     *P = nullptr;
       dfx_op          *H = nullptr, *M = nullptr, *K = nullptr, *Mi = nullptr, *Ml = nullptr;
       dfx_constraints *cons = nullptr;

@@ -197,11 +197,13 @@ int dfx_op_get_info(const dfx_op *op, dfx_op_info *info);
 /* Download the device-resident cell->dof map in the caller's cell order
  * (bit-exact round trip of what dfx_op_create_cellwise uploaded). */
 int dfx_op_export_map(const dfx_op *op, uint32_t *cellNdofs, uint32_t *cellDofMap);
-/* Per-launch timing of this operator's fused cell-apply kernels: with
- * profiling on, every kernel launch is bracketed by HIP events on the
- * context's stream.  get_profile synchronizes, and reports the launches,
- * their summed kernel time, and the algorithmic flops (2 B sum n_e^2) and
- * bytes (matrices + map + gathered X + scattered Y) of those launches. */
+/* Timing of this operator's fused cell-apply kernels per colour step (the
+ * size-class launches of one colour, which run side by side on the
+ * context's two streams): with profiling on, every colour step is bracketed
+ * by HIP events on the context's stream.  get_profile synchronizes, and
+ * reports the colour steps ("launches"), their summed time, and the
+ * algorithmic flops (2 B sum n_e^2) and bytes (matrices + map + gathered X +
+ * scattered Y) of their cells. */
 typedef struct dfx_op_profile
 {
   uint64_t launches;
