@@ -115,6 +115,58 @@ def serving_run(n_requests, rate, slots, seed=5):
                       "6^3-14^3 cells at p=3, 5-40 states, 1e-6 residual"}
 
 
+def poisson_run(ctx):
+    """C2: batched Poisson CG on the electrostatics path -- 16^3 cells at p=3
+    in a 20-bohr cube, zero Dirichlet, Jacobi preconditioner, relative
+    tolerance 1e-10, 8 right-hand sides 4 pi M rho_k with rho_k four
+    normalized Gaussians (width 0.3 bohr, charges 1-6, centres from seed 100+k)
+    sampled at the nodes.  The same cell kernel applies K."""
+    import numpy as np
+    import dfx
+    from dfx import configs
+    cfg = configs.config("c2")
+    t0 = time.perf_counter()
+    prob = dfx.Problem(ctx, p=cfg["p"], n=cfg["n"], lo=cfg["lo"], hi=cfg["hi"], atoms=[],
+                       enrich=False, orthogonalize=False)
+    H, M, K, Minv, Mlump, cons = prob.operators(keep_fp32=False)
+    setup = time.perf_counter() - t0
+    N, nr, w = prob.nDofs, cfg["rhs"], cfg["width"]
+    nd = prob.get("cellNdofs").astype(np.int64)
+    cmap = prob.get("cellMap").astype(np.int64)
+    Kc = prob.get("K")
+    n0 = int(nd[0])
+    assert np.all(nd == n0)
+    diag = np.zeros(N)
+    np.add.at(diag, cmap, Kc.reshape(-1, n0, n0).diagonal(axis1=1, axis2=2).reshape(-1))
+    diag[prob.get("rows")] = 1.0
+    pc = dfx.DiagonalOperator(ctx, 1.0 / diag)
+    xyz = prob.get("nodeXYZ").reshape(-1, 3)
+    rho = np.zeros((N, nr))
+    for k in range(nr):
+        cen, q = configs.poisson_rhs_charges(100 + k)
+        for c, qq in zip(cen, q):
+            r2 = ((xyz - c) ** 2).sum(axis=1)
+            rho[:, k] += qq * (2 * np.pi * w * w) ** -1.5 * np.exp(-r2 / (2 * w * w))
+    R = dfx.MultiVector.from_numpy(ctx, 4 * np.pi * rho)
+    b = dfx.MultiVector(ctx, N, nr)
+    M.apply(R, b)
+    x = dfx.MultiVector(ctx, N, nr)
+    x.set(0.0)
+    cg = dfx.CGLinearSolver(ctx, 10000, 1e-30, cfg["rel_tol"], 1e10)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rc = cg.solve(K, pc, b, x)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    bn = b.l2norms()
+    return {"workload": "c2: 16^3 cells, p=3, %d dofs, %d right-hand sides, Jacobi, rel. tol %g"
+                        % (N, nr, cfg["rel_tol"]),
+            "rc": int(rc), "iterations": int(cg.iterations), "solve_ms": t * 1e3,
+            "ms_per_iteration": t * 1e3 / max(cg.iterations, 1),
+            "max_relative_residual": float(np.max(cg.residualNorms / bn)),
+            "setup_s": round(setup, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +342,11 @@ def main():
                 "residual_tol": 1e-6,
                 "fp64_solve_s": round(out[0][4], 2), "mixed_solve_s": round(out[1][4], 2)}
 
+    log("Poisson CG (c2) ...")
+    poisson = poisson_run(ctx)
+    log(f"Poisson: rc {poisson['rc']}, {poisson['iterations']} iterations, "
+        f"{poisson['solve_ms']:.1f} ms ({poisson['ms_per_iteration']:.3f} ms / iteration)")
+
     serving = None
     if args.serve_requests:
         log(f"serving {args.serve_requests} requests of the C5 stream ...")
@@ -344,6 +401,7 @@ def main():
         "convergence": conv,
         "setup_s": setup_s,
         "serving": serving,
+        "poisson": poisson,
         "roofline": {"bound": bound,
                      "kernel": ("cellApplyBufK<float> (fused gather / FP32 MFMA / scatter); "
                                 "one launch unit = one colour step, its size-class "

@@ -332,7 +332,7 @@ struct dfx_problem
   u32 p = 0, nCells = 0, nClassical = 0, nEnriched = 0, nDofs = 0;
   std::vector<u32>    cellNdofs, cellMap, mapOff;
   std::vector<u64>    matOff;
-  std::vector<double> cellLo, cellH;
+  std::vector<double> cellLo, cellH, nodeXYZ;
   std::vector<u32>    rows, rowSizes, rowOffsets, cols;
   std::vector<double> weights, inhom, mgll, dinv, enrM, enrMinv;
   std::vector<std::vector<u32>> atomCells; // enrichment supports
@@ -382,6 +382,26 @@ namespace
                   }
           }
     P->nClassical = next;
+    {
+      // node coordinates: lattice index cx p + a lies at lo + h (cx + (x_a + 1) / 2)
+      std::vector<double> gx0, gw0;
+      gllRule(np, gx0, gw0);
+      P->nodeXYZ.assign(3 * (size_t)P->nClassical, 0.0);
+      const u64 NN[3] = {NX, NY, NZ};
+      for (u64 k = 0; k < NZ; ++k)
+        for (u64 j = 0; j < NY; ++j)
+          for (u64 i = 0; i < NX; ++i)
+            {
+              const u64 ijk[3] = {i, j, k};
+              const u32 id     = lat[i + NX * (j + NY * k)];
+              for (int d = 0; d < 3; ++d)
+                {
+                  const u64 c = ijk[d] / p, a = ijk[d] % p;
+                  (void)NN;
+                  P->nodeXYZ[3 * (size_t)id + d] = S.lo[d] + h[d] * ((double)c + 0.5 * (gx0[a] + 1.0));
+                }
+            }
+    }
     // Dirichlet: every lattice point on the domain boundary, rows ascending
     for (u64 k = 0; k < NZ; ++k)
       for (u64 j = 0; j < NY; ++j)
@@ -890,6 +910,7 @@ extern "C"
           case DFX_PROB_MGLL: return cpu(P->mgll.data(), P->mgll.size() * 8);
           case DFX_PROB_ENR_MINV: return cpu(P->enrMinv.data(), P->enrMinv.size() * 8);
           case DFX_PROB_ENR_M: return cpu(P->enrM.data(), P->enrM.size() * 8);
+          case DFX_PROB_NODE_XYZ: return cpu(P->nodeXYZ.data(), P->nodeXYZ.size() * 8);
           case DFX_PROB_ALPHA:
             return P->alpha.p ? dev(P->alpha, P->alpha.bytes)
                               : cpu(nullptr, 0);

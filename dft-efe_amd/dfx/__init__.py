@@ -573,6 +573,7 @@ _PROB_ARRAYS = {  # name: (id, dtype, size key)
     "K": (10, np.float64, "matLen"), "dinv": (11, np.float64, "nDofs"),
     "mgll": (12, np.float64, "nDofs"), "enrMinv": (13, np.float64, "nE2"),
     "enrM": (14, np.float64, "nE2"), "alpha": (15, np.float64, "nEC"),
+    "nodeXYZ": (16, np.float64, "nC3"),
 }
 
 
@@ -608,7 +609,8 @@ class Problem:
 
     def get(self, name):
         wid, dt, key = _PROB_ARRAYS[name]
-        n = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical}.get(
+        n = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical,
+             "nC3": 3 * self.nClassical}.get(
             key, getattr(self, key, 0))
         out = np.zeros(n, dtype=dt)
         if n:

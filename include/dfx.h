@@ -363,6 +363,7 @@ typedef struct dfx_problem_spec
 #define DFX_PROB_ENR_MINV   13 /* double [nEnriched^2] (M^ee)^-1, column-major */
 #define DFX_PROB_ENR_M      14 /* double [nEnriched^2] M^ee */
 #define DFX_PROB_ALPHA      15 /* double [nEnriched * nClassical] orthogonalization coefficients */
+#define DFX_PROB_NODE_XYZ   16 /* double [3 nClassical] node coordinates (x, y, z per node) */
 
 int dfx_problem_build(dfx_ctx *ctx, const dfx_problem_spec *spec, dfx_problem **out);
 /* sizes[8]: nCells, nDofs, nClassical, nEnriched, sum n_e, sum n_e^2, nRows, nEntries */

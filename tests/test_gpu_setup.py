@@ -40,6 +40,8 @@ def test_problem_builder_matches_oracle(ctx, name):
         assert rel(G.get(key), d[key]) < 1e-12, key
     assert rel(G.get("dinv"), d["dinv"]) < 1e-14
     assert rel(G.get("mgll"), d["mgll"]) < 1e-14
+    xyz = G.get("nodeXYZ").reshape(-1, 3)
+    assert np.abs(xyz - R.node_xyz()).max() <= 1e-13 * np.abs(R.node_xyz()).max()
     if G.nEnriched:
         a_g, a_r = G.get("alpha").reshape(G.nEnriched, -1), R.alpha()
         assert rel(a_g, a_r) < 1e-9
