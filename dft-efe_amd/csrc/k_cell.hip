@@ -40,6 +40,18 @@
 #ifndef DFX_CELL_AREG_WIDE
 #define DFX_CELL_AREG_WIDE 0
 #endif
+#ifndef DFX_CELL_MINW_MID
+#define DFX_CELL_MINW_MID 3
+#endif
+#ifndef DFX_CELL_ROT
+#define DFX_CELL_ROT 0
+#endif
+#ifndef DFX_CELL_DSC_GATHER
+#define DFX_CELL_DSC_GATHER 0
+#endif
+#ifndef DFX_CELL_MFMA_ALL
+#define DFX_CELL_MFMA_ALL 0
+#endif
 #ifndef DFX_CELL_ORDER
 #define DFX_CELL_ORDER 0
 #endif
@@ -156,11 +168,15 @@ namespace dfx
       struct CellPolicy
       {
         static constexpr bool small = sizeof(T) == 4 && MTW <= 2 && NT > 1 && NT <= 4;
-        static constexpr int  minw  = small ? DFX_CELL_MINW_SMALL : DFX_CELL_MINW;
+        // FP32 cells of 129..192 dofs: three workgroups fit the LDS of a CU
+        static constexpr bool mid   = sizeof(T) == 4 && MTW == 3 && NT > 1 && NT <= 4;
+        static constexpr int  minw  = small ? DFX_CELL_MINW_SMALL
+                                    : mid   ? DFX_CELL_MINW_MID
+                                            : DFX_CELL_MINW;
         static constexpr bool early = small ? DFX_CELL_EARLY_SMALL : DFX_CELL_EARLY;
       };
 
-      template <class T, int NT, int MTW, bool DSC>
+      template <class T, int NT, int MTW, bool DSC, bool TAIL>
       __global__ __launch_bounds__(256, (CellPolicy<T, NT, MTW>::minw)) void cellApplyBufK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
         const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
@@ -174,13 +190,9 @@ namespace dfx
         constexpr int G         = M::G;
         constexpr int KB        = 4 * G;
         constexpr int BC        = 16 * NT;
-        constexpr int VC        = BC / G;
         constexpr int KMAX      = 64 * MTW;
         constexpr int DEPTH     = NT == 1 ? DFX_CELL_DEPTH1 : DFX_CELL_DEPTH;
         constexpr int D         = (KMAX / KB) < DEPTH ? (KMAX / KB) : DEPTH;
-        constexpr int UNITS     = (KMAX / G) * VC; // G x G gather units of the tile
-        constexpr int UPT       = (UNITS + 255) / 256;
-        static_assert(UNITS % 64 == 0, "gather units must fill whole waves");
         // LDS column position: within each 16-column segment the column is
         // rotated by the segment index, so that both the gather's stores
         // (columns G apart) and the MFMA's B reads (16 consecutive columns)
@@ -209,6 +221,10 @@ namespace dfx
         const int      mt   = (n + 15) >> 4;
         const int      nkb  = (n + KB - 1) / KB;
         const int      w    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+        // M tiles wt, wt + 4, ... of the cell: when mt is not a multiple of
+        // 4 some waves own one tile more; the rotation by a hash of the cell
+        // position spreads those waves over the CU's four SIMDs
+        const int      wt   = DFX_CELL_ROT ? (int)((w + ((pos * 2654435761u) >> 30)) & 3u) : w;
         const int      l    = threadIdx.x & 63;
         const rsrc_t   rA   = rsrc(frag + (S == 8 ? cm.fragOff64 : cm.fragOff32),
                                    (u32)(mt * nkb * 64 * G) * S);
@@ -225,7 +241,7 @@ namespace dfx
 #pragma unroll
           for (int t = 0; t < MTW; ++t)
             {
-              const int mtile = w + 4 * t;
+              const int mtile = wt + 4 * t;
               const u32 off   = (mtile < mt && kb < nkb)
                                   ? ((u32)(mtile * nkb + kb) * 64 + l) * (G * S)
                                   : OFF_NONE;
@@ -250,11 +266,19 @@ namespace dfx
         __syncthreads();
 
         const int cl = l & 15;
-        auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(l, v); };
+        auto rowOf = [&](int t, int v) { return (wt + 4 * t) * 16 + M::row(l, v); };
 
-        for (u32 c0 = 0; c0 < B; c0 += BC)
-          {
-            const int bw = min(BC, (int)(B - c0));
+        // one column chunk [c0, c0 + bw) with NTL live 16-column tiles (the
+        // LDS row stride stays BC).  TAIL kernels (B % BC in [1, 16]) run the
+        // last chunk with NTL = 1, so it issues no MFMA, LDS read or store
+        // for columns past B; a separate instantiation because the second
+        // path costs registers
+        auto chunk = [&](auto ntl, const u32 c0, const int bw) {
+            constexpr int NTL   = decltype(ntl)::value;
+            constexpr int VCL   = 16 * NTL / G;
+            constexpr int UNITS = (KMAX / G) * VCL; // G x G gather units of the tile
+            constexpr int UPT   = (UNITS + 255) / 256;
+            static_assert(UNITS % 64 == 0, "gather units must fill whole waves");
             // ---- gather (columns are whole 16-byte vectors: B % G == 0).  A
             // unit is G rows x G columns: G 16-byte loads, transposed in
             // registers into G 16-byte LDS stores (one per column, holding G
@@ -264,7 +288,7 @@ namespace dfx
             for (int u = 0; u < UPT; ++u)
               {
                 const int  t    = threadIdx.x + u * 256;
-                const int  q    = t / VC, c = (t - q * VC) * G; // K group, first column
+                const int  q    = t / VCL, c = (t - q * VCL) * G; // K group, first column
                 const bool live = t < UNITS;                     // wave-uniform
 #pragma unroll
                 for (int r = 0; r < G; ++r)
@@ -286,6 +310,15 @@ namespace dfx
                         v[u][r] = kk >= nc ? ve : vc;
                       }
                   }
+                if constexpr (DSC && DFX_CELL_DSC_GATHER)
+                  {
+                    // rows q G .. q G + G - 1 of the unit scaled by their
+                    // diag (1 past nc); kk < KMAX <= 256 stays in sDsc
+                    const vec_t sc = *reinterpret_cast<const vec_t *>(sDsc + (live ? q * G : 0));
+#pragma unroll
+                    for (int r = 0; r < G; ++r)
+                      v[u][r] = sc[r] * v[u][r];
+                  }
               }
             if constexpr (DFX_CELL_ORDER)
               {
@@ -301,7 +334,7 @@ namespace dfx
             for (int u = 0; u < UPT; ++u)
               {
                 const int t = threadIdx.x + u * 256;
-                const int q = t / VC, c = (t - q * VC) * G;
+                const int q = t / VCL, c = (t - q * VCL) * G;
                 if (t < UNITS)
 #pragma unroll
                   for (int g = 0; g < G; ++g)
@@ -315,7 +348,7 @@ namespace dfx
               }
 
             // ---- old values of the classical rows touched before
-            T    old[EARLY ? MTW : 1][EARLY ? NT : 1][4];
+            T    old[EARLY ? MTW : 1][EARLY ? NTL : 1][4];
             auto loadOld = [&](int t, T (*o)[4]) {
 #pragma unroll
               for (int vv = 0; vv < 4; ++vv)
@@ -326,7 +359,7 @@ namespace dfx
                   const bool rmw  = (row < nc) & !((fb >> (row & 31)) & 1u);
                   const u32  base = ((m - lo) * ldy + c0 + cl) * S;
 #pragma unroll
-                  for (int nt = 0; nt < NT; ++nt)
+                  for (int nt = 0; nt < NTL; ++nt)
                     o[nt][vv] = BF::load(rY, (rmw & (nt * 16 + cl < bw)) ? base + nt * 16 * S
                                                                         : OFF_NONE);
                 }
@@ -335,16 +368,16 @@ namespace dfx
               {
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
-                  if (w + 4 * t < mt)
+                  if (wt + 4 * t < mt)
                     loadOld(t, old[t]);
               }
             __syncthreads();
 
-            acc_t acc[MTW][NT];
+            acc_t acc[MTW][NTL];
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
 #pragma unroll
-              for (int nt = 0; nt < NT; ++nt)
+              for (int nt = 0; nt < NTL; ++nt)
                 acc[t][nt] = acc_t{0, 0, 0, 0};
 
             if constexpr (AREG)
@@ -353,26 +386,26 @@ namespace dfx
                 for (int kb = 0; kb < NKBMAX; ++kb)
                   if (kb < nkb) // block-uniform
                     {
-                      vec_t b[NT];
+                      vec_t b[NTL];
 #pragma unroll
-                      for (int nt = 0; nt < NT; ++nt)
+                      for (int nt = 0; nt < NTL; ++nt)
                         b[nt] = *reinterpret_cast<const vec_t *>(
                           sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
-                      if constexpr (DSC)
+                      if constexpr (DSC && !DFX_CELL_DSC_GATHER)
                         {
                           const vec_t sc =
                             *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
 #pragma unroll
-                          for (int nt = 0; nt < NT; ++nt)
+                          for (int nt = 0; nt < NTL; ++nt)
                             b[nt] = sc * b[nt];
                         }
 #pragma unroll
                       for (int s = 0; s < G; ++s)
 #pragma unroll
                         for (int t = 0; t < MTW; ++t)
-                          if (w + 4 * t < mt)
+                          if (wt + 4 * t < mt)
 #pragma unroll
-                            for (int nt = 0; nt < NT; ++nt)
+                            for (int nt = 0; nt < NTL; ++nt)
                               acc[t][nt] = M::op(ring[kb][t][s], b[nt][s], acc[t][nt]);
                     }
               }
@@ -380,17 +413,17 @@ namespace dfx
               // ring[0] holds K block kb, ring[d] block kb + d
               for (int kb = 0; kb < nkb; ++kb)
               {
-                vec_t b[NT];
+                vec_t b[NTL];
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
+                for (int nt = 0; nt < NTL; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
                     sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
-                if constexpr (DSC)
+                if constexpr (DSC && !DFX_CELL_DSC_GATHER)
                   {
                     const vec_t sc =
                       *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
+                    for (int nt = 0; nt < NTL; ++nt)
                       b[nt] = sc * b[nt];
                   }
                 vec_t a[MTW];
@@ -403,13 +436,15 @@ namespace dfx
                   for (int t = 0; t < MTW; ++t)
                     ring[d][t] = ring[d + 1][t];
                 loadA(kb + NRING, ring[NRING - 1]);
+                // one uniform test per M tile (MFMA_ALL: none; a tile past mt
+                // then multiplies the zero A of the masked load)
 #pragma unroll
-                for (int s = 0; s < G; ++s)
+                for (int t = 0; t < MTW; ++t)
+                  if (DFX_CELL_MFMA_ALL || wt + 4 * t < mt)
 #pragma unroll
-                  for (int t = 0; t < MTW; ++t)
-                    if (w + 4 * t < mt)
+                    for (int s = 0; s < G; ++s)
 #pragma unroll
-                      for (int nt = 0; nt < NT; ++nt)
+                      for (int nt = 0; nt < NTL; ++nt)
                         acc[t][nt] = M::op(a[t][s], b[nt][s], acc[t][nt]);
               }
             // the next column chunk starts again at K block 0
@@ -423,13 +458,13 @@ namespace dfx
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
               {
-                if (w + 4 * t >= mt)
+                if (wt + 4 * t >= mt)
                   continue;
-                T  oldLate[EARLY ? 1 : NT][4];
+                T  oldLate[EARLY ? 1 : NTL][4];
                 T(*o)[4] = EARLY ? old[EARLY ? t : 0] : oldLate;
                 if constexpr (!EARLY)
                   loadOld(t, oldLate);
-                const bool enr = (w + 4 * t) * 16 + 15 >= nc; // wave-uniform
+                const bool enr = (wt + 4 * t) * 16 + 15 >= nc; // wave-uniform
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv)
                   {
@@ -437,7 +472,7 @@ namespace dfx
                     const u32 yb  = ((sMap[row] - lo) * ldy + c0 + cl) * S;
                     const u32 eb  = ((cm.enrOff + (u32)(row - nc)) * B + c0 + cl) * S;
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
+                    for (int nt = 0; nt < NTL; ++nt)
                       {
                         const bool colOk = nt * 16 + cl < bw;
                         BF::store(o[nt][vv] + acc[t][nt][vv], rY,
@@ -450,7 +485,13 @@ namespace dfx
                   }
               }
             __syncthreads();
-          }
+        };
+        const u32 full = TAIL && NT > 1 ? B - B % BC : B;
+        for (u32 c0 = 0; c0 < full; c0 += BC)
+          chunk(std::integral_constant<int, NT>{}, c0, min(BC, (int)(full - c0)));
+        if constexpr (TAIL && NT > 1)
+          if (full < B)
+            chunk(std::integral_constant<int, 1>{}, full, (int)(B - full));
       }
 
       // One block (4 waves) per cell.  Wave w owns the M tiles w, w + 4, ...
@@ -766,14 +807,20 @@ namespace dfx
         // dynamic tile plus the static map / first-touch words (about 1 KiB)
         const size_t lds = (size_t)(buf ? 64 * MTW : L.kpadMax) * BC * S;
         DFX_REQUIRE(lds + 2048 <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
-        const int   which = !buf ? 0 : (dsc ? 2 : 1);
-        const void *fn    = which == 0 ? reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>)
-                          : which == 1
-                            ? reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false>)
-                            : reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true>);
+        // a last chunk of at most 16 columns: the kernel with the narrow tail
+        const bool  tail  = NT > 1 && buf && x.B % BC != 0 && x.B % BC <= 16;
+        const int   which = !buf ? 0 : (dsc ? 2 : 1) + (tail ? 2 : 0);
+        const void *fns[5] = {
+          reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, NT != 1>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, NT != 1>)};
+        const void *fn = fns[which];
         // raised once per kernel to the largest tile seen (several host
         // threads may launch concurrently: serving slots)
-        static std::atomic<size_t> attrSet[3] = {60 * 1024, 60 * 1024, 60 * 1024};
+        static std::atomic<size_t> attrSet[5] = {60 * 1024, 60 * 1024, 60 * 1024, 60 * 1024,
+                                                 60 * 1024};
         size_t cur = attrSet[which].load();
         if (lds > cur)
           {
@@ -784,16 +831,23 @@ namespace dfx
               }
           }
         DFX_REQUIRE(buf || !dsc, "folded diagonal needs the buffer-addressed cell kernel");
-        if (which == 2)
-          cellApplyBufK<T, NT, MTW, true><<<L.end - L.begin, 256, lds, s>>>(
+        auto bufLaunch = [&](auto dscTag, auto tailTag) {
+          constexpr bool D = decltype(dscTag)::value, TL = decltype(tailTag)::value;
+          cellApplyBufK<T, NT, MTW, D, TL><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
-            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical, dsc,
-            xe ? xe : x.p);
-        else if (which == 1)
-          cellApplyBufK<T, NT, MTW, false><<<L.end - L.begin, 256, lds, s>>>(
-            op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
-            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical, nullptr,
-            x.p);
+            x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical,
+            D ? dsc : nullptr, D && xe ? xe : x.p);
+        };
+        using F = std::false_type;
+        using Tt = std::integral_constant<bool, NT != 1>;
+        if (which == 1)
+          bufLaunch(F{}, F{});
+        else if (which == 2)
+          bufLaunch(std::true_type{}, F{});
+        else if (which == 3)
+          bufLaunch(F{}, Tt{});
+        else if (which == 4)
+          bufLaunch(std::true_type{}, Tt{});
         else
           cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
@@ -849,6 +903,8 @@ namespace dfx
       if (nt > cap)
         nt = cap;
       if (nt == 3)
+        nt = 4;
+      if (nt == 5) // 65..80 columns: a 64-column chunk and a narrow tail
         nt = 4;
       if (nt > 4 && nt < 8)
         nt = 8;

@@ -409,9 +409,15 @@ namespace dfx
                        : p.eigenVectorBatchSize;
       const bool mixed = p.precision == DFX_PREC_MIXED;
       Timer tf(ctx->stream);
-      for (u32 s0 = 0; s0 < n; s0 += bs)
+      // a remainder of at most bs / 4 columns is filtered with the last full
+      // batch: every column runs the same recurrence, so each column's result
+      // is the same, and a narrow batch would stream all cell matrices for a
+      // few columns
+      for (u32 s0 = 0, nb = 0; s0 < n; s0 += nb)
         {
-          const u32    nb  = std::min(bs, n - s0);
+          nb = std::min(bs, n - s0);
+          if (n - s0 > nb && (n - s0 - nb) * 4 <= bs)
+            nb = n - s0;
           View<double> Gin = sub(G, s0, nb), Xo = sub(XV, s0, nb);
           if (p.isResidualChebyshevFilter)
             {

@@ -124,6 +124,25 @@ def test_cell_apply_fp32(ctx, probs, name, B):
 
 
 @pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
+@pytest.mark.parametrize("B", [72, 136])
+def test_cell_apply_narrow_tail_chunk_bitwise(ctx, probs, dtype, B):
+    """A last column chunk of at most 16 columns (72 = 64 + 8, 136 = 128 + 8
+    or 4 x 32 + 8) runs with one 16-column tile; every column still gets the
+    bits of a separate apply to the leading and trailing columns."""
+    P = probs["enr"]
+    ops = Ops(ctx, P)
+    npd = np.float64 if dtype == dfx.F64 else np.float32
+    X = guess(P, B, seed=37).astype(npd)
+    outs = []
+    for cols in (slice(0, B), slice(0, B - 8), slice(B - 8, B)):
+        Xc = np.ascontiguousarray(X[:, cols])
+        Yg = dfx.MultiVector(ctx, P.nDofs, Xc.shape[1], dtype)
+        ops.H.apply(mv(ctx, Xc, dtype), Yg)
+        outs.append(Yg.to_numpy())
+    assert np.array_equal(outs[0], np.hstack(outs[1:]))
+
+
+@pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
 def test_cell_apply_kernel_paths_agree_bitwise(ctx, probs, dtype):
     """The buffer-addressed kernel (column count a multiple of the 16-byte
     vector) and the pointer kernel (any other count) give the same bits per
@@ -253,23 +272,29 @@ def test_filter_graph_replays_match_eager(ctx, probs, prec):
         assert np.array_equal(e0, e1) and np.array_equal(v0, v1)
 
 
-def test_chebyshev_filter_blocks_equal_fused(ctx, probs):
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+@pytest.mark.parametrize("residual", [False, True])
+def test_chebyshev_filter_blocks_equal_fused(ctx, probs, prec, residual):
     """Filtering block by block (eigenVectorBatchSize) gives the same bits as
-    filtering all columns at once: columns are independent in every kernel."""
+    filtering all columns at once: columns are independent in every kernel.
+    Batches of 8 (20 = 8 + 8 + 4: a remainder above 8 / 4 stays a batch of
+    its own), 16 (the remainder of 4 joins: one batch of 20) and 9 (the
+    remainder of 2 joins: 9 + 11)."""
     P = probs["c1"]
     ops = Ops(ctx, P)
     nv = 20
     X = guess(P, nv, seed=5)
     res = []
-    for bs in (0, 8):
+    for bs in (0, 8, 16, 9):
         G = mv(ctx, X)
         V = dfx.MultiVector(ctx, P.nDofs, nv)
-        s = dfx.ChebyshevFilteredEigenSolver(ctx, -0.6, 0.5, 40.0, 8, 1e-14, G,
-                                             eigenVectorBatchSize=bs)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, -0.6, 0.5, 40.0, 8, 1e-14, G, residual,
+                                             bs, prec)
         eig = np.linspace(-0.5, 0.2, nv)
         assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
         res.append((eig.copy(), V.to_numpy()))
-    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    for e, v in res[1:]:
+        assert np.array_equal(res[0][0], e) and np.array_equal(res[0][1], v)
 
 
 def test_cholgs_and_rr(ctx, probs):
