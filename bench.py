@@ -167,6 +167,19 @@ def poisson_run(ctx):
             "setup_s": round(setup, 3)}
 
 
+def batch_widths(n, bs):
+    """Column counts of the filter batches: batches of bs, a remainder of at
+    most bs / 4 joining the last full batch (the library's partition)."""
+    out, s0 = [], 0
+    while s0 < n:
+        nb = min(bs, n - s0)
+        if n - s0 > nb and (n - s0 - nb) * 4 <= bs:
+            nb = n - s0
+        out.append(nb)
+        s0 += nb
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +244,21 @@ def main():
     N, n, B, m = prob.nDofs, cfg["states"], cfg["block"], cfg["degree"]
     log(f"setup {setup_s:.1f}s: N={N} (enriched {prob.nEnriched}) sum n_e^2={info['sumNe2']} "
         f"max n_e={info['maxCellDofs']} colours={info['nColors']}")
+
+    # column-chunk widths of the cell kernel per size class, tuned on the
+    # device for every column count the pass uses: the filter batches (FP32
+    # H; FP64 H and M for the residual seed) and all states (FP64 M for
+    # CholGS, FP64 H for Rayleigh-Ritz)
+    t0 = time.time()
+    tuned = {}
+    widths = sorted(set(batch_widths(n, B)))
+    for name, op, dt, cols in (("H", H, dfx.F32, widths), ("H", H, dfx.F64, widths + [n]),
+                               ("M", M, dfx.F64, widths + [n])):
+        for bw in cols:
+            tuned[f"{name} {'f32' if dt == dfx.F32 else 'f64'} B={bw}"] = \
+                op.autotune(dt, bw, reps=3)
+    log(f"autotune {time.time() - t0:.1f}s: " +
+        "; ".join(f"{k}: {v}" for k, v in tuned.items()))
 
     def start_state():
         G = dfx.MultiVector(ctx, N, n)
@@ -391,7 +419,9 @@ def main():
                                 f"block {B}, degree {m}"),
                    "states": n, "block": B, "degree": m, "dofs": N,
                    "enriched_dofs": prob.nEnriched, "sum_ne2": info["sumNe2"],
-                   "max_cell_dofs": info["maxCellDofs"], "parallelism": "1 GPU"},
+                   "max_cell_dofs": info["maxCellDofs"], "parallelism": "1 GPU",
+                   "cell_chunk_tiles_autotuned": {k: {str(c): w for c, w in v.items()}
+                                                  for k, v in tuned.items()}},
         "fp64_pass_ms": t_fp64 * 1e3,
         "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
         "pass_split_ms": tim,

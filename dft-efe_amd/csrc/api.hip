@@ -131,6 +131,97 @@ namespace dfx
     k::c2p<T>(s, op->cons, Y);
   }
 
+  // Column-chunk autotuner: for each M-tile class of the operator (launch
+  // order of first appearance), time `reps` passes over the class's
+  // launches (all colours, one stream) per candidate width on zeroed scratch
+  // multivectors and keep the fastest.  Candidates: 1, 2, 4, 8 tiles of 16
+  // columns up to twice the columns there are, 5 for FP32 with 65..80
+  // columns, and only tiles that fit the LDS of a CU.  Per-column results do
+  // not depend on the width (same K order per column), only the time does.
+  template <class T>
+  std::vector<std::pair<u32, int>> autotune(dfx_op *op, u32 B, u32 reps)
+  {
+    hipStream_t s    = op->ctx->stream;
+    const auto &L    = sizeof(T) == 8 ? op->launches64 : op->launches32;
+    const T    *frag = fragOf<T>(op);
+    const u32   N    = op->nDofs;
+    DevBuf xb((size_t)N * B * sizeof(T)), yb((size_t)N * B * sizeof(T));
+    DFX_HIP(hipMemsetAsync(xb.p, 0, xb.bytes, s));
+    DFX_HIP(hipMemsetAsync(yb.p, 0, yb.bytes, s));
+    T *yE = nullptr;
+    if (op->nEnrSlots)
+      {
+        op->yE.ensure((size_t)op->nEnrSlots * B * sizeof(double));
+        yE = op->yE.as<T>();
+      }
+    View<const T> X{xb.as<T>(), N, B, B};
+    View<T>       Y{yb.as<T>(), N, B, B};
+    const bool    buf = k::cellBufferPath<T>(op, X, View<const T>{Y.p, N, B, B});
+    std::vector<u32> classes;
+    for (const Launch &l : L)
+      if (std::find(classes.begin(), classes.end(), l.mtw) == classes.end())
+        classes.push_back(l.mtw);
+    op->tuned.erase(std::remove_if(op->tuned.begin(), op->tuned.end(),
+                                   [&](const dfx_op::Tuned &t) {
+                                     return t.s == sizeof(T) && t.B == B;
+                                   }),
+                    op->tuned.end());
+    hipEvent_t a, b;
+    DFX_HIP(hipEventCreate(&a));
+    DFX_HIP(hipEventCreate(&b));
+    std::vector<std::pair<u32, int>> out;
+    try
+      {
+        for (u32 c : classes)
+          {
+            u32 kpad = 0;
+            for (const Launch &l : L)
+              if (l.mtw == c)
+                kpad = std::max(kpad, l.kpadMax);
+            const u32 rows = buf ? 64 * c : kpad; // K rows of the LDS tile
+            int       best = 0;
+            float     bestMs = 0.f;
+            for (int nt : {1, 2, 4, 5, 8})
+              {
+                if (nt == 5 && !(sizeof(T) == 4 && B > 64 && B <= 80))
+                  continue;
+                if (nt > 1 && 16u * nt > 2 * B)
+                  continue;
+                if ((size_t)rows * 16 * nt * sizeof(T) + 2048 > 160 * 1024)
+                  continue;
+                auto pass = [&] {
+                  for (const Launch &l : L)
+                    if (l.mtw == c)
+                      k::cellApply<T>(s, op, l, frag, X, Y, yE, nullptr, nullptr, nt);
+                };
+                pass(); // first launch: kernel attributes, instruction cache
+                DFX_HIP(hipEventRecord(a, s));
+                for (u32 r = 0; r < reps; ++r)
+                  pass();
+                DFX_HIP(hipEventRecord(b, s));
+                DFX_HIP(hipEventSynchronize(b));
+                float ms = 0.f;
+                DFX_HIP(hipEventElapsedTime(&ms, a, b));
+                if (!best || ms < bestMs)
+                  best = nt, bestMs = ms;
+              }
+            if (!best)
+              best = k::builtinNT<T>(B, kpad);
+            op->tuned.push_back(dfx_op::Tuned{(u32)sizeof(T), B, c, best});
+            out.emplace_back(c, best);
+          }
+      }
+    catch (...)
+      {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+      }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return out;
+  }
+
   // Y = A (Dinv X) with Dinv a diagonal (+ enriched block) operator and A a
   // cell-wise one sharing homogeneous Dirichlet constraints (no parents, zero
   // values, classical rows only): the diagonal is applied inside the cell
@@ -831,6 +922,36 @@ extern "C"
           op->profLaunches = 0;
           op->profMs = op->profFlops = op->profBytes = 0;
         }
+      return 0;
+    });
+  }
+
+  int dfx_op_autotune(dfx_op *op, int dtype, uint32_t B, uint32_t reps, uint32_t maxOut,
+                      uint32_t *classes, uint32_t *widths, uint32_t *nOut)
+  {
+    if (!op || op->kind != OP_CELLWISE || (dtype != DFX_F64 && dtype != DFX_F32) ||
+        (maxOut && (!classes || !widths)))
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      const u32 s = dtype == DFX_F64 ? 8 : 4;
+      if (B == 0)
+        {
+          op->tuned.erase(std::remove_if(op->tuned.begin(), op->tuned.end(),
+                                         [&](const dfx_op::Tuned &t) { return t.s == s; }),
+                          op->tuned.end());
+          if (nOut)
+            *nOut = 0;
+          return 0;
+        }
+      const auto r = dtype == DFX_F64 ? autotune<double>(op, B, std::max(reps, 1u))
+                                      : autotune<float>(op, B, std::max(reps, 1u));
+      for (size_t i = 0; i < r.size() && i < maxOut; ++i)
+        {
+          classes[i] = r[i].first;
+          widths[i]  = (uint32_t)r[i].second;
+        }
+      if (nOut)
+        *nOut = (uint32_t)r.size();
       return 0;
     });
   }

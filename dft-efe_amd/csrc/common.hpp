@@ -268,6 +268,21 @@ struct dfx_op
   // ----- diagonal
   dfx::DevBuf diag, diag32, enrBlock, enrFrag64, enrFrag32;
   dfx::u32 nEnriched = 0;
+  // ----- column-chunk widths picked by dfx_op_autotune: for scalar size s
+  // and exactly B columns, launches of M-tile class mtw use 16 nt columns
+  struct Tuned
+  {
+    dfx::u32 s, B, mtw;
+    int      nt;
+  };
+  std::vector<Tuned> tuned;
+  int tunedNT(dfx::u32 s, dfx::u32 B, dfx::u32 mtw) const
+  {
+    for (const Tuned &t : tuned)
+      if (t.s == s && t.B == B && t.mtw == mtw)
+        return t.nt;
+    return 0;
+  }
   // ----- profiling of the cell-apply launches
   bool profiling = false;
   int  profMask  = 3; // 1: FP64 launches, 2: FP32 launches

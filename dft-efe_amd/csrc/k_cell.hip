@@ -43,6 +43,9 @@
 #ifndef DFX_CELL_MINW_MID
 #define DFX_CELL_MINW_MID 3
 #endif
+#ifndef DFX_CELL_NT5
+#define DFX_CELL_NT5 1
+#endif
 #ifndef DFX_CELL_ROT
 #define DFX_CELL_ROT 0
 #endif
@@ -808,14 +811,16 @@ namespace dfx
         const size_t lds = (size_t)(buf ? 64 * MTW : L.kpadMax) * BC * S;
         DFX_REQUIRE(lds + 2048 <= 160 * 1024, "cell tile exceeds the 160 KiB LDS of a CU");
         // a last chunk of at most 16 columns: the kernel with the narrow tail
-        const bool  tail  = NT > 1 && buf && x.B % BC != 0 && x.B % BC <= 16;
+        // (NT = 5 serves 65..80 columns in one chunk, never with a tail)
+        constexpr bool TAILOK = NT != 1 && NT != 5;
+        const bool  tail  = TAILOK && buf && x.B % BC != 0 && x.B % BC <= 16;
         const int   which = !buf ? 0 : (dsc ? 2 : 1) + (tail ? 2 : 0);
         const void *fns[5] = {
           reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>),
           reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, false>),
           reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, false>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, NT != 1>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, NT != 1>)};
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, TAILOK>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, TAILOK>)};
         const void *fn = fns[which];
         // raised once per kernel to the largest tile seen (several host
         // threads may launch concurrently: serving slots)
@@ -839,7 +844,7 @@ namespace dfx
             D ? dsc : nullptr, D && xe ? xe : x.p);
         };
         using F = std::false_type;
-        using Tt = std::integral_constant<bool, NT != 1>;
+        using Tt = std::integral_constant<bool, TAILOK>;
         if (which == 1)
           bufLaunch(F{}, F{});
         else if (which == 2)
@@ -900,11 +905,15 @@ namespace dfx
         cap = kpadMax <= 128 ? 8 : 4;
       if (const int o = ntOverride(sizeof(T) == 8))
         cap = o;
+      // FP32, 65..80 columns: one 80-column chunk (the cell matrix read
+      // once); the FP64 tile of 80 columns would leave one workgroup per CU
+      if (DFX_CELL_NT5 && sizeof(T) == 4 && nt == 5)
+        return 5;
       if (nt > cap)
         nt = cap;
       if (nt == 3)
         nt = 4;
-      if (nt == 5) // 65..80 columns: a 64-column chunk and a narrow tail
+      if (nt == 5) // 65..80 columns (FP64): a 64-column chunk and a narrow tail
         nt = 4;
       if (nt > 4 && nt < 8)
         nt = 8;
@@ -912,18 +921,32 @@ namespace dfx
     }
 
     template <class T>
+    int builtinNT(u32 B, u32 kpadMax)
+    {
+      return chooseNT<T>(B, kpadMax);
+    }
+
+    template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                   View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe)
+                   View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe, int nt)
     {
       if (L.end <= L.begin || !x.B)
         return;
-      switch (chooseNT<T>(x.B, L.kpadMax))
+      // explicit width, else the autotuner's for (scalar size, B, class) unless
+      // the environment caps the width, else the built-in choice
+      if (!nt && !ntOverride(sizeof(T) == 8))
+        nt = op->tunedNT(sizeof(T), x.B, L.mtw);
+      if (!nt)
+        nt = chooseNT<T>(x.B, L.kpadMax);
+      switch (nt)
         {
           case 1: return launchCellMtw<T, 1>(s, op, L, frag, x, y, yE, dsc, xe);
           case 2: return launchCellMtw<T, 2>(s, op, L, frag, x, y, yE, dsc, xe);
           case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE, dsc, xe);
           case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE, dsc, xe);
+          case 5: return launchCellMtw<T, 5>(s, op, L, frag, x, y, yE, dsc, xe);
         }
+      throw Error(DFX_E_ARG, "column-chunk width must be 1, 2, 4, 5 or 8 tiles of 16");
     }
 
     namespace
@@ -979,10 +1002,12 @@ namespace dfx
 
     template void cellApply<double>(hipStream_t, const dfx_op *, const Launch &,
                                     const double *, View<const double>, View<double>,
-                                    double *, const double *, const double *);
+                                    double *, const double *, const double *, int);
     template void cellApply<float>(hipStream_t, const dfx_op *, const Launch &,
                                    const float *, View<const float>, View<float>, float *,
-                                   const double *, const float *);
+                                   const double *, const float *, int);
+    template int builtinNT<double>(u32, u32);
+    template int builtinNT<float>(u32, u32);
     template void enrichedReduce<double>(hipStream_t, const dfx_op *, const double *,
                                          View<double>);
     template void enrichedReduce<float>(hipStream_t, const dfx_op *, const float *,

@@ -75,10 +75,16 @@ namespace dfx
     // one colour / size-class launch of the fused gather -> MFMA -> scatter
     // dsc (optional): the launch gathers diag o X for the classical rows
     // (M^-1 folded into the gather) and the enriched rows from xe
+    // nt (optional): the column-chunk width in 16-column tiles (1, 2, 4, 5 or
+    // 8) instead of the tuned or built-in choice
     template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
                    View<const T> x, View<T> y, T *yE, const double *dsc = nullptr,
-                   const T *xe = nullptr);
+                   const T *xe = nullptr, int nt = 0);
+    // the built-in column-chunk width for B columns and a class of padded K
+    // up to kpadMax
+    template <class T>
+    int builtinNT(u32 B, u32 kpadMax);
     // out[s] = d[map[s]] (map[s] < nC) else 1, for every slot of a cell->dof map
     void gatherDiag(hipStream_t s, const double *d, const u32 *map, u64 n, u32 nC,
                     double *out);

@@ -109,6 +109,7 @@ _sig("dfx_lanczos_extreme", _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp)
 _sig("dfx_ks_eigensolve", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_cg_solve", _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _u32, _vp, _vp)
 _sig("dfx_op_set_profiling", _vp, C.c_int)
+_sig("dfx_op_autotune", _vp, C.c_int, _u32, _u32, _u32, _vp, _vp, _vp)
 _sig("dfx_op_get_profile", _vp, _vp, C.c_int)
 _sig("dfx_problem_build", _vp, _vp, _pp)
 _sig("dfx_problem_sizes", _vp, _vp)
@@ -332,6 +333,17 @@ class _Operator:
         restricts to launches of that value type."""
         mode = 0 if not enable else (3 if precision is None else (1 if precision == F64 else 2))
         _lib.dfx_op_set_profiling(self.h, mode)
+
+    def autotune(self, dtype, B, reps=3):
+        """Pick the fastest column-chunk width (tiles of 16 columns) of every
+        size class for applies of dtype with exactly B columns (B = 0 forgets
+        the tuned widths of dtype).  Returns {class (M tiles per wave): width}."""
+        cls = np.zeros(16, dtype=np.uint32)
+        wid = np.zeros(16, dtype=np.uint32)
+        n = C.c_uint32(0)
+        self.ctx.check(_lib.dfx_op_autotune(self.h, int(dtype), int(B), int(reps), 16,
+                                            _ptr(cls), _ptr(wid), C.byref(n)))
+        return {int(c): int(w) for c, w in zip(cls[:n.value], wid[:n.value])}
 
     def profile(self, reset=True):
         """launches, summed kernel ms, algorithmic flops and bytes of the

@@ -214,6 +214,20 @@ typedef struct dfx_op_profile
 /* enable: 0 off; 1 FP64 launches; 2 FP32 launches; 3 both */
 int dfx_op_set_profiling(dfx_op *op, int enable);
 int dfx_op_get_profile(dfx_op *op, dfx_op_profile *out, int reset);
+/* Column-chunk autotuner.  No counterpart in the reference, whose cell-block
+ * sizes are fixed defaults (KohnShamOperatorContextFE.t.cpp:1370-1371).
+ * Times the candidate column-chunk widths of the fused cell-apply kernel
+ * (16 NT columns per chunk, NT in {1, 2, 4, 5, 8}) for every size class of a
+ * cell-wise operator, for dtype and exactly B columns, on scratch device
+ * multivectors (reps passes per candidate), and keeps the fastest width per
+ * class for later applies of that dtype with B columns.  Results of an apply
+ * do not depend on the width (columns are independent, same K order).
+ * classes / widths (room for maxOut entries; may be NULL when maxOut is 0):
+ * the class (M tiles per wave) and the chosen NT, in launch order; *nOut
+ * (nullable) = number of classes.  B = 0 forgets the widths tuned for dtype.
+ * DFX_CELL_NT_F32 / DFX_CELL_NT_F64 in the environment take precedence. */
+int dfx_op_autotune(dfx_op *op, int dtype, uint32_t B, uint32_t reps, uint32_t maxOut,
+                    uint32_t *classes, uint32_t *widths, uint32_t *nOut);
 /* Stand-alone cell gather: Xcell[slot, :] = X[map[slot], :], Xcell is
  * (sum_e n_e) x B  (FECellWiseDataOperations::copyFieldToCellWiseData,
  * src/basis/FECellWiseDataOperations.t.cpp:31-84). */

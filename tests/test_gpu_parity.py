@@ -143,6 +143,29 @@ def test_cell_apply_narrow_tail_chunk_bitwise(ctx, probs, dtype, B):
 
 
 @pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
+def test_autotuned_widths_keep_the_bits(ctx, probs, dtype):
+    """The autotuner picks a column-chunk width per size class among its
+    candidates; applies after tuning give the bits of applies with the
+    built-in widths (columns are independent, same K order)."""
+    P = probs["enr"]
+    ops = Ops(ctx, P)
+    npd = np.float64 if dtype == dfx.F64 else np.float32
+    for B in (24, 72):
+        X = guess(P, B, seed=43).astype(npd)
+
+        def run():
+            Y = dfx.MultiVector(ctx, P.nDofs, B, dtype)
+            ops.H.apply(mv(ctx, X, dtype), Y)
+            return Y.to_numpy()
+
+        before = run()
+        chosen = ops.H.autotune(dtype, B, reps=2)
+        assert chosen and all(w in (1, 2, 4, 5, 8) for w in chosen.values())
+        assert np.array_equal(run(), before)
+    assert ops.H.autotune(dtype, 0) == {}
+
+
+@pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
 def test_cell_apply_kernel_paths_agree_bitwise(ctx, probs, dtype):
     """The buffer-addressed kernel (column count a multiple of the 16-byte
     vector) and the pointer kernel (any other count) give the same bits per
