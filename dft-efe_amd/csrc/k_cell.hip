@@ -46,15 +46,6 @@
 #ifndef DFX_CELL_NT5
 #define DFX_CELL_NT5 1
 #endif
-#ifndef DFX_CELL_ROT
-#define DFX_CELL_ROT 0
-#endif
-#ifndef DFX_CELL_DSC_GATHER
-#define DFX_CELL_DSC_GATHER 0
-#endif
-#ifndef DFX_CELL_MFMA_ALL
-#define DFX_CELL_MFMA_ALL 0
-#endif
 #ifndef DFX_CELL_ORDER
 #define DFX_CELL_ORDER 0
 #endif
@@ -224,10 +215,6 @@ namespace dfx
         const int      mt   = (n + 15) >> 4;
         const int      nkb  = (n + KB - 1) / KB;
         const int      w    = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
-        // M tiles wt, wt + 4, ... of the cell: when mt is not a multiple of
-        // 4 some waves own one tile more; the rotation by a hash of the cell
-        // position spreads those waves over the CU's four SIMDs
-        const int      wt   = DFX_CELL_ROT ? (int)((w + ((pos * 2654435761u) >> 30)) & 3u) : w;
         const int      l    = threadIdx.x & 63;
         const rsrc_t   rA   = rsrc(frag + (S == 8 ? cm.fragOff64 : cm.fragOff32),
                                    (u32)(mt * nkb * 64 * G) * S);
@@ -244,7 +231,7 @@ namespace dfx
 #pragma unroll
           for (int t = 0; t < MTW; ++t)
             {
-              const int mtile = wt + 4 * t;
+              const int mtile = w + 4 * t;
               const u32 off   = (mtile < mt && kb < nkb)
                                   ? ((u32)(mtile * nkb + kb) * 64 + l) * (G * S)
                                   : OFF_NONE;
@@ -259,7 +246,7 @@ namespace dfx
 
         if (threadIdx.x < n)
           sMap[threadIdx.x] = map[cm.mapOff + threadIdx.x];
-        // M^-1 folded in: the MFMA's B operand (X_e) is scaled row by row,
+        // M^-1 folded in: the gathered rows of X_e are scaled row by row,
         // diag on the classical rows and 1 elsewhere (exact)
         if constexpr (DSC) // dsc holds d[map[slot]] per map slot
           sDsc[threadIdx.x] = threadIdx.x < (unsigned)nc ? (T)dsc[cm.mapOff + threadIdx.x]
@@ -269,7 +256,7 @@ namespace dfx
         __syncthreads();
 
         const int cl = l & 15;
-        auto rowOf = [&](int t, int v) { return (wt + 4 * t) * 16 + M::row(l, v); };
+        auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(l, v); };
 
         // one column chunk [c0, c0 + bw) with NTL live 16-column tiles (the
         // LDS row stride stays BC).  TAIL kernels (B % BC in [1, 16]) run the
@@ -313,10 +300,11 @@ namespace dfx
                         v[u][r] = kk >= nc ? ve : vc;
                       }
                   }
-                if constexpr (DSC && DFX_CELL_DSC_GATHER)
+                if constexpr (DSC)
                   {
-                    // rows q G .. q G + G - 1 of the unit scaled by their
-                    // diag (1 past nc); kk < KMAX <= 256 stays in sDsc
+                    // M^-1 folded in: rows q G .. q G + G - 1 of the unit
+                    // scaled by their diag (1 past nc, exact) before the MFMA
+                    // reads them; kk < KMAX <= 256 stays in sDsc
                     const vec_t sc = *reinterpret_cast<const vec_t *>(sDsc + (live ? q * G : 0));
 #pragma unroll
                     for (int r = 0; r < G; ++r)
@@ -371,7 +359,7 @@ namespace dfx
               {
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
-                  if (wt + 4 * t < mt)
+                  if (w + 4 * t < mt)
                     loadOld(t, old[t]);
               }
             __syncthreads();
@@ -394,19 +382,11 @@ namespace dfx
                       for (int nt = 0; nt < NTL; ++nt)
                         b[nt] = *reinterpret_cast<const vec_t *>(
                           sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
-                      if constexpr (DSC && !DFX_CELL_DSC_GATHER)
-                        {
-                          const vec_t sc =
-                            *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
-#pragma unroll
-                          for (int nt = 0; nt < NTL; ++nt)
-                            b[nt] = sc * b[nt];
-                        }
 #pragma unroll
                       for (int s = 0; s < G; ++s)
 #pragma unroll
                         for (int t = 0; t < MTW; ++t)
-                          if (wt + 4 * t < mt)
+                          if (w + 4 * t < mt)
 #pragma unroll
                             for (int nt = 0; nt < NTL; ++nt)
                               acc[t][nt] = M::op(ring[kb][t][s], b[nt][s], acc[t][nt]);
@@ -421,14 +401,6 @@ namespace dfx
                 for (int nt = 0; nt < NTL; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
                     sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
-                if constexpr (DSC && !DFX_CELL_DSC_GATHER)
-                  {
-                    const vec_t sc =
-                      *reinterpret_cast<const vec_t *>(sDsc + kb * KB + G * (l >> 4));
-#pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt)
-                      b[nt] = sc * b[nt];
-                  }
                 vec_t a[MTW];
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
@@ -439,11 +411,10 @@ namespace dfx
                   for (int t = 0; t < MTW; ++t)
                     ring[d][t] = ring[d + 1][t];
                 loadA(kb + NRING, ring[NRING - 1]);
-                // one uniform test per M tile (MFMA_ALL: none; a tile past mt
-                // then multiplies the zero A of the masked load)
+                // one uniform test per M tile
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
-                  if (DFX_CELL_MFMA_ALL || wt + 4 * t < mt)
+                  if (w + 4 * t < mt)
 #pragma unroll
                     for (int s = 0; s < G; ++s)
 #pragma unroll
@@ -461,13 +432,13 @@ namespace dfx
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
               {
-                if (wt + 4 * t >= mt)
+                if (w + 4 * t >= mt)
                   continue;
                 T  oldLate[EARLY ? 1 : NTL][4];
                 T(*o)[4] = EARLY ? old[EARLY ? t : 0] : oldLate;
                 if constexpr (!EARLY)
                   loadOld(t, oldLate);
-                const bool enr = (wt + 4 * t) * 16 + 15 >= nc; // wave-uniform
+                const bool enr = (w + 4 * t) * 16 + 15 >= nc; // wave-uniform
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv)
                   {
