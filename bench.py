@@ -187,6 +187,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="built-in cell-kernel chunk widths (profiling runs: no tuning "
+                         "launches in the trace)")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--convergence-passes", type=int, default=25)
     ap.add_argument("--serve-requests", type=int, default=64,
@@ -254,11 +257,12 @@ def main():
     widths = sorted(set(batch_widths(n, B)))
     for name, op, dt, cols in (("H", H, dfx.F32, widths), ("H", H, dfx.F64, widths + [n]),
                                ("M", M, dfx.F64, widths + [n])):
-        for bw in cols:
+        for bw in cols if not args.no_autotune else ():
             tuned[f"{name} {'f32' if dt == dfx.F32 else 'f64'} B={bw}"] = \
                 op.autotune(dt, bw, reps=3)
-    log(f"autotune {time.time() - t0:.1f}s: " +
-        "; ".join(f"{k}: {v}" for k, v in tuned.items()))
+    if tuned:
+        log(f"autotune {time.time() - t0:.1f}s: " +
+            "; ".join(f"{k}: {v}" for k, v in tuned.items()))
 
     def start_state():
         G = dfx.MultiVector(ctx, N, n)
