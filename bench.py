@@ -90,7 +90,7 @@ def cpu_baseline(prob, n_states, block, degree, budget_cells=4096, threads=None)
                        f"eigensolve are not counted")}
 
 
-def serving_run(n_requests, rate, slots, seed=5):
+def serving_run(n_requests, rate, slots, seed=5, use_graph=True):
     """The C5 stream (or its first n_requests): Poisson arrivals, continuous
     batching on `slots` streams, FP32 filter / FP64 RR.  Latency = result time
     minus arrival time."""
@@ -101,7 +101,8 @@ def serving_run(n_requests, rate, slots, seed=5):
     # first-use costs (code objects, library handles) outside the measurement
     dfx.serve([dict(reqs[0], arrival=0.0)], max_slots=1)
     t0 = time.perf_counter()
-    energies, res = dfx.serve(reqs, max_slots=slots, precision=dfx.PREC_MIXED)
+    energies, res = dfx.serve(reqs, max_slots=slots, precision=dfx.PREC_MIXED,
+                              use_graph=use_graph)
     wall = time.perf_counter() - t0
     lat = np.array([x["doneSeconds"] - r["arrival"] for x, r in zip(res, reqs)])
     ok = sum(1 for x in res if x["status"] == 0)
@@ -111,6 +112,8 @@ def serving_run(n_requests, rate, slots, seed=5):
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "converged": ok, "mean_passes": float(np.mean([x["passes"] for x in res])),
+            "degree_loop": "captured once per request, replayed every pass" if use_graph
+                           else "eager launches",
             "stream": f"SplitMix64 seed {seed}: 1-8 atoms (Z in 1,6), 16-28 bohr, "
                       "6^3-14^3 cells at p=3, 5-40 states, 1e-6 residual"}
 

@@ -288,6 +288,7 @@ namespace dfx
     (void)hipSetDevice(ctx->device);
     if (ctx->stream)
       (void)hipStreamSynchronize(ctx->stream);
+    CaptureLock lk(captureMutex());
     if (ctx->blas)
       rocblas_destroy_handle(ctx->blas);
     for (auto &g : ctx->graphs)
@@ -346,9 +347,12 @@ extern "C"
       DFX_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
       DFX_HIP(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming));
       DFX_HIP(hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming));
-      if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-        throw Error(DFX_E_HIP, "rocblas_create_handle failed");
-      rocblas_set_stream(ctx->blas, ctx->stream);
+      {
+        CaptureLock lk(captureMutex());
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+          throw Error(DFX_E_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(ctx->blas, ctx->stream);
+      }
       ctx->scratchSmall.alloc(1 << 20);
       return 0;
     });

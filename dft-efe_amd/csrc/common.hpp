@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,19 @@ namespace dfx
       throw Error(DFX_E_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e));
   }
 
+  // ------------------------------------------------------------ captures
+  // Graph captures (one host thread per serving slot may capture while
+  // others work) are serialized with the calls that can reach the legacy
+  // stream or synchronize the device implicitly: allocation and release,
+  // rocBLAS / rocSOLVER calls, handle and stream teardown.  Recursive: a
+  // capture that allocates (it should not) does not deadlock.
+  inline std::recursive_mutex &captureMutex()
+  {
+    static std::recursive_mutex m;
+    return m;
+  }
+  using CaptureLock = std::lock_guard<std::recursive_mutex>;
+
   // ------------------------------------------------------------ buffers
   // Owning device allocation (hipMalloc / hipFree).
   struct DevBuf
@@ -84,7 +98,10 @@ namespace dfx
     {
       release();
       if (n)
-        DFX_HIP(hipMalloc(&p, n));
+        {
+          CaptureLock lk(captureMutex());
+          DFX_HIP(hipMalloc(&p, n));
+        }
       bytes = n;
     }
     void ensure(size_t n)
@@ -95,7 +112,10 @@ namespace dfx
     void release()
     {
       if (p)
-        (void)hipFree(p);
+        {
+          CaptureLock lk(captureMutex());
+          (void)hipFree(p);
+        }
       p     = nullptr;
       bytes = 0;
     }

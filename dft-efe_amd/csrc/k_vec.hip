@@ -203,9 +203,10 @@ namespace dfx
       template <class T>
       __global__ void resStepK(T *res, u32 ldr, const T *s2, u32 lds, const T *rn,
                                u32 ldn, const T *r0, u32 ld0, const double *mu,
-                               T alpha1, T alpha2, T c, u32 N, u32 B)
+                               const double *coef, u32 N, u32 B)
       {
-        const u64 n = (u64)N * B;
+        const u64 n      = (u64)N * B;
+        const T   alpha1 = (T)coef[0], alpha2 = (T)coef[1], c = (T)coef[2];
         for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
              t += (u64)gridDim.x * blockDim.x)
           {
@@ -422,15 +423,13 @@ namespace dfx
 
     template <class T>
     void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
-                 View<const T> r0, const double *mu, double alpha1, double alpha2,
-                 double c)
+                 View<const T> r0, const double *mu, const double *coef)
     {
       const u64 n = (u64)res.N * res.B;
       if (!n)
         return;
       resStepK<T><<<gridFor(n), TPB, 0, s>>>(res.p, res.ld, s2.p, s2.ld, rn.p, rn.ld,
-                                             r0.p, r0.ld, mu, (T)alpha1, (T)alpha2,
-                                             (T)c, res.N, res.B);
+                                             r0.p, r0.ld, mu, coef, res.N, res.B);
       checkLaunch("resStep");
     }
 
@@ -518,7 +517,7 @@ namespace dfx
                             View<const T>, u32, View<const T>, double, double,     \
                             double);                                               \
   template void resStep<T>(hipStream_t, View<T>, View<const T>, View<const T>,     \
-                           View<const T>, const double *, double, double, double); \
+                           View<const T>, const double *, const double *);         \
   template void p2c<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void c2p<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void setRows<T>(hipStream_t, const dfx_constraints *, View<T>, double); \

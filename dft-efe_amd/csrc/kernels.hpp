@@ -49,10 +49,11 @@ namespace dfx
     //   s1  = alpha1 s2 + (-c alpha1) rn
     //   res = s1 + alpha2 res
     //   res = res + alpha1 mu[j] r0
+    // coef: device array {alpha1, alpha2, c} of the degree (read by the
+    // kernel, so a captured launch stays valid when the filter bounds change)
     template <class T>
     void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
-                 View<const T> r0, const double *mu, double alpha1, double alpha2,
-                 double c);
+                 View<const T> r0, const double *mu, const double *coef);
 
     // -------------------------------------------------------- constraints
     template <class T>

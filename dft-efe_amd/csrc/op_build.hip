@@ -195,29 +195,34 @@ namespace dfx
     op->fragLen64 = f64;
     op->fragLen32 = f32;
 
+    // one launch per (colour, size class); an operator of few cells (the
+    // launches could not fill the GPU anyway, and every launch costs a
+    // dispatch) runs each colour as one launch with the largest class's tile
+    const char *mergeEnv   = getenv("DFX_MERGE_CLASSES_BELOW");
+    const u32   mergeBelow = mergeEnv ? (u32)atol(mergeEnv) : 4096u;
+    const bool mergeClasses = nCells < mergeBelow;
     op->launches64.clear();
     op->launches32.clear();
     for (u32 q = 0; q < nCells;)
       {
         const u32 e0 = op->hOrder[q];
         u32       r  = q;
-        u32 kp64 = 0, kp32 = 0;
+        u32 kp64 = 0, kp32 = 0, mtw = 0;
         u64 s2 = 0, s1 = 0, sc = 0;
         while (r < nCells && color[op->hOrder[r]] == color[e0] &&
-               mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0]))
+               (mergeClasses || mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0])))
           {
             const u32 n = cellNdofs[op->hOrder[r]];
             kp64        = std::max(kp64, 8 * ((n + 7) / 8));
             kp32        = std::max(kp32, 16 * ((n + 15) / 16));
+            mtw         = std::max(mtw, mtwOf(n));
             s2 += (u64)n * n;
             s1 += n;
             sc += nc[op->hOrder[r]];
             ++r;
           }
-        op->launches64.push_back({q, r, mtwOf(cellNdofs[e0]), kp64, s2, s1, sc, 0,
-                                  (u32)color[e0]});
-        op->launches32.push_back({q, r, mtwOf(cellNdofs[e0]), kp32, s2, s1, sc, 0,
-                                  (u32)color[e0]});
+        op->launches64.push_back({q, r, mtw, kp64, s2, s1, sc, 0, (u32)color[e0]});
+        op->launches32.push_back({q, r, mtw, kp32, s2, s1, sc, 0, (u32)color[e0]});
         q = r;
       }
 

@@ -12,6 +12,8 @@
 // of its setup (FEBasisManager / computeFEMatrices, here dfx_problem_build)
 // and KohnShamEigenSolver::solve (src/ksdft/KohnShamEigenSolver.t.cpp:214-536).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -96,8 +98,8 @@ namespace dfx
     };
 
     // one request on its own context: build, operators, seeded guess, KS solve
-    int runRequest(int device, int precision, const dfx_request &r, double *energies,
-                   dfx_request_result &out)
+    int runRequest(int device, int precision, bool useGraph, const dfx_request &r,
+                   double *energies, dfx_request_result &out)
     {
       dfx_ctx *ctx = nullptr;
       int      rc  = dfx_ctx_create(device, &ctx);
@@ -143,12 +145,21 @@ namespace dfx
         kp.smearingTemperature         = 500.0;
         kp.fermiEnergyTolerance        = 1e-10;
         kp.fracOccupancyTolerance      = 1e-3;
+        // a small request's degree loop is launch-bound: capture it once
+        // and replay it on every pass
+        kp.useGraph                    = useGraph ? 1 : 0;
         dfx_ks_result res{};
         s = dfx_ks_eigensolve(ctx, &kp, H, M, Mi, Ml, G, L, energies, W, nullptr, &res);
         out.passes = res.passes;
         return s;
       };
       rc = run();
+      static const bool verbose = [] {
+        const char *e = getenv("DFX_SERVE_VERBOSE");
+        return e && atoi(e) != 0;
+      }();
+      if (rc < 0 && verbose)
+        fprintf(stderr, "dfx_serve: request failed (%d): %s\n", rc, ctx->err.c_str());
       for (dfx_mv *v : {G, L, W})
         if (v)
           dfx_mv_destroy(v);
@@ -240,7 +251,7 @@ extern "C"
           out.admitSeconds        = since(clk::now());
           try
             {
-              out.status = runRequest(p->device, p->precision, reqs[idx],
+              out.status = runRequest(p->device, p->precision, p->useGraph != 0, reqs[idx],
                                       energies + energyOff[idx], out);
             }
           catch (...)

@@ -309,6 +309,22 @@ namespace
       }
   }
 
+  // enriched x enriched sub-block of each listed cell's (column-major) mass
+  // matrix, packed: out[off[i] + a + m_i * b] = M_e[ncl + a, ncl + b],
+  // m_i = n_e - ncl
+  __global__ void extractEnrBlockK(const double *M, const u64 *matOff, const u32 *ne,
+                                   const u32 *cellList, const u64 *outOff, u32 ncl,
+                                   double *out)
+  {
+    const u32 i = blockIdx.x, e = cellList[i];
+    const u32 n = ne[e], m = n - ncl;
+    for (u32 t = threadIdx.x; t < m * m; t += blockDim.x)
+      {
+        const u32 a = t % m, b = t / m;
+        out[outOff[i] + t] = M[matOff[e] + (ncl + a) + (u64)n * (ncl + b)];
+      }
+  }
+
   __global__ void scatterAlpha(const double *x, const u32 *glob, const u32 *interior,
                                u32 nLocal, u32 g0, u32 nClassical, double *alpha)
   {
@@ -597,6 +613,7 @@ namespace
         upload(dnv, nv, s);
         const double one = 1.0, zero = 0.0;
         auto gemm = [&](const DevBuf &A, const DevBuf &B, const DevBuf &Cc, rocblas_int kdim) {
+          CaptureLock       lk(captureMutex());
           rocblas_status st = rocblas_dgemm_batched(
             ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
             (rocblas_int)n, (rocblas_int)n, kdim, &one, A.as<const double *>(), kdim,
@@ -693,9 +710,9 @@ namespace
                                              dNe.as<u32>(), dCells.as<u32>(), dCol.as<u32>(),
                                              ncl, rhsSlots.as<double>());
         checkLaunch("extract");
-        DFX_HIP(hipStreamSynchronize(s));
         std::vector<double> hBlocks((size_t)nc * ncl * ncl);
-        DFX_HIP(hipMemcpy(hBlocks.data(), blocks.p, blocks.bytes, hipMemcpyDeviceToHost));
+        DFX_HIP(hipMemcpyAsync(hBlocks.data(), blocks.p, blocks.bytes, hipMemcpyDeviceToHost, s));
+        DFX_HIP(hipStreamSynchronize(s));
         std::vector<u32> cn(nc, ncl);
         dfx_constraints *cons = nullptr;
         dfx_op *op = nullptr, *pc = nullptr;
@@ -763,19 +780,44 @@ namespace
       return;
     const u32 ncl = (P->p + 1) * (P->p + 1) * (P->p + 1);
     P->enrM.assign((size_t)nE * nE, 0.0);
-    std::vector<double> blk;
+    // the enriched sub-blocks of the enriched cells, packed on the device and
+    // copied in one transfer; summed on the host in cell order
+    std::vector<u32> cells;
+    std::vector<u64> offs;
+    u64              total = 0;
     for (u32 c = 0; c < P->nCells; ++c)
+      if (P->cellNdofs[c] != ncl)
+        {
+          const u64 m = P->cellNdofs[c] - ncl;
+          cells.push_back(c);
+          offs.push_back(total);
+          total += m * m;
+        }
+    if (!cells.empty())
       {
-        const u32 n = P->cellNdofs[c];
-        if (n == ncl)
-          continue;
-        blk.resize((size_t)n * n);
-        DFX_HIP(hipMemcpy(blk.data(), P->M.as<double>() + P->matOff[c], blk.size() * 8,
-                          hipMemcpyDeviceToHost));
-        for (u32 j = ncl; j < n; ++j)
-          for (u32 i = ncl; i < n; ++i)
-            P->enrM[(P->cellMap[P->mapOff[c] + i] - nC) +
-                    (size_t)nE * (P->cellMap[P->mapOff[c] + j] - nC)] += blk[i + (size_t)n * j];
+        hipStream_t s = P->ctx->stream;
+        DevBuf      dCells, dOffs, dMatOff, dNe, packed((size_t)total * 8);
+        upload(dCells, cells, s);
+        upload(dOffs, offs, s);
+        upload(dMatOff, P->matOff, s);
+        upload(dNe, P->cellNdofs, s);
+        extractEnrBlockK<<<(u32)cells.size(), 256, 0, s>>>(
+          P->M.as<double>(), dMatOff.as<u64>(), dNe.as<u32>(), dCells.as<u32>(),
+          dOffs.as<u64>(), ncl, packed.as<double>());
+        checkLaunch("extractEnrBlock");
+        std::vector<double> h((size_t)total);
+        DFX_HIP(hipMemcpyAsync(h.data(), packed.p, (size_t)total * 8, hipMemcpyDeviceToHost, s));
+        DFX_HIP(hipStreamSynchronize(s));
+        for (size_t q = 0; q < cells.size(); ++q)
+          {
+            const u32    c = cells[q], n = P->cellNdofs[c], m = n - ncl;
+            const double *blk = h.data() + offs[q];
+            for (u32 j = ncl; j < n; ++j)
+              for (u32 i = ncl; i < n; ++i)
+                P->enrM[(P->cellMap[P->mapOff[c] + i] - nC) +
+                        (size_t)nE * (P->cellMap[P->mapOff[c] + j] - nC)] +=
+                  blk[(i - ncl) + (size_t)m * (j - ncl)];
+          }
       }
     // inverse through the Cholesky factor: L L^T = M, inv = L^-T L^-1
     std::vector<double> L(P->enrM);
@@ -889,7 +931,10 @@ extern "C"
       };
       auto dev = [&](const DevBuf &b, size_t bytes) {
         if (bytes)
-          DFX_HIP(hipMemcpy(host, b.p, bytes, hipMemcpyDeviceToHost));
+          {
+            DFX_HIP(hipMemcpyAsync(host, b.p, bytes, hipMemcpyDeviceToHost, P->ctx->stream));
+            DFX_HIP(hipStreamSynchronize(P->ctx->stream));
+          }
         return 0;
       };
       const size_t nm = P->matOff.back() * 8;

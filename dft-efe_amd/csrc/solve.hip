@@ -175,7 +175,8 @@ namespace dfx
           DFX_HIP(hipGraphExecDestroy(g.exec));
           g.exec = nullptr;
         }
-      hipGraph_t graph = nullptr;
+      hipGraph_t  graph = nullptr;
+      CaptureLock lk(captureMutex());
       DFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       try
         {
@@ -199,13 +200,6 @@ namespace dfx
       DFX_HIP(hipGraphLaunch(g.exec, s));
     }
 
-    inline uint64_t bitsOf(double v)
-    {
-      uint64_t u;
-      std::memcpy(&u, &v, 8);
-      return u;
-    }
-
     template <class T>
     void residualFilter(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv,
                         const double *lam, View<double> Xin, u32 m, double a,
@@ -227,16 +221,20 @@ namespace dfx
       for (u32 j = 0; j < nb; ++j)
         l2[j] = 1.0 * ones[j] * l2[j] + alpha1 * lamF[j] * l1[j];
       const double rnScale = alpha1;
-      std::vector<double> table((size_t)(m + 2) * nb + 2 * nb, 0.0);
-      std::vector<double> a1(m + 1, 0.0), a2(m + 1, 0.0);
+      // per-column mu rows 0..m+1, ones, lambda, then {alpha1, alpha2, c0}
+      // per degree: the degree loop reads every bound-dependent scalar from
+      // this device table, so a captured loop is replayed for new bounds
+      const size_t coefOff = (size_t)(m + 2) * nb + 2 * nb;
+      std::vector<double> table(coefOff + 3 * (size_t)(m + 1), 0.0);
       {
         double sg = sigma;
         for (u32 deg = 2; deg <= m; ++deg)
           {
             const double sigma2 = 1.0 / (gamma - sg);
             const double al1    = 2.0 * sigma2 / e, al2 = -(sg * sigma2);
-            a1[deg]             = al1;
-            a2[deg]             = al2;
+            table[coefOff + 3 * deg]     = al1;
+            table[coefOff + 3 * deg + 1] = al2;
+            table[coefOff + 3 * deg + 2] = c0;
             std::copy(l2.begin(), l2.end(), table.begin() + (size_t)deg * nb);
             for (u32 j = 0; j < nb; ++j)
               l1[j] = (-c0 * al1) * l2[j] + al2 * l1[j];
@@ -278,8 +276,8 @@ namespace dfx
             applyOp<T>(BInv, rn, s1);
             applyOp<T>(A, s1, s2);
           }
-        k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb, a1[deg],
-                      a2[deg], c0);
+        k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb,
+                      dtab + coefOff + 3 * (size_t)deg);
         std::swap(rn, rs);
       };
       // degree 2 runs directly (first-use allocations and kernel attributes);
@@ -288,17 +286,17 @@ namespace dfx
       if (m >= 2)
         step(2);
       const bool graph = useGraph && !A->profiling && !BInv->profiling;
-      std::vector<uint64_t> key = {(uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)BInv,
-                                   (uint64_t)(uintptr_t)rn.p, (uint64_t)(uintptr_t)rs.p,
-                                   (uint64_t)(uintptr_t)s1.p, (uint64_t)(uintptr_t)s2.p,
-                                   (uint64_t)(uintptr_t)r0.p, (uint64_t)(uintptr_t)dtab,
-                                   N, nb, m, bitsOf(c0)};
-      for (u32 d = 3; d <= m; ++d)
-        {
-          key.push_back(bitsOf(a1[d]));
-          key.push_back(bitsOf(a2[d]));
-        }
-      degreeLoop(ctx, graph, sizeof(T) == 8 ? "rf.f64" : "rf.f32", key, 3, m, step, [&] {
+      // the captured launches depend on operators, buffers and shapes only
+      // (coefficients and mu come from dtab), so one graph per block width
+      // serves every pass of a solve
+      const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)BInv,
+                                         (uint64_t)(uintptr_t)rn.p, (uint64_t)(uintptr_t)rs.p,
+                                         (uint64_t)(uintptr_t)s1.p, (uint64_t)(uintptr_t)s2.p,
+                                         (uint64_t)(uintptr_t)r0.p, (uint64_t)(uintptr_t)dtab,
+                                         N, nb, m};
+      const std::string gname = std::string(sizeof(T) == 8 ? "rf.f64." : "rf.f32.") +
+                                std::to_string(nb);
+      degreeLoop(ctx, graph, gname, key, 3, m, step, [&] {
         for (u32 d = 3; d <= m; ++d)
           std::swap(rn, rs);
       });
@@ -322,8 +320,13 @@ namespace dfx
       double *part = workDoubles(ctx, "gram.part", k::gramPartialsNeeded(N, n));
       k::gram<double>(s, cv(X), cv(MX), part, k::gramPartialsNeeded(N, n), S);
       int *info = work(ctx, "info", 64).as<int>();
-      if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, (rocblas_int)n, S, (rocblas_int)n,
-                           info) != rocblas_status_success)
+      rocblas_status st;
+      {
+        CaptureLock lk(captureMutex());
+        st = rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, (rocblas_int)n, S, (rocblas_int)n,
+                           info);
+      }
+      if (st != rocblas_status_success)
         throw Error(DFX_E_HIP, "rocsolver_dpotrf failed");
       int hinfo = 0;
       DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -340,8 +343,13 @@ namespace dfx
         for (u32 i = 0; i < j; ++i)
           hS[i + (size_t)n * j] = 0.0;
       toDevice(ctx, S, hS.data(), hS.size());
-      if (rocsolver_dtrtri(ctx->blas, rocblas_fill_lower, rocblas_diagonal_non_unit,
-                           (rocblas_int)n, S, (rocblas_int)n, info) != rocblas_status_success)
+      rocblas_status st2;
+      {
+        CaptureLock lk(captureMutex());
+        st2 = rocsolver_dtrtri(ctx->blas, rocblas_fill_lower, rocblas_diagonal_non_unit,
+                           (rocblas_int)n, S, (rocblas_int)n, info);
+      }
+      if (st2 != rocblas_status_success)
         throw Error(DFX_E_HIP, "rocsolver_dtrtri failed");
       DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
       // Xo = X L^-T: Q = (L^-1)^T
@@ -372,9 +380,13 @@ namespace dfx
       k::gram<double>(s, cv(X), cv(HX), part, k::gramPartialsNeeded(N, n), S);
       double *D    = workDoubles(ctx, "rr.D", 2 * (size_t)n);
       int    *info = work(ctx, "info", 64).as<int>();
-      if (rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower,
-                           (rocblas_int)n, S, (rocblas_int)n, D, D + n, info) !=
-          rocblas_status_success)
+      rocblas_status st;
+      {
+        CaptureLock lk(captureMutex());
+        st = rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_lower,
+                           (rocblas_int)n, S, (rocblas_int)n, D, D + n, info);
+      }
+      if (st != rocblas_status_success)
         throw Error(DFX_E_HIP, "rocsolver_dsyevd failed");
       int hinfo = 0;
       DFX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -500,8 +512,12 @@ namespace dfx
       if (kk > 1)
         toDevice(ctx, d + kk, bv.data(), kk - 1);
       int *info = work(ctx, "info", 64).as<int>();
-      if (rocsolver_dsterf(ctx->blas, (rocblas_int)kk, d, d + kk, info) !=
-          rocblas_status_success)
+      rocblas_status st;
+      {
+        CaptureLock lk(captureMutex());
+        st = rocsolver_dsterf(ctx->blas, (rocblas_int)kk, d, d + kk, info);
+      }
+      if (st != rocblas_status_success)
         throw Error(DFX_E_HIP, "rocsolver_dsterf failed");
       std::vector<double> w(kk);
       DFX_HIP(hipMemcpyAsync(w.data(), d, kk * sizeof(double), hipMemcpyDeviceToHost, s));

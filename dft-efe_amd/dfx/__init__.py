@@ -673,7 +673,8 @@ class _RequestResult(C.Structure):
 
 class _ServeParams(C.Structure):
     _fields_ = [("device", C.c_int), ("maxSlots", C.c_uint32),
-                ("memoryBudgetBytes", C.c_uint64), ("precision", C.c_int)]
+                ("memoryBudgetBytes", C.c_uint64), ("precision", C.c_int),
+                ("useGraph", C.c_int)]
 
 
 def _request_struct(r, keep):
@@ -716,15 +717,18 @@ def request_footprint(r):
     return int(out.value)
 
 
-def serve(requests, max_slots=4, memory_budget=200 << 30, precision=PREC_MIXED, device=0):
+def serve(requests, max_slots=4, memory_budget=200 << 30, precision=PREC_MIXED, device=0,
+          use_graph=True):
     """Run a stream of independent Kohn-Sham eigenproblems with continuous
-    batching (dfx_serve).  Returns (energies per request, result dicts)."""
+    batching (dfx_serve).  use_graph: each request replays its captured
+    degree loop on every pass.  Returns (energies per request, result dicts)."""
     keep = []
     arr = (_Request * len(requests))(*[_request_struct(r, keep) for r in requests])
     res = (_RequestResult * len(requests))()
     total = sum(int(r["states"]) for r in requests)
     en = np.zeros(max(total, 1))
-    p = _ServeParams(int(device), int(max_slots), int(memory_budget), int(precision))
+    p = _ServeParams(int(device), int(max_slots), int(memory_budget), int(precision),
+                     1 if use_graph else 0)
     rc = _lib.dfx_serve(C.byref(p), len(requests), arr, _ptr(en), res)
     if rc < 0:
         raise DfxError(rc, "dfx_serve failed (a request hit a library error)")

@@ -426,6 +426,7 @@ typedef struct dfx_serve_params
   uint32_t maxSlots;           /* requests in flight (one host thread + stream each) */
   uint64_t memoryBudgetBytes;  /* admission limit on the summed footprints */
   int      precision;          /* DFX_PREC_* */
+  int      useGraph;           /* replay each request's captured degree loop (dfx_ks_params) */
 } dfx_serve_params;
 /* estimated peak device bytes of one request (what admission counts) */
 int dfx_request_footprint(const dfx_request *r, uint64_t *bytes);

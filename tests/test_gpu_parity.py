@@ -30,6 +30,10 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def guess_like(n, k, seed):
+    return O.rng_uniform(seed, -1.0, 1.0, n * k).reshape(n, k)
+
+
 def mv(ctx, a, dtype=dfx.F64):
     return dfx.MultiVector.from_numpy(ctx, a, dtype)
 
@@ -140,6 +144,29 @@ def test_cell_apply_narrow_tail_chunk_bitwise(ctx, probs, dtype, B):
         ops.H.apply(mv(ctx, Xc, dtype), Yg)
         outs.append(Yg.to_numpy())
     assert np.array_equal(outs[0], np.hstack(outs[1:]))
+
+
+@pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
+def test_size_classes_merged_or_apart_give_the_same_bits(ctx, dtype, monkeypatch):
+    """Operators of few cells run each colour as one launch (the largest size
+    class's tile); large ones launch every size class of a colour apart, side
+    by side on two streams.  Both give the same bits: every cell computes the
+    same products and rows are completed in the same cell order."""
+    spec = dict(p=3, n=(4, 4, 4), lo=(-12.0,) * 3, hi=(12.0,) * 3,
+                atoms=[(0.5, -0.5, 0.25, 6)], enrich=True)
+    npd = np.float64 if dtype == dfx.F64 else np.float32
+    outs, classes = [], []
+    for below in ("0", "1000000"):
+        monkeypatch.setenv("DFX_MERGE_CLASSES_BELOW", below)
+        G = dfx.Problem(ctx, orthogonalize=True, **spec)
+        H = G.operators(keep_fp32=True)[0]
+        X = guess_like(G.nDofs, 24, seed=47).astype(npd)
+        Y = dfx.MultiVector(ctx, G.nDofs, 24, dtype)
+        H.apply(mv(ctx, X, dtype), Y)
+        outs.append(Y.to_numpy())
+        classes.append(sorted(H.autotune(dtype, 24, reps=1)))
+    assert np.array_equal(outs[0], outs[1])
+    assert classes[0] == [1, 2] and classes[1] == [2]   # apart: two classes; merged: one
 
 
 @pytest.mark.parametrize("dtype", [dfx.F64, dfx.F32])
