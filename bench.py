@@ -377,6 +377,28 @@ def main():
                 "residual_tol": 1e-6,
                 "fp64_solve_s": round(out[0][4], 2), "mixed_solve_s": round(out[1][4], 2)}
 
+    # electron density of the 200-state subspace at every quadrature point
+    # (next to the eigensolve: DensityCalculator::computeRho), first 70
+    # states occupied
+    occ = np.zeros(n)
+    occ[:70] = 1.0
+    dens = None
+    try:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rho = prob.density(occ, V)
+        ctx.synchronize()
+        t_rho = time.perf_counter() - t0
+        nquad = int(prob.get("quadOffsets")[-1])
+        dens = {"states": n, "quadrature_points": nquad, "ms": t_rho * 1e3,
+                "interpolation_gflop": 2.0 * n * float(np.sum(
+                    np.diff(prob.get("quadOffsets")).astype(np.float64) *
+                    prob.get("cellNdofs"))) / 1e9}
+        log(f"density: {nquad} quadrature points, {n} states, {t_rho * 1e3:.1f} ms")
+        del rho
+    except dfx.DfxError as e:
+        log(f"density skipped: {e}")
+
     log("Poisson CG (c2) ...")
     poisson = poisson_run(ctx)
     log(f"Poisson: rc {poisson['rc']}, {poisson['iterations']} iterations, "
@@ -432,6 +454,7 @@ def main():
         "fp64_pass_ms": t_fp64 * 1e3,
         "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
         "pass_split_ms": tim,
+        "density": dens,
         "cell_apply_tflops_fp32": tflops(prof32),
         "cell_apply_tflops_fp64": tflops(prof64),
         "cell_apply_fp64_frac_of_peak": tflops(prof64) / PEAK_FP64_MFMA,

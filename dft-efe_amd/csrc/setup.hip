@@ -325,6 +325,35 @@ namespace
       }
   }
 
+  // the wavefunction rows of each cell of a batch, B x n column-major per
+  // cell (the interpolation GEMM's A operand): Xc[t][k B + i] = X[map_c[k], i]
+  __global__ void gatherCellRowsK(const double *X, u32 ldx, u32 B, const u32 *map,
+                                  const u32 *mapOff, const u32 *cells, u32 n, double *Xc)
+  {
+    const u32 t = blockIdx.x, c = cells[t];
+    for (u32 e = threadIdx.x; e < n * B; e += blockDim.x)
+      {
+        const u32 k = e / B, i = e - k * B;
+        Xc[(u64)t * n * B + e] = X[(u64)map[mapOff[c] + k] * ldx + i];
+      }
+  }
+
+  // rho[quadOff[c] + q] = sum_i 2 psi_i(q)^2 occ[i], states in order
+  // (DensityCalculator.t.cpp:87-98); psi of cell t at psi[(t nQ + q) B + i]
+  __global__ void rhoFromPsiK(const double *psi, u32 B, u32 nQ, const u32 *cells,
+                              const u64 *quadOff, const double *occ, double *rho)
+  {
+    const u32 t = blockIdx.y, c = cells[t];
+    for (u32 q = blockIdx.x * blockDim.x + threadIdx.x; q < nQ; q += gridDim.x * blockDim.x)
+      {
+        const double *p = psi + ((u64)t * nQ + q) * B;
+        double        b = 0.0;
+        for (u32 i = 0; i < B; ++i)
+          b += 2.0 * (p[i] * p[i]) * occ[i];
+        rho[quadOff[c] + q] = b;
+      }
+  }
+
   __global__ void scatterAlpha(const double *x, const u32 *glob, const u32 *interior,
                                u32 nLocal, u32 g0, u32 nClassical, double *alpha)
   {
@@ -493,7 +522,15 @@ namespace
 
   // Tabulate + batched GEMMs for a list of cells (caller indices), writing
   // their raw blocks of H, M, K on the device.
-  void assembleCells(dfx_problem *P, const std::vector<u32> &cellList, const double *alpha)
+  // Tabulates the basis of `cellList` at the quadrature points (values F,
+  // weighted values FW, FV = F W V, gradients G, GW; nQ x n column-major per
+  // cell) in batches of one shape (n, nq) within a buffer budget, and calls
+  // batch(cells, cnt, n, nQ, F, FW, FV, G, GW) for each batch (cells: the
+  // batch's cells, in the batch's order).  extraPerCell: bytes the caller
+  // allocates per cell of a batch.
+  template <class Batch>
+  void tabulateBatches(dfx_problem *P, const std::vector<u32> &cellList, const double *alpha,
+                       size_t extraPerCell, Batch batch)
   {
     if (cellList.empty())
       return;
@@ -546,7 +583,7 @@ namespace
         // batch: cells of one shape, within the memory budget
         const u32 n = P->cellNdofs[order[i0]], nq = nqOf(order[i0]);
         const u64 nQ = (u64)nq * nq * nq;
-        const size_t per = (size_t)9 * nQ * n * sizeof(double);
+        const size_t per = (size_t)9 * nQ * n * sizeof(double) + extraPerCell;
         size_t i1 = i0;
         while (i1 < order.size() && P->cellNdofs[order[i1]] == n && nqOf(order[i1]) == nq &&
                (i1 - i0 + 1) * per <= budget)
@@ -585,13 +622,28 @@ namespace
                                       bF.as<double>(), bFW.as<double>(), bFV.as<double>(),
                                       bG.as<double>(), bGW.as<double>());
         checkLaunch("tabulate");
+        batch(order.data() + i0, cnt, n, nQ, bF, bFW, bFV, bG, bGW);
+        DFX_HIP(hipStreamSynchronize(s));
+        i0 = i1;
+      }
+  }
+
+  // cell matrices of `cellList` from the tabulated basis: M = F^T W F,
+  // K = G^T W G, H = F^T W V F + K / 2 (rocBLAS batched GEMMs)
+  void assembleCells(dfx_problem *P, const std::vector<u32> &cellList, const double *alpha)
+  {
+    dfx_ctx    *ctx = P->ctx;
+    hipStream_t s   = ctx->stream;
+    tabulateBatches(P, cellList, alpha, 0,
+                    [&](const u32 *cells, u32 cnt, u32 n, u64 nQ, DevBuf &bF, DevBuf &bFW,
+                        DevBuf &bFV, DevBuf &bG, DevBuf &bGW) {
         // pointer arrays: A (F or G), B (FW / FV / GW), C (H / M / K)
         std::vector<const double *> pF(cnt), pFW(cnt), pFV(cnt), pG(cnt), pGW(cnt);
         std::vector<double *>       pH(cnt), pM(cnt), pK(cnt);
         std::vector<u32>            nv(cnt, n);
         for (u32 t = 0; t < cnt; ++t)
           {
-            const u32 c = order[i0 + t];
+            const u32 c = cells[t];
             pF[t]  = bF.as<double>() + (u64)t * nQ * n;
             pFW[t] = bFW.as<double>() + (u64)t * nQ * n;
             pFV[t] = bFV.as<double>() + (u64)t * nQ * n;
@@ -629,9 +681,114 @@ namespace
         addHalfK<<<cnt, 256, 0, s>>>(dpH.as<double *>(), dpK.as<double *>(), dnv.as<u32>(),
                                      cnt);             // ... + K / 2
         checkLaunch("addHalfK");
+        // the pointer arrays are released when this batch returns
         DFX_HIP(hipStreamSynchronize(s));
-        i0 = i1;
+                    });
+  }
+
+  // quadrature layout of the problem's cells: cell c's points are
+  // [off[c], off[c + 1]), x fastest; W (optional) = JxW of every point
+  void quadLayout(const dfx_problem *P, std::vector<u64> &off, std::vector<double> *W)
+  {
+    const auto &S   = P->spec;
+    const u32   np  = P->p + 1, ncl = np * np * np;
+    const u32   nqC = S.nq ? S.nq : np, nqE = S.nqEnr ? S.nqEnr : 2 * np;
+    std::vector<double> xC, wC, xE, wE;
+    gaussRule(nqC, xC, wC);
+    gaussRule(nqE, xE, wE);
+    off.assign(P->nCells + 1, 0);
+    if (W)
+      W->clear();
+    for (u32 c = 0; c < P->nCells; ++c)
+      {
+        const bool enr = P->cellNdofs[c] > ncl;
+        const u32  nq  = enr ? nqE : nqC;
+        off[c + 1]     = off[c] + (u64)nq * nq * nq;
+        if (!W)
+          continue;
+        const auto  &w    = enr ? wE : wC;
+        const double detJ = P->cellH[3 * (size_t)c] * P->cellH[3 * (size_t)c + 1] *
+                            P->cellH[3 * (size_t)c + 2] / 8.0;
+        for (u32 qc = 0; qc < nq; ++qc)
+          for (u32 qb = 0; qb < nq; ++qb)
+            for (u32 qa = 0; qa < nq; ++qa)
+              W->push_back(((w[qa] * w[qb]) * w[qc]) * detJ);
       }
+  }
+
+  // DensityCalculator::computeRho (src/ksdft/DensityCalculator.t.cpp:288-420)
+  // with the interpolation of FEBasisOperations::interpolate
+  // (src/basis/FEBasisOperations.t.cpp:996-1277): per batch of cells of one
+  // shape, the basis re-tabulated at the quadrature points, the cells'
+  // wavefunction rows gathered, psi = Xc F^T by a batched GEMM, and rho
+  // reduced over the states.  One batch of states (all of X).
+  void density(dfx_problem *P, const double *occupation, View<const double> X,
+               double *rho)
+  {
+    dfx_ctx    *ctx = P->ctx;
+    hipStream_t s   = ctx->stream;
+    const u32   B   = X.B;
+    std::vector<u64> qoff;
+    quadLayout(P, qoff, nullptr);
+    DevBuf dQoff, dOcc, dMap, dMapOff;
+    upload(dQoff, qoff, s);
+    upload(dOcc, occupation, B, s);
+    upload(dMap, P->cellMap, s);
+    upload(dMapOff, P->mapOff, s);
+    std::vector<u32> all(P->nCells);
+    std::iota(all.begin(), all.end(), 0u);
+    u32 maxN = 0;
+    u64 maxQ = 0;
+    for (u32 c = 0; c < P->nCells; ++c)
+      {
+        maxN = std::max(maxN, P->cellNdofs[c]);
+        maxQ = std::max(maxQ, qoff[c + 1] - qoff[c]);
+      }
+    const double *alpha = P->alpha.p ? P->alpha.as<double>() : nullptr;
+    tabulateBatches(P, all, alpha, (size_t)(maxN + maxQ) * B * sizeof(double),
+                    [&](const u32 *cells, u32 cnt, u32 n, u64 nQ, DevBuf &bF, DevBuf &,
+                        DevBuf &, DevBuf &, DevBuf &) {
+                      DevBuf &bXc  = work(ctx, "dens.xc", (size_t)cnt * n * B * 8);
+                      DevBuf &bPsi = work(ctx, "dens.psi", (size_t)cnt * nQ * B * 8);
+                      DevBuf  dCells;
+                      upload(dCells, cells, cnt, s);
+                      gatherCellRowsK<<<cnt, 256, 0, s>>>(X.p, X.ld, B, dMap.as<u32>(),
+                                                          dMapOff.as<u32>(), dCells.as<u32>(),
+                                                          n, bXc.as<double>());
+                      checkLaunch("gatherCellRows");
+                      std::vector<const double *> pA(cnt), pB(cnt);
+                      std::vector<double *>       pC(cnt);
+                      for (u32 t = 0; t < cnt; ++t)
+                        {
+                          pA[t] = bXc.as<double>() + (u64)t * n * B;
+                          pB[t] = bF.as<double>() + (u64)t * nQ * n;
+                          pC[t] = bPsi.as<double>() + (u64)t * nQ * B;
+                        }
+                      DevBuf dpA, dpB, dpC;
+                      upload(dpA, pA, s);
+                      upload(dpB, pB, s);
+                      upload(dpC, pC, s);
+                      const double one = 1.0, zero = 0.0;
+                      {
+                        CaptureLock lk(captureMutex());
+                        rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+                        // psi (B x nQ) = Xc (B x n) F^T (F: nQ x n)
+                        if (rocblas_dgemm_batched(ctx->blas, rocblas_operation_none,
+                                                  rocblas_operation_transpose, (rocblas_int)B,
+                                                  (rocblas_int)nQ, (rocblas_int)n, &one,
+                                                  dpA.as<const double *>(), (rocblas_int)B,
+                                                  dpB.as<const double *>(), (rocblas_int)nQ,
+                                                  &zero, dpC.as<double *>(), (rocblas_int)B,
+                                                  (rocblas_int)cnt) != rocblas_status_success)
+                          throw Error(DFX_E_HIP, "rocblas_dgemm_batched failed");
+                      }
+                      rhoFromPsiK<<<dim3((u32)((nQ + 255) / 256), cnt), 256, 0, s>>>(
+                        bPsi.as<double>(), B, (u32)nQ, dCells.as<u32>(), dQoff.as<u64>(),
+                        dOcc.as<double>(), rho);
+                      checkLaunch("rhoFromPsi");
+                      // the batch's pointer arrays are released when it returns
+                      DFX_HIP(hipStreamSynchronize(s));
+                    });
   }
 
   // Per-atom L2 projection of the enrichment functions on the classical
@@ -959,8 +1116,37 @@ extern "C"
           case DFX_PROB_ALPHA:
             return P->alpha.p ? dev(P->alpha, P->alpha.bytes)
                               : cpu(nullptr, 0);
+          case DFX_PROB_QUAD_OFFSETS:
+            {
+              std::vector<u64> off;
+              quadLayout(P, off, nullptr);
+              return cpu(off.data(), off.size() * 8);
+            }
+          case DFX_PROB_QUAD_W:
+            {
+              std::vector<u64>    off;
+              std::vector<double> w;
+              quadLayout(P, off, &w);
+              return cpu(w.data(), w.size() * 8);
+            }
         }
       throw Error(DFX_E_ARG, "unknown problem array");
+    });
+  }
+
+  int dfx_problem_density(dfx_problem *P, const double *occupation, dfx_mv *X, dfx_mv *rho)
+  {
+    if (!P || !occupation || !X || !rho || X->dtype != DFX_F64 || rho->dtype != DFX_F64 ||
+        X->N != P->nDofs || rho->B != 1 || !X->B)
+      return DFX_E_ARG;
+    return guard(P->ctx, [&] {
+      std::vector<u64> off;
+      quadLayout(P, off, nullptr);
+      DFX_REQUIRE(rho->N == off.back(), "rho must hold one value per quadrature point");
+      density(P, occupation,
+              View<const double>{static_cast<const double *>(X->d), X->N, X->B, X->B},
+              static_cast<double *>(rho->d));
+      return 0;
     });
   }
 

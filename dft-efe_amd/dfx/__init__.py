@@ -115,6 +115,7 @@ _sig("dfx_problem_build", _vp, _vp, _pp)
 _sig("dfx_problem_sizes", _vp, _vp)
 _sig("dfx_problem_get", _vp, C.c_int, _vp)
 _sig("dfx_problem_create_operators", _vp, _u32, _pp, _pp, _pp, _pp, _pp, _pp)
+_sig("dfx_problem_density", _vp, _vp, _vp, _vp)
 _sig("dfx_problem_destroy", _vp)
 _sig("dfx_request_footprint", _vp, _vp)
 _sig("dfx_serve", _vp, _u32, _vp, _vp, _vp)
@@ -586,6 +587,7 @@ _PROB_ARRAYS = {  # name: (id, dtype, size key)
     "mgll": (12, np.float64, "nDofs"), "enrMinv": (13, np.float64, "nE2"),
     "enrM": (14, np.float64, "nE2"), "alpha": (15, np.float64, "nEC"),
     "nodeXYZ": (16, np.float64, "nC3"),
+    "quadOffsets": (17, np.uint64, "nCells1"), "quadW": (18, np.float64, "nQuad"),
 }
 
 
@@ -621,13 +623,25 @@ class Problem:
 
     def get(self, name):
         wid, dt, key = _PROB_ARRAYS[name]
-        n = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical,
-             "nC3": 3 * self.nClassical}.get(
-            key, getattr(self, key, 0))
+        sizes = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical,
+                 "nC3": 3 * self.nClassical, "nCells1": self.nCells + 1}
+        if key == "nQuad":
+            sizes["nQuad"] = int(self.get("quadOffsets")[-1])
+        n = sizes.get(key, getattr(self, key, 0))
         out = np.zeros(n, dtype=dt)
         if n:
             self.ctx.check(_lib.dfx_problem_get(self.h, wid, _ptr(out)))
         return out
+
+    def density(self, occupation, X):
+        """Electron density at every quadrature point (layout: get("quadOffsets"))
+        from FP64 wavefunctions X (nDofs x nStates): sum_i 2 f_i psi_i^2.
+        Returns a device MultiVector (points x 1)."""
+        occ = np.ascontiguousarray(occupation, dtype=np.float64)
+        nq = int(self.get("quadOffsets")[-1])
+        rho = MultiVector(self.ctx, nq, 1)
+        self.ctx.check(_lib.dfx_problem_density(self.h, _ptr(occ), X.h, rho.h))
+        return rho
 
     def operators(self, keep_fp32=True):
         """(H, M, K, MInv, MLump, constraints) as operator objects."""

@@ -378,6 +378,9 @@ typedef struct dfx_problem_spec
 #define DFX_PROB_ENR_M      14 /* double [nEnriched^2] M^ee */
 #define DFX_PROB_ALPHA      15 /* double [nEnriched * nClassical] orthogonalization coefficients */
 #define DFX_PROB_NODE_XYZ   16 /* double [3 nClassical] node coordinates (x, y, z per node) */
+#define DFX_PROB_QUAD_OFFSETS 17 /* uint64 [nCells + 1] cell c's quadrature points are
+                                   [off[c], off[c + 1]), x fastest within a cell */
+#define DFX_PROB_QUAD_W     18 /* double [off[nCells]] JxW of every quadrature point */
 
 int dfx_problem_build(dfx_ctx *ctx, const dfx_problem_spec *spec, dfx_problem **out);
 /* sizes[8]: nCells, nDofs, nClassical, nEnriched, sum n_e, sum n_e^2, nRows, nEntries */
@@ -389,6 +392,18 @@ int dfx_problem_get(const dfx_problem *prob, int which, void *host);
 int dfx_problem_create_operators(dfx_problem *prob, uint32_t flags, dfx_op **H,
                                  dfx_op **M, dfx_op **K, dfx_op **MInv,
                                  dfx_op **MLump, dfx_constraints **cons);
+/* Electron density at every quadrature point of the problem's cells,
+ * rho[q] = sum_i 2 occupation[i] psi_i(x_q)^2, psi_i(x_q) = sum_k N_k(x_q)
+ * X[map_e[k], i], with the problem's own basis (orthogonalized enrichment
+ * included) re-tabulated on the device.  Replaces ksdft::DensityCalculator::
+ * computeRho (src/ksdft/DensityCalculator.h:41-122, .t.cpp:288-420) with the
+ * interpolation of FEBasisOperations::interpolate (src/basis/
+ * FEBasisOperations.t.cpp:996-1277); all states in one batch; X is used as
+ * stored (no constraint distribution, as the reference).  X: FP64 nDofs x
+ * nStates; occupation: host, nStates; rho: FP64, (quadrature points) x 1,
+ * laid out by DFX_PROB_QUAD_OFFSETS. */
+int dfx_problem_density(dfx_problem *prob, const double *occupation, dfx_mv *X,
+                        dfx_mv *rho);
 int dfx_problem_destroy(dfx_problem *prob);
 
 /* ======================================================================

@@ -35,15 +35,16 @@ extern "C"
 {
   const char *orc_error() { return g_err.c_str(); }
 
-  void *orc_build(int p, const int *n, const double *lo, const double *hi,
-                  const int *refLo, const int *refHi, int nq, int nqEnr,
-                  int nAtoms, const double *atomXYZ, const int *atomZ,
-                  double vSmear, int enrich, double rcut, double smooth,
-                  int orthogonalize)
+  void *orc_build_ex(int p, const int *n, const double *lo, const double *hi,
+                     const int *refLo, const int *refHi, int nq, int nqEnr,
+                     int nAtoms, const double *atomXYZ, const int *atomZ,
+                     double vSmear, int enrich, double rcut, double smooth,
+                     int orthogonalize, int keepBasis)
   {
     try
       {
         MeshSpec s;
+        s.keepBasis = keepBasis != 0;
         s.p = p;
         for (int d = 0; d < 3; ++d)
           {
@@ -69,6 +70,44 @@ extern "C"
       {
         g_err = e.what();
         return nullptr;
+      }
+  }
+
+  void *orc_build(int p, const int *n, const double *lo, const double *hi,
+                  const int *refLo, const int *refHi, int nq, int nqEnr,
+                  int nAtoms, const double *atomXYZ, const int *atomZ,
+                  double vSmear, int enrich, double rcut, double smooth,
+                  int orthogonalize)
+  {
+    return orc_build_ex(p, n, lo, hi, refLo, refHi, nq, nqEnr, nAtoms, atomXYZ, atomZ,
+                        vSmear, enrich, rcut, smooth, orthogonalize, 0);
+  }
+
+  // quadrature layout of a problem built with keepBasis: cell e's points are
+  // [quadOff[e], quadOff[e + 1]); quadOff (nCells + 1) and JxW (quadOff[nCells])
+  int orc_quadrature(void *h, uint64_t *quadOff, double *quadW)
+  {
+    const Problem &P = *static_cast<Problem *>(h);
+    if (P.quadOff.empty())
+      return -1;
+    if (quadOff)
+      std::copy(P.quadOff.begin(), P.quadOff.end(), quadOff);
+    if (quadW)
+      std::copy(P.quadW.begin(), P.quadW.end(), quadW);
+    return 0;
+  }
+
+  int orc_density(void *h, const double *X, uint32_t B, const double *occ, double *rho)
+  {
+    try
+      {
+        density(*static_cast<Problem *>(h), X, B, occ, rho);
+        return 0;
+      }
+    catch (const std::exception &e)
+      {
+        g_err = e.what();
+        return -1;
       }
   }
 

@@ -700,6 +700,20 @@ namespace oracle
     gaussLegendre(nqC, qxC, qwC);
     gaussLegendre(nqE, qxE, qwE);
 
+    if (S.keepBasis)
+      {
+        P.basisOff.assign(P.nCells + 1, 0);
+        P.quadOff.assign(P.nCells + 1, 0);
+        for (u32 e = 0; e < P.nCells; ++e)
+          {
+            const u64 nq = P.cellNdofs[e] > (u32)nl ? qxE.size() : qxC.size();
+            P.quadOff[e + 1]  = P.quadOff[e] + nq * nq * nq;
+            P.basisOff[e + 1] = P.basisOff[e] + nq * nq * nq * P.cellNdofs[e];
+          }
+        P.basisF.assign(P.basisOff[P.nCells], 0.0);
+        P.quadW.assign(P.quadOff[P.nCells], 0.0);
+      }
+
     // cell matrices of the cells in `todo`; with `alpha`, the enriched
     // functions are phi_g - sum_l alpha[g][map(l)] N_l (orthogonalized basis)
     auto computeCells = [&](const std::vector<u32> &todo, const std::vector<double> *alpha) {
@@ -775,6 +789,11 @@ namespace oracle
                     gz[(size_t)l * nQ + q] = grad[2];
                   }
               }
+        if (S.keepBasis)
+          {
+            std::copy(f.begin(), f.end(), P.basisF.begin() + P.basisOff[e]);
+            std::copy(W.begin(), W.end(), P.quadW.begin() + P.quadOff[e]);
+          }
         double *He = &P.H[P.cellMatOffset[e]];
         double *Me = &P.M[P.cellMatOffset[e]];
         double *Ke = &P.K[P.cellMatOffset[e]];
@@ -872,5 +891,45 @@ namespace oracle
              Lm.data(), (int)nE, 0.0, P.enrMinv.data(), (int)nE);
       }
     return P;
+  }
+} // namespace oracle
+
+namespace oracle
+{
+  // ksdft::DensityCalculator::computeRho (src/ksdft/DensityCalculator.t.cpp:
+  // 288-420) on one batch holding every state: the wavefunctions are
+  // interpolated to the quadrature points cell by cell, with no constraint
+  // distribution (FEBasisOperations::interpolate, src/basis/
+  // FEBasisOperations.t.cpp:996-1277: gather, then C_e = A_e B_e^T), and
+  // rho(q) = sum_i 2 |psi_i(q)|^2 f_i summed in state order
+  // (computeRhoInBatch, DensityCalculator.t.cpp:80-101).
+  void density(const Problem &P, const double *X, u32 B, const double *occ, double *rho)
+  {
+    if (P.quadOff.empty())
+      throw std::runtime_error("density needs the basis values (build with keepBasis)");
+    std::vector<double> psi;
+    for (u32 e = 0; e < P.nCells; ++e)
+      {
+        const u32     ne  = P.cellNdofs[e];
+        const u64     nQ  = P.quadOff[e + 1] - P.quadOff[e];
+        const double *f   = &P.basisF[P.basisOff[e]];
+        const u32    *map = &P.cellMap[P.cellMapOffset[e]];
+        psi.assign((size_t)nQ * B, 0.0);
+        for (u64 q = 0; q < nQ; ++q)
+          for (u32 i = 0; i < B; ++i)
+            {
+              double v = 0.0;
+              for (u32 k = 0; k < ne; ++k)
+                v += X[(size_t)map[k] * B + i] * f[(size_t)k * nQ + q];
+              psi[q * B + i] = v;
+            }
+        for (u64 q = 0; q < nQ; ++q)
+          {
+            double b = 0.0;
+            for (u32 i = 0; i < B; ++i)
+              b += 2.0 * (psi[q * B + i] * psi[q * B + i]) * occ[i];
+            rho[P.quadOff[e] + q] = b;
+          }
+      }
   }
 } // namespace oracle

@@ -62,6 +62,7 @@ namespace oracle
     double rcut    = 4.0;        // smooth cutoff start
     double smooth  = 2.0;        // support ends at rcut (1 + 1/smooth)
     bool   orthogonalize = false;// L2-orthogonalize enrichment against the classical space
+    bool   keepBasis = false;    // keep basis values and JxW at the quadrature points
   };
 
   // Flattened constraints, same arrays as
@@ -88,6 +89,11 @@ namespace oracle
     std::vector<double> enrMinv;                  // nEnriched^2 dense (M^ee)^-1, col-major
     std::vector<double> alpha;                    // nEnriched x nClassical projection coefficients
     std::vector<int>    isHanging, isDirichlet;   // per classical node
+    // with MeshSpec::keepBasis: per cell e, basis values N_k(x_q) at
+    // basisF[basisOff[e] + k * nQ_e + q] (the final, orthogonalized basis)
+    // and JxW at quadW[quadOff[e] + q]; nQ_e = quadOff[e + 1] - quadOff[e]
+    std::vector<double> basisF, quadW;
+    std::vector<u64>    basisOff, quadOff;
   };
 
   Problem buildProblem(const MeshSpec &spec);
@@ -108,6 +114,11 @@ namespace oracle
   void distributeParentToChild(const Constraints &c, double *x, u32 B);
   void distributeChildToParent(const Constraints &c, double *y, u32 B);
   void setConstrainedNodes(const Constraints &c, double *x, u32 B, double v);
+
+  // electron density at the quadrature points (needs keepBasis):
+  // rho[quadOff[e] + q] = sum_i 2 occ[i] psi_i(x_q)^2,
+  // psi_i(x_q) = sum_k N_k(x_q) X[map_e[k], i]
+  void density(const Problem &P, const double *X, u32 B, const double *occ, double *rho);
 
   void gatherCells(const Problem &P, const double *x, u32 B, double *xCell);
   void scatterAddCells(const Problem &P, const double *yCell, u32 B, double *y);

@@ -32,13 +32,15 @@ def lib():
         L = C.CDLL(path)
         L.orc_error.restype = C.c_char_p
         L.orc_build.restype = C.c_void_p
+        L.orc_build_ex.restype = C.c_void_p
         L.orc_from_arrays.restype = C.c_void_p
         L.orc_time_apply.restype = C.c_double
         for name in ("orc_free", "orc_sizes", "orc_get_u32", "orc_get_f64",
                      "orc_p2c", "orc_c2p", "orc_gather", "orc_scatter_add",
                      "orc_apply", "orc_chebyshev_filter", "orc_residual_filter",
                      "orc_cholgs", "orc_rr", "orc_chfsi", "orc_lanczos",
-                     "orc_residual_norms", "orc_cg", "orc_time_apply"):
+                     "orc_residual_norms", "orc_cg", "orc_time_apply", "orc_quadrature",
+                     "orc_density"):
             getattr(L, name).argtypes = None
         _LIB = L
     return _LIB
@@ -168,7 +170,8 @@ class Problem:
     @classmethod
     def build(cls, p=3, n=(8, 8, 8), lo=(-10., -10., -10.), hi=(10., 10., 10.),
               ref_lo=(0, 0, 0), ref_hi=(0, 0, 0), nq=0, nq_enr=0, atoms=(),
-              v_smear=0.5, enrich=False, rcut=4.0, smooth=2.0, orthogonalize=False):
+              v_smear=0.5, enrich=False, rcut=4.0, smooth=2.0, orthogonalize=False,
+              keep_basis=False):
         n = np.array(n, dtype=np.int32)
         lo = np.array(lo, dtype=np.float64); hi = np.array(hi, dtype=np.float64)
         rl = np.array(ref_lo, dtype=np.int32); rh = np.array(ref_hi, dtype=np.int32)
@@ -176,11 +179,12 @@ class Problem:
         Z = np.array([int(a[3]) for a in atoms], dtype=np.int32)
         if xyz.size == 0:
             xyz = np.zeros(3); Z = np.zeros(1, dtype=np.int32)
-        h = lib().orc_build(C.c_int(p), _p(n), _p(lo), _p(hi), _p(rl), _p(rh),
-                            C.c_int(nq), C.c_int(nq_enr), C.c_int(len(atoms)),
-                            _p(xyz), _p(Z), C.c_double(v_smear),
-                            C.c_int(1 if enrich else 0), C.c_double(rcut),
-                            C.c_double(smooth), C.c_int(1 if orthogonalize else 0))
+        h = lib().orc_build_ex(C.c_int(p), _p(n), _p(lo), _p(hi), _p(rl), _p(rh),
+                               C.c_int(nq), C.c_int(nq_enr), C.c_int(len(atoms)),
+                               _p(xyz), _p(Z), C.c_double(v_smear),
+                               C.c_int(1 if enrich else 0), C.c_double(rcut),
+                               C.c_double(smooth), C.c_int(1 if orthogonalize else 0),
+                               C.c_int(1 if keep_basis else 0))
         P = cls(h)
         P.p = p
         return P
@@ -402,6 +406,28 @@ class Problem:
             rc = 6
         return dict(rc=rc, eig=eig, X=X, passes=passes, res=res, fermi=mu, nocc=nocc,
                     bounds=(lo, hi, beta), a=a, b=b, c=c, degree=m)
+
+    def quadrature(self):
+        """(quadOff [nCells + 1], JxW [quadOff[-1]]) of a problem built with
+        keep_basis."""
+        off = np.zeros(self.nCells + 1, dtype=np.uint64)
+        if lib().orc_quadrature(self.h, _p(off), None) != 0:
+            raise RuntimeError("problem built without keep_basis")
+        w = np.zeros(int(off[-1]))
+        lib().orc_quadrature(self.h, None, _p(w))
+        return off, w
+
+    def density(self, X, occupation):
+        """rho at every quadrature point (cell-major): sum_i 2 f_i psi_i^2."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        occ = np.ascontiguousarray(occupation, dtype=np.float64)
+        off, _ = self.quadrature()
+        rho = np.zeros(int(off[-1]))
+        rc = lib().orc_density(self.h, _p(X), C.c_uint32(X.shape[1]), _p(occ),
+                               _p(rho))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error().decode())
+        return rho
 
     def time_apply(self, kind, X, reps=3, threads=1):
         """Median seconds of the threaded CPU apply: (total, cell part, O(N B)

@@ -94,3 +94,36 @@ def test_problem_operators_apply(ctx):
         op.apply(Xg, Y)
         outs.append(Y.to_numpy())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_density_matches_oracle(ctx, name):
+    """Electron density at the quadrature points (DensityCalculator::
+    computeRho): the quadrature layout and weights as the oracle's, rho as the
+    oracle's (enriched bases differ by the orthogonalization coefficients,
+    which agree to 1e-9), and sum JxW rho = sum_i 2 f_i x_i^T M x_i with the
+    GPU problem's own M."""
+    spec = SPECS[name]
+    orth = spec.get("enrich", False)
+    G = dfx.Problem(ctx, orthogonalize=orth, **spec)
+    R = O.Problem.build(orthogonalize=orth, keep_basis=True, **spec)
+    off_r, w_r = R.quadrature()
+    assert np.array_equal(G.get("quadOffsets"), off_r)
+    w_g = G.get("quadW")
+    assert np.abs(w_g - w_r).max() <= 1e-15 * w_r.max()
+    X = O.rng_uniform(7, -1.0, 1.0, G.nDofs * 6).reshape(G.nDofs, 6)
+    occ = np.array([1.0, 1.0, 0.9, 0.5, 0.1, 0.0])
+    rho_g = G.density(occ, dfx.MultiVector.from_numpy(ctx, X)).to_numpy().ravel()
+    rho_r = R.density(X, occ)
+    assert rel(rho_g, rho_r) < (1e-8 if orth else 1e-12)
+    # the identity with the GPU problem's own mass matrix
+    nd, cmap, M = G.get("cellNdofs"), G.get("cellMap"), G.get("M")
+    MX = np.zeros_like(X)
+    off = moff = 0
+    for n in nd:
+        ids = cmap[off:off + n]
+        MX[ids] += M[moff:moff + n * n].reshape(n, n, order="F") @ X[ids]
+        off += n
+        moff += n * n
+    rhs = float(np.sum(2.0 * occ * np.sum(X * MX, axis=0)))
+    assert abs(float(np.dot(w_g, rho_g)) - rhs) < 1e-11 * abs(rhs)

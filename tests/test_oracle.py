@@ -452,3 +452,44 @@ def test_ks_eigensolve_v0_converges_to_tensor_closed_form():
     occ = r["nocc"]
     assert occ >= 1 and np.all(r["res"][:occ] <= 1e-8)
     assert abs(r["eig"][0] - all3[0]) < 1e-10
+
+
+def _assembled_mass_apply(P, X):
+    """sum_e Q_e^T M_e Q_e X with no constraints (the density's own basis)."""
+    d = P.arrays()
+    MX = np.zeros_like(X)
+    offm = offc = 0
+    for n in d["cellNdofs"]:
+        ids = d["cellMap"][offc:offc + n]
+        Me = d["M"][offm:offm + n * n].reshape(n, n, order="F")
+        MX[ids] += Me @ X[ids]
+        offc += n
+        offm += n * n
+    return MX
+
+
+@pytest.mark.parametrize("enrich", [False, True])
+def test_density_integrates_to_the_occupied_mass(enrich):
+    """sum_q JxW rho = sum_i 2 f_i x_i^T M x_i holds exactly for the
+    density's interpolation (M is assembled from the same basis values and
+    weights): a wrong factor, square, occupation index, weight or basis
+    layout breaks it.  The weights integrate the box volume."""
+    P = O.Problem.build(p=2, n=(3, 3, 3), lo=(-6.,) * 3, hi=(6.,) * 3,
+                        atoms=[(0.3, -0.2, 0.1, 6)], enrich=enrich, orthogonalize=enrich,
+                        keep_basis=True)
+    off, w = P.quadrature()
+    assert abs(w.sum() - 12.0 ** 3) < 1e-10
+    X = O.rng_uniform(3, -1, 1, P.nDofs * 5).reshape(P.nDofs, 5)
+    occ = np.array([1.0, 0.8, 0.5, 0.2, 0.0])
+    rho = P.density(X, occ)
+    assert rho.min() >= 0.0
+    rhs = float(np.sum(2.0 * occ * np.sum(X * _assembled_mass_apply(P, X), axis=0)))
+    assert abs((w * rho).sum() - rhs) < 1e-13 * abs(rhs)
+
+
+def test_density_of_the_constant_function():
+    """The Lagrange basis is a partition of unity: X = 1 on every node gives
+    psi = 1 at every quadrature point, so rho = 2 f exactly."""
+    P = O.Problem.build(p=3, n=(2, 3, 2), lo=(-3.,) * 3, hi=(3.,) * 3, keep_basis=True)
+    rho = P.density(np.ones((P.nDofs, 1)), np.array([0.75]))
+    assert np.abs(rho - 1.5).max() < 1e-13
