@@ -106,10 +106,11 @@ namespace dfx
     void gram(hipStream_t s, View<const T> X, View<const T> Y, double *partials,
               size_t partialsCap, double *S);
     // Xo = X Q  (Q n x n column-major FP64, device); Xo must not alias X;
-    // qrScratch holds n * 16 * ceil(n / 16) doubles
+    // qrScratch holds n * 16 * ceil(n / 16) doubles.  With a rocBLAS handle
+    // (bound to stream s) an FP64 rotation is one library GEMM
     template <class T>
     void rotate(hipStream_t s, View<const T> X, const double *Q, View<T> Xo,
-                double *qrScratch);
+                double *qrScratch, rocblas_handle blas = nullptr);
     size_t gramPartialsNeeded(u32 N, u32 n);
   } // namespace k
 } // namespace dfx

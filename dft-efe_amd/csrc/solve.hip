@@ -364,7 +364,7 @@ namespace dfx
           hQ[j + (size_t)n * i] = hS[i + (size_t)n * j];
       toDevice(ctx, Q, hQ.data(), hQ.size());
       double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
-      k::rotate<double>(s, cv(X), Q, Xo, qr);
+      k::rotate<double>(s, cv(X), Q, Xo, qr, ctx->blas);
       return 0;
     }
 
@@ -395,7 +395,7 @@ namespace dfx
       if (hinfo != 0)
         return DFX_EIG_LAPACK_ERROR;
       double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
-      k::rotate<double>(s, cv(X), S, Xo, qr);
+      k::rotate<double>(s, cv(X), S, Xo, qr, ctx->blas);
       return 0;
     }
 
