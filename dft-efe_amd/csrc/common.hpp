@@ -61,8 +61,10 @@ namespace dfx
   // capture that allocates (it should not) does not deadlock.
   inline std::recursive_mutex &captureMutex()
   {
-    static std::recursive_mutex m;
-    return m;
+    // never destroyed: buffers released during static destruction still
+    // find it
+    static std::recursive_mutex *m = new std::recursive_mutex;
+    return *m;
   }
   using CaptureLock = std::lock_guard<std::recursive_mutex>;
 

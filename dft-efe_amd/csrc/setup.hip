@@ -585,8 +585,9 @@ namespace
         const u64 nQ = (u64)nq * nq * nq;
         const size_t per = (size_t)9 * nQ * n * sizeof(double) + extraPerCell;
         size_t i1 = i0;
+        // (at most 65535 cells: one grid row per cell in the batch kernels)
         while (i1 < order.size() && P->cellNdofs[order[i1]] == n && nqOf(order[i1]) == nq &&
-               (i1 - i0 + 1) * per <= budget)
+               (i1 - i0 + 1) * per <= budget && i1 - i0 < 65535)
           ++i1;
         if (i1 == i0)
           ++i1;
