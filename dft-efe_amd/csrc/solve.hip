@@ -610,6 +610,31 @@ namespace dfx
   } // namespace
 } // namespace dfx
 
+namespace dfx
+{
+  namespace
+  {
+    // CG step scalars per column, as the host formed them: coef[j] = 1,
+    // coef[B + j] = zr / pw (alpha), coef[2B + j] = -zr / pw
+    __global__ void cgAlphaK(const double *zr, const double *pw, u32 B, double *coef)
+    {
+      const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+      if (j >= B)
+        return;
+      coef[j]         = 1.0;
+      coef[B + j]     = zr[j] / pw[j];
+      coef[2 * B + j] = -zr[j] / pw[j];
+    }
+    // beta[j] = zrNew / zr
+    __global__ void cgBetaK(const double *zrNew, const double *zr, u32 B, double *beta)
+    {
+      const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+      if (j < B)
+        beta[j] = zrNew[j] / zr[j];
+    }
+  } // namespace
+} // namespace dfx
+
 extern "C"
 {
   int dfx_chfsi_create(dfx_ctx *ctx, const dfx_chfsi_params *p, dfx_mv *guess,
@@ -816,7 +841,11 @@ extern "C"
     });
   }
 
-  // CGLinearSolver::solve (src/linearAlgebra/CGLinearSolver.t.cpp:68-313)
+  // CGLinearSolver::solve (src/linearAlgebra/CGLinearSolver.t.cpp:68-313).
+  // The per-column scalars alpha = z.r / p.w and beta = z.r_new / z.r are
+  // formed on the device from device-resident dots (the same IEEE divisions
+  // as on the host), so one iteration synchronizes once, for the residual
+  // norms of the convergence test.
   int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *bmv, dfx_mv *xmv,
                    double absTol, double relTol, double divTol, uint32_t maxIter,
                    uint32_t *iterations, double *residualNorms)
@@ -833,8 +862,12 @@ extern "C"
       View<double> z  = workView<double>(ctx, "cgs.z", N, B);
       View<double> pv = workView<double>(ctx, "cgs.p", N, B);
       double *coef = workDoubles(ctx, "cgs.coef", 4 * (size_t)B);
+      // device dots: z.r (two, alternating), p.w, r.r; partial sums
+      double *dots = workDoubles(ctx, "cgs.dots", 4 * (size_t)B);
+      double *part = workDoubles(ctx, "cgs.part", 1024 * (size_t)B);
+      double *dZr = dots, *dZrNew = dots + B, *dPw = dots + 2 * B, *dRr = dots + 3 * B;
       k::convert<double, double>(s, x, cv(xo));
-      std::vector<double> bNorm(B), rNorm(B), zr(B), pw(B), zrNew(B), h(4 * (size_t)B);
+      std::vector<double> bNorm(B), rNorm(B);
       colDotsView<double>(ctx, cv(b), cv(b), bNorm.data());
       for (auto &v : bNorm)
         v = std::sqrt(v);
@@ -850,29 +883,27 @@ extern "C"
               k::axpby<double>(s, r, 1.0, cv(b), -1.0, cv(w));
               applyOp<double>(pc, r, z);
               k::convert<double, double>(s, pv, cv(z));
+              k::colDots<double>(s, cv(z), cv(r), part, dZr);
             }
           else
             {
+              // z.r of this iteration is the previous iteration's z.r_new
               applyOp<double>(A, pv, w);
-              colDotsView<double>(ctx, cv(z), cv(r), zr.data());
-              colDotsView<double>(ctx, cv(pv), cv(w), pw.data());
-              for (u32 j = 0; j < B; ++j)
-                {
-                  h[j]         = 1.0;
-                  h[B + j]     = zr[j] / pw[j];
-                  h[2 * B + j] = -zr[j] / pw[j];
-                }
-              toDevice(ctx, coef, h.data(), 3 * (size_t)B);
+              k::colDots<double>(s, cv(pv), cv(w), part, dPw);
+              cgAlphaK<<<(B + 63) / 64, 64, 0, s>>>(dZr, dPw, B, coef);
+              checkLaunch("cgAlpha");
               k::axpbyBlocked<double>(s, x, 1.0, coef, cv(x), 1.0, coef + B, cv(pv));
               k::axpbyBlocked<double>(s, r, 1.0, coef, cv(r), 1.0, coef + 2 * B, cv(w));
               applyOp<double>(pc, r, z);
-              colDotsView<double>(ctx, cv(z), cv(r), zrNew.data());
-              for (u32 j = 0; j < B; ++j)
-                h[3 * B + j] = zrNew[j] / zr[j];
-              toDevice(ctx, coef + 3 * B, h.data() + 3 * B, B);
+              k::colDots<double>(s, cv(z), cv(r), part, dZrNew);
+              cgBetaK<<<(B + 63) / 64, 64, 0, s>>>(dZrNew, dZr, B, coef + 3 * B);
+              checkLaunch("cgBeta");
               k::axpbyBlocked<double>(s, pv, 1.0, coef, cv(z), 1.0, coef + 3 * B, cv(pv));
+              std::swap(dZr, dZrNew);
             }
-          colDotsView<double>(ctx, cv(r), cv(r), rNorm.data());
+          k::colDots<double>(s, cv(r), cv(r), part, dRr);
+          DFX_HIP(hipMemcpyAsync(rNorm.data(), dRr, B * sizeof(double), hipMemcpyDeviceToHost, s));
+          DFX_HIP(hipStreamSynchronize(s));
           for (auto &v : rNorm)
             v = std::sqrt(v);
           std::vector<double> mask(2 * (size_t)B, 0.0);
