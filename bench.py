@@ -118,6 +118,51 @@ def serving_run(n_requests, rate, slots, seed=5, use_graph=True):
                       "6^3-14^3 cells at p=3, 5-40 states, 1e-6 residual"}
 
 
+def c3_run(ctx, max_passes=40):
+    """C3: enriched FE ChFSI (4 atoms, 16^3 cells at p = 3, 40 eigenpairs,
+    degree 40) solved to a 1e-6 residual once in FP64 and once with the FP32
+    filter, from the same start: passes, time per pass, eigenvalue
+    agreement."""
+    import numpy as np
+    import dfx
+    from dfx import configs
+    cfg = configs.config("c3")
+    prob = dfx.Problem(ctx, p=cfg["p"], n=cfg["n"], lo=cfg["lo"], hi=cfg["hi"],
+                       atoms=cfg["atoms"], enrich=cfg["enrich"], orthogonalize=True)
+    H, M, K, Minv, Mlump, cons = prob.operators(keep_fp32=True)
+    N, n = prob.nDofs, cfg["states"]
+    out = {}
+    for prec in (dfx.PREC_FP64, dfx.PREC_MIXED):
+        G = dfx.MultiVector(ctx, N, n)
+        G.fill_random(1, -1.0, 1.0)
+        cons.distribute_parent_to_child(G)
+        L = dfx.MultiVector(ctx, N, 1)
+        L.fill_random(2, -1.0, 1.0)
+        cons.distribute_parent_to_child(L)
+        W = dfx.MultiVector(ctx, N, n)
+        ks = dfx.KohnShamEigenSolver(ctx, n, 1e-6, max_passes, G, L,
+                                     polynomialDegree=cfg["degree"],
+                                     waveFunctionBatchSize=cfg["block"], precision=prec,
+                                     numElectrons=cfg["electrons"])
+        e = np.zeros(n)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rc = ks.solve(H, e, W, M, Minv, Mlump)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        out[prec] = (rc, e, ks.result, dt)
+    occ = int(out[dfx.PREC_FP64][2]["numOccupied"])
+    f64, mix = out[dfx.PREC_FP64], out[dfx.PREC_MIXED]
+    return {"workload": f"c3: {len(cfg['atoms'])} atoms, 16^3 cells, p=3, {N} dofs, "
+                        f"{n} states, degree {cfg['degree']}",
+            "fp64_rc": int(f64[0]), "fp64_passes": int(f64[2]["passes"]),
+            "fp64_ms_per_pass": f64[3] * 1e3 / max(int(f64[2]["passes"]), 1),
+            "mixed_rc": int(mix[0]), "mixed_passes": int(mix[2]["passes"]),
+            "mixed_ms_per_pass": mix[3] * 1e3 / max(int(mix[2]["passes"]), 1),
+            "occupied_states": occ,
+            "max_eig_err_mixed_vs_fp64_Ha": float(np.abs(mix[1][:occ] - f64[1][:occ]).max())}
+
+
 def poisson_run(ctx):
     """C2: batched Poisson CG on the electrostatics path -- 16^3 cells at p=3
     in a 20-bohr cube, zero Dirichlet, Jacobi preconditioner, relative
@@ -399,6 +444,14 @@ def main():
     except dfx.DfxError as e:
         log(f"density skipped: {e}")
 
+    c3 = None
+    if not args.no_convergence:
+        log("C3 enriched ChFSI, FP64 and mixed ...")
+        c3 = c3_run(ctx)
+        log(f"C3: FP64 {c3['fp64_passes']} passes {c3['fp64_ms_per_pass']:.1f} ms/pass, mixed "
+            f"{c3['mixed_passes']} passes {c3['mixed_ms_per_pass']:.1f} ms/pass, max occupied "
+            f"eigenvalue difference {c3['max_eig_err_mixed_vs_fp64_Ha']:.2e} Ha")
+
     log("Poisson CG (c2) ...")
     poisson = poisson_run(ctx)
     log(f"Poisson: rc {poisson['rc']}, {poisson['iterations']} iterations, "
@@ -455,6 +508,7 @@ def main():
         "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
         "pass_split_ms": tim,
         "density": dens,
+        "c3": c3,
         "cell_apply_tflops_fp32": tflops(prof32),
         "cell_apply_tflops_fp64": tflops(prof64),
         "cell_apply_fp64_frac_of_peak": tflops(prof64) / PEAK_FP64_MFMA,
