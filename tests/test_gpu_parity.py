@@ -262,10 +262,15 @@ def test_chebyshev_filter_fp64(ctx, probs, name, residual):
 
 @pytest.mark.parametrize("name", ["c1", "enr"])
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
-def test_residual_filter_folded_inverse_is_bit_identical(ctx, probs, name, prec):
+@pytest.mark.parametrize("merge_below", ["4096", "0"])
+def test_residual_filter_folded_inverse_is_bit_identical(ctx, probs, name, prec, merge_below,
+                                                        monkeypatch):
     """The residual filter folds M^-1 into the H apply's gather when both share
     homogeneous Dirichlet constraints.  With M^-1 built on a second, identical
-    constraints object the folding does not apply: both runs give the same bits."""
+    constraints object the folding does not apply: both runs give the same bits.
+    Also with every size class launched apart (as for large operators: classes
+    side by side on two streams)."""
+    monkeypatch.setenv("DFX_MERGE_CLASSES_BELOW", merge_below)
     P = probs[name]
     ops = Ops(ctx, P)
     d = ops.d
