@@ -117,6 +117,13 @@ _sig("dfx_problem_get", _vp, C.c_int, _vp)
 _sig("dfx_problem_create_operators", _vp, _u32, _pp, _pp, _pp, _pp, _pp, _pp)
 _sig("dfx_problem_density", _vp, _vp, _vp, _vp)
 _sig("dfx_problem_destroy", _vp)
+_sig("dfx_atom_read", C.c_char_p, _vp, _u32, _pp)
+_sig("dfx_atom_metadata", _vp, C.c_char_p, C.c_char_p, C.c_uint64, _vp)
+_sig("dfx_atom_radial_grid", _vp, _vp, _vp)
+_sig("dfx_atom_num_functions", _vp, C.c_char_p, _vp)
+_sig("dfx_atom_function", _vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_atom_evaluate", _vp, _vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp)
+_sig("dfx_atom_destroy", _vp)
 _sig("dfx_request_footprint", _vp, _vp)
 _sig("dfx_serve", _vp, _u32, _vp, _vp, _vp)
 
@@ -589,6 +596,105 @@ _PROB_ARRAYS = {  # name: (id, dtype, size key)
     "nodeXYZ": (16, np.float64, "nC3"),
     "quadOffsets": (17, np.uint64, "nCells1"), "quadW": (18, np.float64, "nQuad"),
 }
+
+
+def _host_check(rc):
+    if rc < 0:
+        raise DfxError(rc, _lib.dfx_ctx_last_error(None).decode())
+    return rc
+
+
+class SphericalDataNumerical:
+    """One radial function of an atom file: R(r) Y_lm f_c(r) with R the
+    zero-end-slope cubic spline through the file's values
+    (atoms::SphericalDataNumerical, src/atoms/SphericalDataNumerical.h:40-140)."""
+
+    def __init__(self, atom, field, k):
+        self.atom, self.field, self.k = atom, field, k
+        NR = atom.NR
+        q = np.zeros(3, dtype=np.int32)
+        cut, sm = C.c_double(), C.c_double()
+        self.values = np.zeros(NR)
+        self.spline_b, self.spline_c, self.spline_d = np.zeros(NR), np.zeros(NR), np.zeros(NR)
+        _host_check(_lib.dfx_atom_function(atom.h, field.encode(), k, _ptr(q), C.byref(cut),
+                                           C.byref(sm), _ptr(self.values), _ptr(self.spline_b),
+                                           _ptr(self.spline_c), _ptr(self.spline_d)))
+        self.qNumbers = [int(v) for v in q]
+        self.cutoff, self.smoothness = cut.value, sm.value
+
+    def getQNumbers(self):
+        return list(self.qNumbers)
+
+    def evaluate(self, ctx, points, origin=(0.0, 0.0, 0.0), gradient=True):
+        """Values (and gradients) at points (n x 3) on the device; returns
+        numpy arrays (n,) and (n, 3)."""
+        P = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        n = P.shape[0]
+        pm = MultiVector(ctx, n, 3)
+        if n:
+            pm.upload(P)
+        vm = MultiVector(ctx, n, 1)
+        gm = MultiVector(ctx, n, 3) if gradient else None
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        ctx.check(_lib.dfx_atom_evaluate(ctx.h, self.atom.h, self.field.encode(), self.k,
+                                         _ptr(o), pm.h, vm.h, gm.h if gm else None))
+        v = vm.to_numpy().reshape(n) if n else np.zeros(0)
+        g = (gm.to_numpy() if n else np.zeros((0, 3))) if gm else None
+        return (v, g) if gradient else v
+
+
+class AtomSphericalDataEnrichment:
+    """An atom file of the reference's enrichment XML format
+    (atoms::AtomSphericalDataEnrichment(fileName, fieldNames, metadataNames),
+    src/atoms/AtomSphericalDataEnrichment.h:41-89): metadata elements, the
+    radial grid and, per field, its numerical spherical functions."""
+
+    def __init__(self, fileName, fieldNames, metadataNames=()):
+        names = [f.encode() for f in fieldNames]
+        arr = (C.c_char_p * max(1, len(names)))(*names)
+        h = C.c_void_p()
+        _host_check(_lib.dfx_atom_read(os.fspath(fileName).encode(), arr, len(names),
+                                       C.byref(h)))
+        self.h = h
+        self.fileName = os.fspath(fileName)
+        nr = C.c_uint32()
+        _lib.dfx_atom_radial_grid(self.h, C.byref(nr), None)
+        self.NR = nr.value
+        self._metadata = {m: self.getMetadata(m) for m in metadataNames}
+        self._data = {}
+        for f in fieldNames:
+            cnt = C.c_uint32()
+            _host_check(_lib.dfx_atom_num_functions(self.h, f.encode(), C.byref(cnt)))
+            self._data[f] = [SphericalDataNumerical(self, f, k) for k in range(cnt.value)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.dfx_atom_destroy(self.h)
+            self.h = None
+
+    def getFileName(self):
+        return self.fileName
+
+    def getMetadata(self, name):
+        need = C.c_uint64()
+        _host_check(_lib.dfx_atom_metadata(self.h, name.encode(), None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _host_check(_lib.dfx_atom_metadata(self.h, name.encode(), buf, need.value, None))
+        return buf.value.decode()
+
+    def radialGrid(self):
+        r = np.zeros(self.NR)
+        _host_check(_lib.dfx_atom_radial_grid(self.h, None, _ptr(r)))
+        return r
+
+    def getSphericalData(self, fieldName, qNumbers=None):
+        funs = self._data[fieldName]
+        if qNumbers is None:
+            return funs
+        for f in funs:
+            if f.qNumbers == list(qNumbers):
+                return f
+        raise KeyError(f"no function with quantum numbers {qNumbers} in {fieldName}")
 
 
 class Problem:

@@ -40,6 +40,7 @@ typedef struct dfx_op          dfx_op;
 typedef struct dfx_chfsi       dfx_chfsi;
 typedef struct dfx_problem     dfx_problem;
 typedef struct dfx_server      dfx_server;
+typedef struct dfx_atom        dfx_atom;
 
 /* ---- status codes ---------------------------------------------------- */
 #define DFX_SUCCESS          0
@@ -333,6 +334,52 @@ int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,
 int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *b, dfx_mv *x,
                  double absTol, double relTol, double divTol, uint32_t maxIter,
                  uint32_t *iterations, double *residualNorms);
+
+/* ======================================================================
+ * Atom data files (the reference's enrichment XML format) and their
+ * numerical radial functions.  Replaces AtomSphericalDataEnrichment (src/atoms/
+ * AtomSphericalDataEnrichment.h:41-89, .cpp:511-640: libxml2 XPath reads of
+ * /dft-efe:atom/dft-efe:<name>) and SphericalDataNumerical (src/atoms/
+ * SphericalDataNumerical.h:40-140, .cpp:40-140, 323-420) for l <= 1.
+ * Host-side reads; only dfx_atom_evaluate touches the GPU.
+ * ==================================================================== */
+
+/* Parse `path` and keep the functions of every listed field (e.g. "orbital",
+ * "vnuclear"): per field element its qNumbers (n l m), cutoffInfo (cutoff,
+ * smoothness) and NR radial values, each interpolated by the C^2 cubic spline
+ * with zero end slopes (utils::Spline defaults, src/utils/Spline.h:87-95).
+ * Errors (DFX_E_ARG, message in dfx_ctx_last_error(NULL)) as the reference's
+ * checks: unparsable file, missing or repeated single elements, NR not
+ * positive or not matching <r>, value counts, |m| > l, repeated quantum
+ * numbers, incomplete m = -l..l shells; also a radial grid that is not
+ * strictly increasing or has fewer than two points (the spline needs it). */
+int dfx_atom_read(const char *path, const char *const *fieldNames, uint32_t nFields,
+                  dfx_atom **out);
+/* Trimmed text of the single child element `name` of <atom> (symbol, Z,
+ * charge, NR, ...): AtomSphericalDataEnrichment::getMetadata (.h:74-75).
+ * Writes at most len bytes including the terminating 0; *needed = length + 1. */
+int dfx_atom_metadata(const dfx_atom *a, const char *name, char *buf, uint64_t len,
+                      uint64_t *needed);
+/* The radial grid (NR points; r may be NULL to query NR). */
+int dfx_atom_radial_grid(const dfx_atom *a, uint32_t *NR, double *r);
+/* Functions of a field, in file order (getSphericalData, .h:56-62). */
+int dfx_atom_num_functions(const dfx_atom *a, const char *field, uint32_t *count);
+/* Function k of a field: quantum numbers, cutoff info, radial values and the
+ * spline coefficients (value y_i + b_i h + c_i h^2 + d_i h^3 at h = r - r_i on
+ * [r_i, r_i+1); left of r_0 y_0 + b_0 h; right of r_{NR-1} quadratic).  Any
+ * output may be NULL; arrays hold NR values. */
+int dfx_atom_function(const dfx_atom *a, const char *field, uint32_t k, int *qNumbers,
+                      double *cutoff, double *smoothness, double *values, double *splineB,
+                      double *splineC, double *splineD);
+/* On the device: f(x) = R(r) Y_lm(theta, phi) f_c(r) of function k at every
+ * point (points: FP64 nPts x 3), relative to origin (host, 3 values), and its
+ * gradient (FP64 nPts x 3, may be NULL): SphericalDataNumerical::getValue /
+ * getGradientValue (.cpp:330-420).  Zero past cutoff (1 + 1/smoothness); at the
+ * origin itself the gradient is 0 (the reference asserts r > 0 there).
+ * l > 1: DFX_E_UNSUPPORTED. */
+int dfx_atom_evaluate(dfx_ctx *ctx, const dfx_atom *a, const char *field, uint32_t k,
+                      const double *origin, dfx_mv *points, dfx_mv *values, dfx_mv *gradients);
+int dfx_atom_destroy(dfx_atom *a);
 
 /* ======================================================================
  * Problem builder: structured hex mesh, FE_Q(p) on GLL points, classical +
