@@ -24,23 +24,9 @@
 #include <sstream>
 
 #include "api_util.hpp"
+#include "atom.hpp"
 
 using namespace dfx;
-
-// ------------------------------------------------------------------ data
-struct dfx_atom
-{
-  struct Function
-  {
-    int                 q[3]; // n, l, m
-    double              cutoff, smoothness;
-    std::vector<double> y, b, c, d; // spline: y + b h + c h^2 + d h^3 on [r_i, r_i+1)
-  };
-  std::string                                     file;
-  std::map<std::string, std::vector<std::string>> text; // direct children of <atom>
-  std::vector<double>                             r;
-  std::map<std::string, std::vector<Function>>   fields;
-};
 
 namespace
 {
@@ -334,56 +320,6 @@ namespace
     F.d[n - 1]     = 0.0;
   }
 
-  // ------------------------------------------------------------ device
-  struct SplineDev
-  {
-    const double *x, *y, *b, *c, *d;
-    u32           n;
-  };
-
-  // value and slope of the spline at r (Spline.cpp:490-518 semantics: left of
-  // x_0 linear with slope b_0, right of x_{n-1} quadratic)
-  __device__ void splineEval(const SplineDev &S, double r, double &v, double &dv)
-  {
-    // largest i with x_i <= r, 0 if none
-    u32 lo = 0, hi = S.n; // x[lo] <= r < x[hi] (virtual x[n] = +inf)
-    if (r < S.x[0])
-      {
-        const double h = r - S.x[0];
-        v              = S.b[0] * h + S.y[0];
-        dv             = S.b[0];
-        return;
-      }
-    while (hi - lo > 1)
-      {
-        const u32 mid = (lo + hi) >> 1;
-        if (S.x[mid] <= r)
-          lo = mid;
-        else
-          hi = mid;
-      }
-    const double h = r - S.x[lo];
-    if (lo == S.n - 1)
-      {
-        v  = (S.c[lo] * h + S.b[lo]) * h + S.y[lo];
-        dv = 2.0 * S.c[lo] * h + S.b[lo];
-        return;
-      }
-    v  = ((S.d[lo] * h + S.c[lo]) * h + S.b[lo]) * h + S.y[lo];
-    dv = (3.0 * S.d[lo] * h + 2.0 * S.c[lo]) * h + S.b[lo];
-  }
-
-  __device__ double cutoffVal(double r, double rc, double d, double &dv)
-  {
-    auto f1  = [](double x) { return x <= 0.0 ? 0.0 : exp(-1.0 / x); };
-    auto f1d = [&](double x) { return x <= 0.0 ? 0.0 : f1(x) / (x * x); };
-    const double y = 1.0 - d * (r - rc) / rc;
-    const double a = f1(y), b = f1(1.0 - y);
-    const double s = a + b;
-    dv             = ((f1d(y) * b + a * f1d(1.0 - y)) / (s * s)) * (-d / rc);
-    return a / s;
-  }
-
   // f(x) = R(r) Y_lm(x / r) f_c(r) and its gradient at every point (rows of
   // P, 3 values each) relative to the origin; l <= 1
   __global__ void atomEvalK(SplineDev S, int l, int m, double rc, double sm, double ox, double oy,
@@ -393,33 +329,8 @@ namespace
     if (i >= nPts)
       return;
     const double dx[3] = {P[3 * (u64)i] - ox, P[3 * (u64)i + 1] - oy, P[3 * (u64)i + 2] - oz};
-    const double r     = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
-    double       f = 0.0, g[3] = {0.0, 0.0, 0.0};
-    if (r <= rc + rc / sm)
-      {
-        double R, dR, dfc;
-        splineEval(S, r, R, dR);
-        const double fc = cutoffVal(r, rc, sm, dfc);
-        const double u  = R * fc, du = dR * fc + R * dfc; // radial part and its slope
-        if (l == 0)
-          {
-            const double y00 = 0.28209479177387814; // 1 / (2 sqrt(pi))
-            f                = y00 * u;
-            if (r > 0.0)
-              for (int k = 0; k < 3; ++k)
-                g[k] = y00 * du * dx[k] / r;
-          }
-        else if (r > 0.0)
-          {
-            // Y_1m = sqrt(3 / (4 pi)) (y, z, x)_m / r for m = -1, 0, 1
-            const double c1   = 0.48860251190291992;
-            const int    comp = m == -1 ? 1 : (m == 0 ? 2 : 0);
-            const double q    = u / r, dq = du / r - u / (r * r); // f = c1 x_comp q(r)
-            f                 = c1 * dx[comp] * q;
-            for (int k = 0; k < 3; ++k)
-              g[k] = c1 * (dx[comp] * dq * dx[k] / r + (k == comp ? q : 0.0));
-          }
-      }
+    double       f, g[3];
+    radialYlm(S, l, m, rc, sm, dx, f, g);
     val[i] = f;
     if (grad)
       {

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "api_util.hpp"
+#include "atom.hpp"
 
 namespace dfx
 {
@@ -39,9 +40,11 @@ namespace dfx
       std::vector<u32> extra(nCells, 0);
       if (s.enrich && s.nAtoms)
         {
-          const double rs = s.rcut * (1.0 + 1.0 / s.smooth);
           for (u32 a = 0; a < s.nAtoms; ++a)
             {
+              u32    nf;
+              double rs;
+              atomEnrichment(s, a, nf, rs);
               u32 lo[3], hi[3];
               for (int d = 0; d < 3; ++d)
                 {
@@ -55,7 +58,7 @@ namespace dfx
               for (u32 k = lo[2]; k <= hi[2]; ++k)
                 for (u32 j = lo[1]; j <= hi[1]; ++j)
                   for (u32 i = lo[0]; i <= hi[0]; ++i)
-                    extra[((u64)k * s.n[1] + j) * s.n[0] + i] += 5;
+                    extra[((u64)k * s.n[1] + j) * s.n[0] + i] += nf;
             }
         }
       sumN = sumN2 = 0;
@@ -79,7 +82,15 @@ namespace dfx
       cellSizes(r.spec, sumN, sumN2, maxN, nCells);
       const u32 p  = r.spec.p;
       const u64 nC = (u64)(r.spec.n[0] * p + 1) * (r.spec.n[1] * p + 1) * (r.spec.n[2] * p + 1);
-      const u64 N  = nC + (r.spec.enrich ? 5ull * r.spec.nAtoms : 0);
+      u64 nE = 0;
+      for (u32 a = 0; r.spec.enrich && a < r.spec.nAtoms; ++a)
+        {
+          u32    nf;
+          double rs;
+          atomEnrichment(r.spec, a, nf, rs);
+          nE += nf;
+        }
+      const u64 N = nC + nE;
       const u64 nq = r.spec.nqEnr ? r.spec.nqEnr : 2 * (p + 1);
       const u64 nQ = nq * nq * nq;
       const u64 mats  = 3 * 8 * sumN2 + 3 * 12 * (sumN2 + sumN2 / 4);

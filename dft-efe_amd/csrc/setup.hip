@@ -21,13 +21,14 @@
 #include <numeric>
 
 #include "api_util.hpp"
+#include "atom.hpp"
 #include "kernels.hpp"
 
 using namespace dfx;
 
 namespace
 {
-  constexpr int NFUN = 5; // 1s, 2s, 2p_y, 2p_z, 2p_x
+  constexpr int NFUN = 5; // analytic set: 1s, 2s, 2p_y, 2p_z, 2p_x
 
   // ----------------------------------------------------------- quadrature
   void legendre(int n, double x, double &p, double &dp)
@@ -188,12 +189,40 @@ namespace
       }
   }
 
+  // one enriched basis function: analytic Slater function `kind` (< NFUN)
+  // of a nucleus of charge Z, or (kind == NFUN) R(r) Y_lm f_c(r) with R the
+  // spline at spl + splOff (knots, y, b, c, d: splN values each)
+  struct EnrFun
+  {
+    double r[3];
+    double rc, sm;
+    int    Z, kind, l, m;
+    u32    splOff, splN;
+  };
+
+  __device__ void enrichFun(const EnrFun &E, const double *spl, const double *x, double &val,
+                            double *g)
+  {
+    if (E.kind < NFUN)
+      {
+        const AtomDev A{{E.r[0], E.r[1], E.r[2]}, E.Z, 0};
+        enrichDev(A, E.kind, E.rc, E.sm, x, val, g);
+        return;
+      }
+    const double *b = spl + E.splOff;
+    const SplineDev S{b, b + E.splN, b + 2 * (u64)E.splN, b + 3 * (u64)E.splN,
+                      b + 4 * (u64)E.splN, E.splN};
+    const double dx[3] = {x[0] - E.r[0], x[1] - E.r[1], x[2] - E.r[2]};
+    radialYlm(S, E.l, E.m, E.rc, E.sm, dx, val, g);
+  }
+
   // One block per cell; threads over Gauss points.  Writes, column-major in
   // (point, function):  F, FW = W F, FV = W V F  (nQ x n) and G, GW (3 nQ x n).
   __global__ void tabulateK(const TabCell *cells, const u32 *map, u32 nClassical,
                             const double *tabL, const double *tabD, const double *tabX,
                             const double *tabW, int np, const AtomDev *atoms, u32 nAtoms,
-                            double smear, double rc, double sm, const double *alpha,
+                            double smear, const EnrFun *funs, const double *spl,
+                            const double *alpha,
                             double *F, double *FW, double *FV, double *G, double *GW)
   {
     const TabCell C  = cells[blockIdx.x];
@@ -246,7 +275,7 @@ namespace
           {
             const u32 g = cmap[j] - nClassical;
             double    f, gr[3];
-            enrichDev(atoms[g / NFUN], g % NFUN, rc, sm, x, f, gr);
+            enrichFun(funs[g], spl, x, f, gr);
             if (alpha)
               {
                 const double *al = alpha + (u64)g * nClassical;
@@ -295,16 +324,16 @@ namespace
       }
   }
 
-  // rhs slots: out[(i*ncl + r)*5 + k] = raw_e[r, col_k]
+  // rhs slots: out[(i*ncl + r)*nf + k] = raw_e[r, col_k]
   __global__ void extractMixedBlock(const double *raw, const u64 *rawOff, const u32 *ne,
                                     const u32 *cellList, const u32 *colStart, u32 ncl,
-                                    double *out)
+                                    u32 nf, double *out)
   {
     const u32 i = blockIdx.x, e = cellList[i];
-    for (u32 t = threadIdx.x; t < ncl * NFUN; t += blockDim.x)
+    for (u32 t = threadIdx.x; t < ncl * nf; t += blockDim.x)
       {
-        const u32 r = t / NFUN, k = t % NFUN;
-        out[(u64)i * ncl * NFUN + t] =
+        const u32 r = t / nf, k = t % nf;
+        out[(u64)i * ncl * nf + t] =
           raw[rawOff[e] + r + (u64)ne[e] * (colStart[i] + k)];
       }
   }
@@ -355,12 +384,12 @@ namespace
   }
 
   __global__ void scatterAlpha(const double *x, const u32 *glob, const u32 *interior,
-                               u32 nLocal, u32 g0, u32 nClassical, double *alpha)
+                               u32 nLocal, u32 g0, u32 nf, u32 nClassical, double *alpha)
   {
-    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < nLocal * NFUN;
+    for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < nLocal * nf;
          t += gridDim.x * blockDim.x)
       {
-        const u32 i = t / NFUN, k = t % NFUN;
+        const u32 i = t / nf, k = t % nf;
         if (interior[i])
           alpha[(u64)(g0 + k) * nClassical + glob[i]] = x[t];
       }
@@ -381,6 +410,11 @@ struct dfx_problem
   std::vector<u32>    rows, rowSizes, rowOffsets, cols;
   std::vector<double> weights, inhom, mgll, dinv, enrM, enrMinv;
   std::vector<std::vector<u32>> atomCells; // enrichment supports
+  std::vector<const dfx_atom *> atomData;  // numerical enrichment (empty: analytic)
+  std::string                   enrichField;
+  std::vector<u32>    funOff;               // atom a: enriched ids funOff[a] .. funOff[a+1]-1
+  std::vector<EnrFun> funs;                 // per enriched id
+  std::vector<double> splines;              // numerical functions' knots and coefficients
   DevBuf H, M, K, alpha;                    // raw (caller-layout) cell matrices
 };
 
@@ -459,16 +493,63 @@ namespace
     P->inhom.assign(P->rows.size(), 0.0);
 
     // enrichment supports: cells whose box meets the atom's support cube
-    const u32    nA = S.enrich ? S.nAtoms : 0;
-    const double rs = S.rcut + S.rcut / S.smooth;
-    P->nEnriched    = nA * NFUN;
+    // (per atom its functions: the analytic set with the spec's cutoff, or
+    // the atom data's functions, each with its own; the support is the
+    // largest of the atom's function supports)
+    const u32 nA = S.enrich ? S.nAtoms : 0;
+    std::vector<double> rsA(nA);
+    P->funOff.assign(nA + 1, 0);
+    P->funs.clear();
+    P->splines.clear();
+    std::map<std::pair<const dfx_atom *, u32>, u32> splAt; // shared by atoms of one file
+    for (u32 a = 0; a < nA; ++a)
+      {
+        u32 nf;
+        atomEnrichment(S, a, nf, rsA[a]);
+        P->funOff[a + 1] = P->funOff[a] + nf;
+        for (u32 k = 0; k < nf; ++k)
+          {
+            EnrFun E{};
+            for (int d = 0; d < 3; ++d)
+              E.r[d] = S.atomXYZ[3 * a + d];
+            E.Z = S.atomZ[a];
+            if (!S.atomData)
+              {
+                E.kind = (int)k;
+                E.rc   = S.rcut;
+                E.sm   = S.smooth;
+              }
+            else
+              {
+                const dfx_atom::Function &F = atomFunctions(S, a)[k];
+                E.kind = NFUN;
+                E.l    = F.q[1];
+                E.m    = F.q[2];
+                E.rc   = F.cutoff;
+                E.sm   = F.smoothness;
+                const auto key = std::make_pair(S.atomData[a], k);
+                auto       it  = splAt.find(key);
+                if (it == splAt.end())
+                  {
+                    it = splAt.emplace(key, (u32)P->splines.size()).first;
+                    for (const auto *v : {&S.atomData[a]->r, &F.y, &F.b, &F.c, &F.d})
+                      P->splines.insert(P->splines.end(), v->begin(), v->end());
+                  }
+                E.splOff = it->second;
+                E.splN   = (u32)F.y.size();
+              }
+            P->funs.push_back(E);
+          }
+      }
+    P->nEnriched    = P->funOff[nA];
     P->nDofs        = P->nClassical + P->nEnriched;
     P->atomCells.assign(nA, {});
     std::vector<std::vector<u32>> cellAtoms(P->nCells);
     for (u32 a = 0; a < nA; ++a)
       for (u32 c = 0; c < P->nCells; ++c)
         {
-          bool in = true;
+          const double rs = rsA[a];
+          bool         in = true;
           for (int d = 0; d < 3 && in; ++d)
             {
               const double lo = P->cellLo[3 * (size_t)c + d], hi = lo + P->cellH[3 * (size_t)c + d];
@@ -486,15 +567,17 @@ namespace
     P->matOff.assign(P->nCells + 1, 0);
     for (u32 c = 0; c < P->nCells; ++c)
       {
-        const u32 n = ncl + NFUN * (u32)cellAtoms[c].size();
+        u32 n = ncl;
+        for (u32 a : cellAtoms[c])
+          n += P->funOff[a + 1] - P->funOff[a];
         P->cellNdofs[c]     = n;
         P->mapOff[c + 1]    = P->mapOff[c] + n;
         P->matOff[c + 1]    = P->matOff[c] + (u64)n * n;
         P->cellMap.insert(P->cellMap.end(), cmapC.begin() + (size_t)c * ncl,
                           cmapC.begin() + (size_t)(c + 1) * ncl);
         for (u32 a : cellAtoms[c])
-          for (u32 k = 0; k < NFUN; ++k)
-            P->cellMap.push_back(P->nClassical + a * NFUN + k);
+          for (u32 k = P->funOff[a]; k < P->funOff[a + 1]; ++k)
+            P->cellMap.push_back(P->nClassical + k);
       }
 
     // GLL lumped mass, C2P (no hanging rows here), constrained -> 1, inverse,
@@ -567,6 +650,9 @@ namespace
     for (u32 a = 0; a < S.nAtoms; ++a)
       at[a] = {{S.atomXYZ[3 * a], S.atomXYZ[3 * a + 1], S.atomXYZ[3 * a + 2]}, S.atomZ[a], 0};
     upload(dAtoms, at, s);
+    DevBuf dFuns, dSpl;
+    upload(dFuns, P->funs, s);
+    upload(dSpl, P->splines, s);
 
     // group by (n, nq) so each group is one batched GEMM shape
     std::vector<u32> order(cellList);
@@ -619,7 +705,8 @@ namespace
         tabulateK<<<cnt, 256, 0, s>>>(dTc.as<TabCell>(), dMap.as<u32>(), P->nClassical,
                                       dL.as<double>(), dD.as<double>(), dX.as<double>(),
                                       dW.as<double>(), (int)np, dAtoms.as<AtomDev>(),
-                                      S.nAtoms, S.vSmear, S.rcut, S.smooth, alpha,
+                                      S.nAtoms, S.vSmear, dFuns.as<EnrFun>(),
+                                      dSpl.as<double>(), alpha,
                                       bF.as<double>(), bFW.as<double>(), bFV.as<double>(),
                                       bG.as<double>(), bGW.as<double>());
         checkLaunch("tabulate");
@@ -852,7 +939,7 @@ namespace
           {
             const u32 c = cells[i];
             u32       j = ncl;
-            while (P->cellMap[P->mapOff[c] + j] != P->nClassical + a * NFUN)
+            while (P->cellMap[P->mapOff[c] + j] != P->nClassical + P->funOff[a])
               ++j;
             colStart[i] = j;
           }
@@ -860,13 +947,14 @@ namespace
         upload(dCells, cells, s);
         upload(dCol, colStart, s);
         blocks.alloc((size_t)nc * ncl * ncl * sizeof(double));
-        rhsSlots.alloc((size_t)nc * ncl * NFUN * sizeof(double));
+        const u32 nf = P->funOff[a + 1] - P->funOff[a];
+        rhsSlots.alloc((size_t)nc * ncl * nf * sizeof(double));
         extractClassicalBlock<<<nc, 256, 0, s>>>(P->M.as<double>(), dRawOff.as<u64>(),
                                                  dNe.as<u32>(), dCells.as<u32>(), ncl,
                                                  blocks.as<double>());
         extractMixedBlock<<<nc, 256, 0, s>>>(P->M.as<double>(), dRawOff.as<u64>(),
                                              dNe.as<u32>(), dCells.as<u32>(), dCol.as<u32>(),
-                                             ncl, rhsSlots.as<double>());
+                                             ncl, nf, rhsSlots.as<double>());
         checkLaunch("extract");
         std::vector<double> hBlocks((size_t)nc * ncl * ncl);
         DFX_HIP(hipMemcpyAsync(hBlocks.data(), blocks.p, blocks.bytes, hipMemcpyDeviceToHost, s));
@@ -898,8 +986,8 @@ namespace
                                    cons, 0, &op),
             "orthogonalize mass operator");
         // rhs d = sum_e Q_e^T (mixed block), then zero the non-interior rows
-        chk(dfx_mv_wrap(ctx, rhsSlots.p, nc * ncl, NFUN, DFX_F64, &rs), "wrap");
-        chk(dfx_mv_create(ctx, nl, NFUN, DFX_F64, &rhs), "rhs");
+        chk(dfx_mv_wrap(ctx, rhsSlots.p, nc * ncl, nf, DFX_F64, &rs), "wrap");
+        chk(dfx_mv_create(ctx, nl, nf, DFX_F64, &rhs), "rhs");
         chk(dfx_op_scatter_add(op, rs, rhs), "rhs assembly");
         chk(dfx_constraints_set_constrained_nodes(cons, rhs, 0.0), "rhs rows");
         // Jacobi: assembled diagonal of the local mass
@@ -910,7 +998,7 @@ namespace
         for (u32 i = 0; i < nl; ++i)
           diag[i] = interior[i] ? 1.0 / diag[i] : 1.0;
         chk(dfx_op_create_diagonal(ctx, nl, diag.data(), 0, nullptr, nullptr, &pc), "pc");
-        chk(dfx_mv_create(ctx, nl, NFUN, DFX_F64, &x), "x");
+        chk(dfx_mv_create(ctx, nl, nf, DFX_F64, &x), "x");
         u32 iters = 0;
         int rc = dfx_cg_solve(ctx, op, pc, rhs, x, 1e-300, 1e-13, 1e10, 10000, &iters, nullptr);
         if (rc != 0)
@@ -922,7 +1010,7 @@ namespace
         upload(dGlob, glob, s);
         upload(dInt, interior, s);
         scatterAlpha<<<256, 256, 0, s>>>(static_cast<double *>(x->d), dGlob.as<u32>(),
-                                         dInt.as<u32>(), nl, a * NFUN, P->nClassical,
+                                         dInt.as<u32>(), nl, P->funOff[a], nf, P->nClassical,
                                          P->alpha.as<double>());
         checkLaunch("scatterAlpha");
         DFX_HIP(hipStreamSynchronize(s));
@@ -1033,6 +1121,27 @@ extern "C"
       P->atomZ.assign(spec->atomZ, spec->atomZ + spec->nAtoms);
       P->spec.atomXYZ = P->atomXYZ.data();
       P->spec.atomZ   = P->atomZ.data();
+      if (spec->enrich && spec->nAtoms && spec->atomData)
+        {
+          P->atomData.assign(spec->atomData, spec->atomData + spec->nAtoms);
+          P->enrichField      = spec->enrichField ? spec->enrichField : "orbital";
+          P->spec.atomData    = P->atomData.data();
+          P->spec.enrichField = P->enrichField.c_str();
+          for (u32 a = 0; a < spec->nAtoms; ++a)
+            {
+              DFX_REQUIRE(P->atomData[a], "atom data missing for an atom");
+              const auto &F = atomFunctions(P->spec, a);
+              DFX_REQUIRE(!F.empty(), "atom data with no enrichment function");
+              for (const auto &f : F)
+                if (f.q[1] > 1)
+                  throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
+            }
+        }
+      else
+        {
+          P->spec.atomData    = nullptr;
+          P->spec.enrichField = nullptr;
+        }
       buildHost(P.get());
       const u64 nm = P->matOff.back();
       P->H.alloc(nm * 8);

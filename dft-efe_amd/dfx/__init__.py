@@ -581,7 +581,8 @@ class _ProblemSpec(C.Structure):
                 ("hi", C.c_double * 3), ("nq", C.c_uint32), ("nqEnr", C.c_uint32),
                 ("nAtoms", C.c_uint32), ("atomXYZ", C.c_void_p), ("atomZ", C.c_void_p),
                 ("vSmear", C.c_double), ("enrich", C.c_int), ("rcut", C.c_double),
-                ("smooth", C.c_double), ("orthogonalize", C.c_int)]
+                ("smooth", C.c_double), ("orthogonalize", C.c_int),
+                ("atomData", C.c_void_p), ("enrichField", C.c_char_p)]
 
 
 _PROB_ARRAYS = {  # name: (id, dtype, size key)
@@ -703,11 +704,20 @@ class Problem:
 
     def __init__(self, ctx, p=3, n=(8, 8, 8), lo=(-10., -10., -10.), hi=(10., 10., 10.),
                  atoms=(), nq=0, nq_enr=0, v_smear=0.5, enrich=False, rcut=4.0,
-                 smooth=2.0, orthogonalize=True):
+                 smooth=2.0, orthogonalize=True, atom_data=None, enrich_field="orbital"):
+        """atom_data: per atom an AtomSphericalDataEnrichment whose field
+        `enrich_field` supplies that atom's enrichment functions (numerical
+        radial data, each with its own cutoff); None: the analytic Slater set
+        with cutoff rcut and smoothness smooth."""
         self.ctx = ctx
         xyz = np.ascontiguousarray([a[:3] for a in atoms], dtype=np.float64).reshape(-1)
         Z = np.ascontiguousarray([int(a[3]) for a in atoms], dtype=np.int32)
-        self._keep = (xyz, Z)
+        handles = None
+        if atom_data is not None:
+            if len(atom_data) != len(atoms):
+                raise ValueError("atom_data needs one entry per atom")
+            handles = (C.c_void_p * max(1, len(atoms)))(*[d.h.value for d in atom_data])
+        self._keep = (xyz, Z, handles, list(atom_data or ()))
         s = _ProblemSpec()
         s.p = p
         s.n = (C.c_uint32 * 3)(*[int(v) for v in n])
@@ -718,6 +728,8 @@ class Problem:
         s.atomZ = Z.ctypes.data if len(atoms) else None
         s.vSmear, s.enrich, s.rcut, s.smooth = v_smear, 1 if enrich else 0, rcut, smooth
         s.orthogonalize = 1 if orthogonalize else 0
+        s.atomData = C.cast(handles, C.c_void_p) if handles is not None else None
+        s.enrichField = enrich_field.encode() if handles is not None else None
         h = C.c_void_p()
         ctx.check(_lib.dfx_problem_build(ctx.h, C.byref(s), C.byref(h)))
         self.h = h

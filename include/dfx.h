@@ -390,7 +390,9 @@ int dfx_atom_destroy(dfx_atom *a);
  * 41-433; EFE enrichment, src/basis/EnrichmentClassicalInterfaceSpherical.h:76-127).
  * Numbering: cells lexicographic (x fastest); nodes numbered by first
  * appearance over cells, cell-local order tensor-lexicographic; enriched dof
- * of atom a, function k (1s, 2s, 2p_y, 2p_z, 2p_x) is nClassical + 5a + k.
+ * of atom a, function k is nClassical + (functions of atoms 0..a-1) + k, with
+ * the analytic set 1s, 2s, 2p_y, 2p_z, 2p_x (five per atom) or the atom data's
+ * functions in file order.
  * ==================================================================== */
 typedef struct dfx_problem_spec
 {
@@ -406,6 +408,12 @@ typedef struct dfx_problem_spec
   int           enrich;       /* 1s, 2s, 2p Slater functions per atom */
   double        rcut, smooth; /* smooth cutoff, support ends at rcut (1 + 1/smooth) */
   int           orthogonalize;/* L2-orthogonalize enrichment against the classical space */
+  /* numerical enrichment (enrich != 0): atom a carries the functions of field
+   * enrichField ("orbital" when NULL) of atomData[a] (AtomSphericalDataContainer,
+   * src/atoms/AtomSphericalDataContainer.h:41-98), each with its own cutoff;
+   * NULL: the analytic Slater set above.  The handles must outlive the build. */
+  const dfx_atom *const *atomData;
+  const char   *enrichField;
 } dfx_problem_spec;
 
 #define DFX_PROB_CELL_NDOFS 0  /* uint32 [nCells] */

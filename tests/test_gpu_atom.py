@@ -72,3 +72,80 @@ def test_d_shell_is_refused(ctx, tmp_path):
     a = dfx.AtomSphericalDataEnrichment(path, ["orbital"])
     with pytest.raises(dfx.DfxError, match="l > 1"):
         a.getSphericalData("orbital")[0].evaluate(ctx, np.zeros((4, 3)))
+
+
+# ------------------------------------------------ problem builder with atom files
+def _slater_file(tmp_path, Z, rcut, smooth):
+    """The analytic Slater set (1s: e^-Zr, 2s: r e^-Zr, 2p: r e^-Zr, m = -1, 0, 1)
+    written as an atom file on a fine grid."""
+    r = np.linspace(0.0, 16.0, 8001)
+    e = np.exp(-Z * r)
+    funcs = [([1, 0, 0], (rcut, smooth), e), ([2, 0, 0], (rcut, smooth), r * e)] + \
+        [([2, 1, m], (rcut, smooth), r * e) for m in (-1, 0, 1)]
+    return write_atom(tmp_path / f"slater{Z}.xml", r, funcs)
+
+
+@pytest.mark.parametrize("orth", [False, True])
+def test_atom_file_enrichment_reproduces_the_analytic_set(ctx, tmp_path, orth):
+    """An atom file holding the analytic Slater functions gives the problem the
+    oracle-pinned analytic builder gives: the same maps bit for bit and the
+    same cell matrices up to the spline's interpolation error."""
+    import dfx
+    atoms = [(-1.5, 0.5, 0.2, 1), (2.0, -1.0, 0.7, 3)]
+    kw = dict(p=2, n=(6, 6, 6), lo=(-12.,) * 3, hi=(12.,) * 3, atoms=atoms, enrich=True,
+              nq_enr=6, rcut=4.0, smooth=2.0, orthogonalize=orth)
+    files = {Z: dfx.AtomSphericalDataEnrichment(_slater_file(tmp_path, Z, 4.0, 2.0),
+                                                ["orbital"]) for Z in (1, 3)}
+    Pa = dfx.Problem(ctx, **kw)
+    Pn = dfx.Problem(ctx, atom_data=[files[a[3]] for a in atoms], **kw)
+    assert (Pn.nDofs, Pn.nEnriched) == (Pa.nDofs, Pa.nEnriched)
+    assert Pn.nEnriched == 10
+    for k in ("cellNdofs", "cellMap", "rows"):
+        assert np.array_equal(Pn.get(k), Pa.get(k))
+    for k in ("H", "M", "K"):
+        a, b = Pa.get(k), Pn.get(k)
+        err = np.abs(a - b).max() / np.abs(a).max()
+        print(f"orth={orth} {k}: max rel diff {err:.2e}", flush=True)
+        assert err < 1e-7
+    if orth:
+        a, b = Pa.get("alpha"), Pn.get("alpha")
+        assert np.abs(a - b).max() < 1e-7 * np.abs(a).max()
+
+
+def _free_dense(P):
+    """Dense H and M over the free dofs (Dirichlet rows removed)."""
+    cm, nd = P.get("cellMap"), P.get("cellNdofs")
+    H, M = np.zeros((P.nDofs, P.nDofs)), np.zeros((P.nDofs, P.nDofs))
+    hs, ms = P.get("H"), P.get("M")
+    off = moff = 0
+    for n in nd:
+        ids = cm[off:off + n]
+        H[np.ix_(ids, ids)] += hs[moff:moff + n * n].reshape(n, n, order="F")
+        M[np.ix_(ids, ids)] += ms[moff:moff + n * n].reshape(n, n, order="F")
+        off += n
+        moff += n * n
+    free = np.setdiff1d(np.arange(P.nDofs), P.get("rows"))
+    return H[np.ix_(free, free)], M[np.ix_(free, free)]
+
+
+def test_hydrogen_file_enrichment_lowers_the_ground_state(ctx):
+    """Rayleigh-Ritz: adding the H.xml 1s orbital (cutoff 2, smoothness 0.6) to
+    a coarse classical space can only lower the lowest eigenvalue of the
+    smeared-Coulomb Hamiltonian (every cell on the same 8-point rule, so both
+    discretizations share the bilinear forms and the spaces are nested)."""
+    import dfx
+    import scipy.linalg as sl
+    Hf = dfx.AtomSphericalDataEnrichment(os.path.join(GOLD, "H.xml"), ["orbital"])
+    kw = dict(p=2, n=(5, 5, 5), lo=(-10.,) * 3, hi=(10.,) * 3, atoms=[(0.1, -0.2, 0.3, 1)],
+              v_smear=0.2, nq=8, nq_enr=8)
+    Pc = dfx.Problem(ctx, **kw)
+    Pe = dfx.Problem(ctx, enrich=True, atom_data=[Hf], **kw)
+    assert Pe.nEnriched == 1 and Pe.nClassical == Pc.nClassical
+    ev = []
+    for P in (Pc, Pe):
+        H, M = _free_dense(P)
+        ev.append(sl.eigh(H, M, eigvals_only=True, subset_by_index=[0, 2]))
+    print(f"lowest eigenvalues classical {ev[0]} enriched {ev[1]}", flush=True)
+    assert np.all(ev[1] <= ev[0] + 1e-10)
+    assert ev[0][0] - ev[1][0] > 0.01          # the orbital carries the cusp region
+    assert -0.5 < ev[1][0] < -0.3              # above -1/2 Ha (smeared nucleus)
