@@ -829,6 +829,22 @@ def _request_struct(r, keep):
     s.enrich = 1 if r.get("enrich", True) else 0
     s.rcut, s.smooth = float(r.get("rcut", 4.0)), float(r.get("smooth", 2.0))
     s.orthogonalize = 1 if r.get("orthogonalize", True) else 0
+    files = r.get("atom_files")
+    if files:
+        # one atom file per element symbol or per atom; the handles live in
+        # `keep` for the duration of the call
+        per = [files[a[4]] if len(a) > 4 else files[i] for i, a in enumerate(atoms)] \
+            if isinstance(files, dict) else list(files)
+        if len(per) != len(atoms):
+            raise ValueError("atom_files: one file per atom (or per symbol)")
+        cache = {}
+        data = [cache.setdefault(f, AtomSphericalDataEnrichment(f, [r.get("enrich_field", "orbital")]))
+                for f in per]
+        handles = (C.c_void_p * len(atoms))(*[d.h.value for d in data])
+        field = r.get("enrich_field", "orbital").encode()
+        keep.append((data, handles, field))
+        s.atomData = C.cast(handles, C.c_void_p)
+        s.enrichField = field
     q.numStates = int(r["states"])
     q.tolerance = float(r.get("tol", 1e-6))
     q.numElectrons = float(r["electrons"])
@@ -836,6 +852,85 @@ def _request_struct(r, keep):
     q.guessSeed = int(r.get("seed", 1))
     q.arrivalSeconds = float(r.get("arrival", 0.0))
     return q
+
+
+# ------------------------------------------------------------ wire format
+# One JSON object per request and per response (JSON lines).
+REQUEST_SCHEMA = {
+    # key: (type, required, meaning)
+    "id": (str, True, "caller's request id, echoed in the response"),
+    "p": (int, True, "FE order"),
+    "n": (list, True, "cells per direction [nx, ny, nz]"),
+    "lo": (list, True, "domain lower corner [x, y, z] (bohr)"),
+    "hi": (list, True, "domain upper corner"),
+    "atoms": (list, True, "[[x, y, z, Z], ...] or [[x, y, z, Z, symbol], ...]"),
+    "states": (int, True, "eigenpairs wanted"),
+    "electrons": (float, True, "electron count (occupations, convergence set)"),
+    "tol": (float, False, "residual tolerance of the occupied states, default 1e-6"),
+    "max_passes": (int, False, "ChFSI pass limit, default 50"),
+    "seed": (int, False, "initial-subspace seed, default 1"),
+    "arrival": (float, False, "arrival time (s) in the stream, default 0"),
+    "enrich": (bool, False, "enrichment on, default true"),
+    "rcut": (float, False, "analytic enrichment cutoff, default 4"),
+    "smooth": (float, False, "analytic enrichment smoothness, default 2"),
+    "v_smear": (float, False, "nuclear smearing radius, default 0.5"),
+    "orthogonalize": (bool, False, "orthogonalized enrichment, default true"),
+    "atom_files": ((dict, list), False, "atom XML file per symbol (dict) or per atom (list)"),
+    "enrich_field": (str, False, "field of the atom files, default \"orbital\""),
+}
+RESPONSE_KEYS = ("id", "status", "passes", "eigenvalues", "arrival", "admit_seconds",
+                 "done_seconds", "latency_seconds", "footprint_bytes", "slot")
+
+
+def parse_request(line):
+    """A request dict from one JSON line, checked against REQUEST_SCHEMA
+    (ValueError names the offending key)."""
+    import json
+    r = json.loads(line) if isinstance(line, (str, bytes)) else dict(line)
+    if not isinstance(r, dict):
+        raise ValueError("a request is a JSON object")
+    for k in r:
+        if k not in REQUEST_SCHEMA:
+            raise ValueError(f"unknown request key {k!r}")
+    for k, (t, req, _) in REQUEST_SCHEMA.items():
+        if k not in r:
+            if req:
+                raise ValueError(f"missing request key {k!r}")
+            continue
+        v = r[k]
+        ok = isinstance(v, t) and not (t is int and isinstance(v, bool))
+        if t is float:
+            ok = isinstance(v, (int, float)) and not isinstance(v, bool)
+        if not ok:
+            raise ValueError(f"request key {k!r}: expected {t}")
+    for k, m in (("n", 3), ("lo", 3), ("hi", 3)):
+        if len(r[k]) != m:
+            raise ValueError(f"request key {k!r}: expected {m} values")
+    for a in r["atoms"]:
+        if not (isinstance(a, list) and len(a) in (4, 5)):
+            raise ValueError("request key 'atoms': entries [x, y, z, Z] or [x, y, z, Z, symbol]")
+    if isinstance(r.get("atom_files"), dict) and any(len(a) != 5 for a in r["atoms"]):
+        raise ValueError("atom_files by symbol needs [x, y, z, Z, symbol] atoms")
+    return r
+
+
+def serve_json(lines, **kw):
+    """Serve JSON-line requests; returns one JSON line per request, in input
+    order: id, status (0 or the library / solver code), passes, eigenvalues,
+    arrival, admit / done / latency seconds, footprint bytes, slot."""
+    import json
+    reqs = [parse_request(l) for l in lines if (l.strip() if isinstance(l, str) else l)]
+    energies, res = serve(reqs, **kw)
+    out = []
+    for r, e, x in zip(reqs, energies, res):
+        arr = float(r.get("arrival", 0.0))
+        out.append(json.dumps({
+            "id": r["id"], "status": int(x["status"]), "passes": int(x["passes"]),
+            "eigenvalues": [float(v) for v in e], "arrival": arr,
+            "admit_seconds": float(x["admitSeconds"]), "done_seconds": float(x["doneSeconds"]),
+            "latency_seconds": float(x["doneSeconds"]) - arr,
+            "footprint_bytes": int(x["footprintBytes"]), "slot": int(x["slot"])}))
+    return out
 
 
 def request_footprint(r):

@@ -62,3 +62,29 @@ def test_graph_replay_gives_the_eager_results():
         assert ra["status"] == 0 and rb["status"] == 0
         assert ra["passes"] == rb["passes"] and ra["passes"] > 2
         assert np.array_equal(a, b)
+
+
+def test_json_requests_with_atom_files_alone_and_in_the_mix():
+    """JSON-line requests, one of them enriched from the reference's H.xml:
+    responses come back in input order with the eigenvalues of serve() on the
+    same dicts, and the atom-file request gives the same bits alone and in
+    the mix."""
+    import json
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "atoms", "H.xml")
+    base = dict(p=3, n=[5, 5, 5], lo=[-8.0] * 3, hi=[8.0] * 3, states=6, electrons=2.0,
+                tol=1e-6, max_passes=100)
+    reqs = [dict(base, id="analytic", atoms=[[-0.7, 0.1, 0.0, 1], [0.7, -0.1, 0.0, 1]],
+                 seed=3, arrival=0.0),
+            dict(base, id="h-file", atoms=[[-0.7, 0.1, 0.0, 1, "H"], [0.7, -0.1, 0.0, 1, "H"]],
+                 atom_files={"H": gold}, seed=4, arrival=0.01),
+            dict(base, id="third", n=[4, 4, 4], atoms=[[0.3, 0.2, -0.1, 6]], electrons=6.0,
+                 seed=5, arrival=0.02)]
+    out = [json.loads(l) for l in dfx.serve_json([json.dumps(r) for r in reqs], max_slots=3)]
+    assert [o["id"] for o in out] == ["analytic", "h-file", "third"]
+    assert all(o["status"] == 0 and o["latency_seconds"] >= 0 for o in out)
+    alone = json.loads(dfx.serve_json([json.dumps(dict(reqs[1], arrival=0.0))], max_slots=1)[0])
+    assert alone["eigenvalues"] == out[1]["eigenvalues"]      # bit for bit (JSON floats round-trip)
+    assert alone["passes"] == out[1]["passes"]
+    # the atom-file request is a different basis from the analytic one
+    assert out[1]["eigenvalues"] != out[0]["eigenvalues"]
