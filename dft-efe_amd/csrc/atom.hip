@@ -22,6 +22,7 @@
 #include <fstream>
 #include <set>
 #include <sstream>
+#include <type_traits>
 
 #include "api_util.hpp"
 #include "atom.hpp"
@@ -274,10 +275,12 @@ namespace
     return true;
   }
 
-  // C^2 cubic spline through (x_i, y_i) with f'(x_0) = f'(x_{n-1}) = 0:
+  // C^2 cubic spline through (x_i, y_i) with a first- or second-derivative
+  // condition at each end (utils::Spline's bd_type; default: slopes 0):
   // c_i = f''(x_i) / 2 from the tridiagonal system (Thomas algorithm; the
   // matrix is strictly diagonally dominant), then b_i and d_i per interval.
-  void clampedSpline(const std::vector<double> &x, dfx_atom::Function &F)
+  void cubicSpline(const std::vector<double> &x, dfx_atom::Function &F, bool leftSecond = false,
+                   double leftValue = 0.0, bool rightSecond = false, double rightValue = 0.0)
   {
     const size_t n = x.size();
     const auto  &y = F.y;
@@ -297,7 +300,22 @@ namespace
         di[i + 1] += 2.0 * h;
         rhs[i + 1] -= 3.0 * dy;
       }
-    // boundary slopes are zero: no further terms on rows 0 and n-1
+    if (leftSecond) // f''(x_0) given: c_0 = value / 2
+      {
+        di[0]  = 1.0;
+        up[0]  = 0.0;
+        rhs[0] = 0.5 * leftValue;
+      }
+    else // f'(x_0) given
+      rhs[0] -= 3.0 * leftValue;
+    if (rightSecond)
+      {
+        lo[n - 1]  = 0.0;
+        di[n - 1]  = 1.0;
+        rhs[n - 1] = 0.5 * rightValue;
+      }
+    else
+      rhs[n - 1] += 3.0 * rightValue;
     for (size_t i = 1; i < n; ++i)
       {
         const double w = lo[i] / di[i - 1];
@@ -318,19 +336,41 @@ namespace
     const double h = x[n - 1] - x[n - 2];
     F.b[n - 1]     = (3.0 * F.d[n - 2] * h + 2.0 * F.c[n - 2]) * h + F.b[n - 2];
     F.d[n - 1]     = 0.0;
+    // left extrapolation: linear after a slope condition, else quadratic
+    F.cLeft = leftSecond ? F.c[0] : 0.0;
   }
 
   // f(x) = R(r) Y_lm(x / r) f_c(r) and its gradient at every point (rows of
   // P, 3 values each) relative to the origin; l <= 1
-  __global__ void atomEvalK(SplineDev S, int l, int m, double rc, double sm, double ox, double oy,
-                            double oz, const double *P, u32 nPts, double *val, double *grad)
+  __global__ void atomEvalK(SplineDev S, int l, int m, double rc, double sm, int mixed,
+                            double tail, double ox, double oy, double oz, const double *P,
+                            u32 nPts, double *val, double *grad)
   {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPts)
       return;
     const double dx[3] = {P[3 * (u64)i] - ox, P[3 * (u64)i + 1] - oy, P[3 * (u64)i + 2] - oz};
     double       f, g[3];
-    radialYlm(S, l, m, rc, sm, dx, f, g);
+    if (mixed)
+      {
+        // SphericalDataMixed (src/atoms/SphericalDataMixed.cpp:43-69): the
+        // spline up to the last knot, tail / r beyond, times Y_00
+        const double y00 = 0.28209479177387814;
+        const double r   = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        double       R, dR;
+        if (r <= S.x[S.n - 1])
+          splineEval(S, r, R, dR);
+        else
+          {
+            R  = tail / r;
+            dR = -tail / (r * r);
+          }
+        f = y00 * R;
+        for (int k = 0; k < 3; ++k)
+          g[k] = r > 0.0 ? y00 * dR * dx[k] / r : 0.0;
+      }
+    else
+      radialYlm(S, l, m, rc, sm, dx, f, g);
     val[i] = f;
     if (grad)
       {
@@ -468,7 +508,7 @@ int dfx_atom_read(const char *path, const char *const *fieldNames, uint32_t nFie
             F.q[0] = q[0], F.q[1] = q[1], F.q[2] = q[2];
             F.cutoff     = co[0];
             F.smoothness = co[1];
-            clampedSpline(A->r, F);
+            cubicSpline(A->r, F);
             fun.push_back(std::move(F));
           }
         if (qset.size() != fun.size())
@@ -487,6 +527,327 @@ int dfx_atom_read(const char *path, const char *const *fieldNames, uint32_t nFie
         A->fields[fname] = std::move(fun);
       }
     *out = A.release();
+    return 0;
+  });
+}
+
+// Pseudopotential files, UPF v2 (AtomSphericalDataPSP, src/atoms/
+// AtomSphericalDataPSP.cpp:291-830): PP_HEADER attributes as metadata (with
+// its checks of mesh_size, z_valence, l_max, number_of_proj), the mesh
+// PP_MESH/PP_R, and per field (Defaults.cpp:41-47: vlocal -> PP_LOCAL, beta
+// -> PP_NONLOCAL/PP_BETA.*, pswfc -> PP_PSWFC/PP_CHI.*, nlcc -> PP_NLCC,
+// rhoatom -> PP_RHOATOM) the radial data with the reference's transforms:
+// Ry -> Ha for vlocal and D_ij, beta and pswfc divided by r, rhoatom by
+// 4 pi r^2 (first value copied from the second), everything but beta and
+// pswfc divided by Y_00; support from the last |value| >= 1e-10 (rhoatom: the
+// whole mesh), smoothness 1e10; vlocal is mixed: the spline of the points
+// within 10.0001 bohr (f'' at the first point = the first divided
+// difference, f' at the last = -v / r) and -|z_valence| / (Y_00 r) beyond.
+int dfx_atom_read_upf(const char *path, const char *const *fieldNames, uint32_t nFields,
+                      dfx_atom **out)
+{
+  return guard(nullptr, [&] {
+    DFX_REQUIRE(path && out && (nFields == 0 || fieldNames), "null argument");
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+      throw Error(DFX_E_ARG, std::string("Error parsing the XML file:") + path +
+                               " (cannot open)");
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    const std::string file(path);
+    XParser           xp{text, file};
+    xp.misc();
+    XNode root = xp.element();
+    xp.misc();
+    if (xp.i != text.size())
+      xp.fail("content after the root element");
+    if (localName(root.name) != "UPF")
+      throw Error(DFX_E_ARG, "XPath expression /UPF not found in " + file);
+    auto A  = std::make_unique<dfx_atom>();
+    A->file = file;
+    auto children = [&](const XNode &n, const char *name) {
+      std::vector<const XNode *> v;
+      for (const XNode &c : n.kids)
+        if (c.name == name)
+          v.push_back(&c);
+      return v;
+    };
+    auto single = [&](const XNode &n, const char *name, const std::string &xpath) {
+      const auto v = children(n, name);
+      if (v.empty())
+        throw Error(DFX_E_ARG, "XPath expression " + xpath + " not found.");
+      if (v.size() != 1)
+        throw Error(DFX_E_ARG, "Found more than one " + xpath + " element in " + file);
+      return v[0];
+    };
+    // header
+    const XNode *hdr = single(root, "PP_HEADER", "/UPF/PP_HEADER");
+    int    meshSize = 0, lmax = 0, nProj = 0;
+    double zv       = 0.0;
+    bool   coreCorr = false;
+    auto   num = [&](const std::string &v, const char *what, auto &dst) {
+      std::vector<double> d;
+      if (!toDoubles(v, d) || d.size() != 1)
+        throw Error(DFX_E_ARG, std::string("Error while converting /UPF/PP_HEADER element in ") +
+                                 file + " (" + what + ")");
+      dst = (std::remove_reference_t<decltype(dst)>)d[0];
+      if ((double)dst != d[0])
+        throw Error(DFX_E_ARG, std::string("Error while converting /UPF/PP_HEADER element in ") +
+                                 file + " to int (" + what + ")");
+    };
+    for (const auto &at : hdr->attrs)
+      {
+        const std::string v = trim(at.second);
+        A->text[at.first].push_back(v);
+        if (at.first == "mesh_size")
+          {
+            num(v, "mesh_size", meshSize);
+            if (meshSize <= 0)
+              throw Error(DFX_E_ARG, "Non-positive numRadial points found for/UPF/PP_HEADER "
+                                     "element in " + file);
+          }
+        else if (at.first == "z_valence")
+          {
+            num(v, "z_valence", zv);
+            if (!(zv > 0))
+              throw Error(DFX_E_ARG, "Non-positive z valance found for/UPF/PP_HEADER element in " +
+                                       file);
+          }
+        else if (at.first == "l_max")
+          {
+            num(v, "l_max", lmax);
+            if (lmax < 0)
+              throw Error(DFX_E_ARG, "Non-positive lmax found for/UPF/PP_HEADER element in " + file);
+          }
+        else if (at.first == "number_of_proj")
+          {
+            num(v, "number_of_proj", nProj);
+            if (nProj < 0)
+              throw Error(DFX_E_ARG, "Non-positive lmax found for/UPF/PP_HEADER element in " + file);
+          }
+        else if (at.first == "core_correction")
+          {
+            std::string b;
+            for (char ch : v)
+              b += (char)tolower((unsigned char)ch);
+            if (b == "t" || b == "true" || b == ".true." || b == "1")
+              coreCorr = true;
+            else if (b == "f" || b == "false" || b == ".false." || b == "0")
+              coreCorr = false;
+            else
+              throw Error(DFX_E_ARG, "Error while converting core_correction in " + file);
+          }
+      }
+    DFX_REQUIRE(meshSize > 0, "no mesh_size in /UPF/PP_HEADER of " + file);
+    // mesh
+    const XNode *mesh = single(root, "PP_MESH", "/UPF/PP_MESH");
+    const XNode *pr   = single(*mesh, "PP_R", "/UPF/PP_MESH/PP_R");
+    if (!toDoubles(pr->text, A->r))
+      throw Error(DFX_E_ARG, "Error while converting /UPF/PP_MESH/PP_R element in " + file +
+                               " to double");
+    if (A->r.size() != (size_t)meshSize)
+      throw Error(DFX_E_ARG, "Mismatch in number of radial points specified and the number of "
+                             " radial points provided in " + file);
+    DFX_REQUIRE(meshSize >= 2, "the radial mesh needs at least two points: " + file);
+    for (size_t i = 0; i + 1 < A->r.size(); ++i)
+      DFX_REQUIRE(A->r[i + 1] > A->r[i], "the radial mesh is not strictly increasing: " + file);
+    const std::vector<double> &r = A->r;
+    // Y_00 = C_00 D_0 Q_0 (SphericalHarmonicFunctions.cpp:378-430)
+    const double invY00 = 1.0 / (std::sqrt(1.0 / 2.0) * (1.0 / std::sqrt(2 * M_PI)) * 1.0);
+    static const std::map<std::string, std::string> elementOf = {
+      {"vlocal", "PP_LOCAL"}, {"beta", "PP_NONLOCAL"}, {"pswfc", "PP_PSWFC"},
+      {"nlcc", "PP_NLCC"},   {"rhoatom", "PP_RHOATOM"}};
+    for (uint32_t fi = 0; fi < nFields; ++fi)
+      {
+        DFX_REQUIRE(fieldNames[fi], "null field name");
+        const std::string fname = fieldNames[fi];
+        const auto        el    = elementOf.find(fname);
+        if (el == elementOf.end())
+          throw Error(DFX_E_ARG, "No element name '" + fname +
+                                   "' found in UPF_PARENT_PATH_LOOKUP. The fieldNames vector "
+                                   "for UPF are standarized by authors.");
+        const std::string xpath = "/UPF/" + el->second;
+        std::vector<std::pair<const std::string *, std::vector<int>>> entries;
+        std::vector<std::vector<int>>                                 seen;
+        auto add = [&](const std::string *t, std::vector<int> q) {
+          if (std::find(seen.begin(), seen.end(), q) != seen.end())
+            throw Error(DFX_E_ARG, "Trying to enter more than one " + xpath +
+                                     " of same quantum number vector from " + file);
+          seen.push_back(q);
+          entries.emplace_back(t, q);
+        };
+        auto attr = [](const XNode &n, const char *name) -> const std::string * {
+          for (const auto &a : n.attrs)
+            if (a.first == name)
+              return &a.second;
+          return nullptr;
+        };
+        static const std::string zeros;
+        std::string zeroText;
+        if (fname == "beta")
+          {
+            const XNode *nl = single(root, "PP_NONLOCAL", xpath);
+            int l = 0, p = 0, lPrev = 0, lMaxBeta = 0;
+            for (size_t i = 0; i < nl->kids.size(); ++i)
+              {
+                const XNode &c = nl->kids[i];
+                if (c.name.find("PP_BETA") != std::string::npos)
+                  {
+                    const std::string *am = attr(c, "angular_momentum");
+                    std::vector<int>   li;
+                    if (!am || !toInts(*am, li) || li.size() != 1)
+                      throw Error(DFX_E_ARG, "Error while converting angular_momentum in " +
+                                               xpath + " element in " + file);
+                    l = li[0];
+                    if (l < 0)
+                      throw Error(DFX_E_ARG, "Non-positive l found for" + xpath +
+                                               " element in " + file);
+                    // projector index within a run of equal l (the reference's
+                    // counter: lPrev is first set at the second child)
+                    if (i != 0)
+                      {
+                        p     = (l != lPrev) ? 0 : p + 1;
+                        lPrev = l;
+                      }
+                    for (int m = -l; m <= l; ++m)
+                      add(&c.text, {p, l, m});
+                    lMaxBeta = std::max(lMaxBeta, l);
+                  }
+                else if (c.name.find("PP_DIJ") != std::string::npos)
+                  {
+                    std::vector<double> dij;
+                    if (!toDoubles(c.text, dij) || dij.size() != (size_t)nProj * nProj)
+                      throw Error(DFX_E_ARG, "Error while converting values in " + xpath +
+                                               " element in " + file + " to double");
+                    std::string out;
+                    char        buf[64];
+                    for (double v : dij)
+                      {
+                        snprintf(buf, sizeof buf, "%.16f ", v * 0.5); // Ry -> Ha
+                        out += buf;
+                      }
+                    A->text["dij"] = {out};
+                  }
+              }
+            if (lMaxBeta != lmax)
+              throw Error(DFX_E_ARG, "The lmax from PSP HEADER does not match from" + xpath +
+                                       " element in " + file);
+          }
+        else if (fname == "pswfc")
+          {
+            const XNode *pw = single(root, "PP_PSWFC", xpath);
+            for (const XNode &c : pw->kids)
+              if (c.name.find("PP_CHI") != std::string::npos)
+                {
+                  const std::string *ls = attr(c, "l"), *lab = attr(c, "label");
+                  std::vector<int>   li;
+                  if (!ls || !toInts(*ls, li) || li.size() != 1 || li[0] < 0 || !lab ||
+                      lab->empty() || !isdigit((unsigned char)(*lab)[0]))
+                    throw Error(DFX_E_ARG, "Error while converting l / label in " + xpath +
+                                             " element in " + file);
+                  const int l = li[0], n = (*lab)[0] - '0';
+                  for (int m = -l; m <= l; ++m)
+                    add(&c.text, {n, l, m});
+                }
+          }
+        else if (fname == "nlcc" && !coreCorr)
+          {
+            for (int i = 0; i < meshSize; ++i)
+              zeroText += i ? " 0" : "0";
+            add(&zeroText, {0, 0, 0});
+          }
+        else
+          add(&single(root, el->second.c_str(), xpath)->text, {0, 0, 0});
+
+        std::vector<dfx_atom::Function> fun;
+        for (const auto &e : entries)
+          {
+            dfx_atom::Function F;
+            if (!toDoubles(*e.first, F.y))
+              throw Error(DFX_E_ARG, "Error while converting values in " + xpath + " element in " +
+                                       file + " to double");
+            if (F.y.size() != (size_t)meshSize)
+              throw Error(DFX_E_ARG, "Mismatch in number of points specified and number of points "
+                                     "provided in " + xpath + " element in " + file);
+            auto &v = F.y;
+            if (fname == "vlocal")
+              for (double &x : v)
+                x *= 0.5;
+            else if (fname == "beta" || fname == "pswfc")
+              {
+                for (size_t i = 1; i < v.size(); ++i)
+                  v[i] = v[i] / r[i];
+                v[0] = v[1];
+              }
+            else if (fname == "rhoatom")
+              {
+                for (size_t i = 1; i < v.size(); ++i)
+                  v[i] = v[i] / (4 * M_PI * r[i] * r[i]);
+                v[0] = v[1];
+              }
+            double cutoff = 1.0e10;
+            for (size_t j = v.size() - 1; j > 0; --j)
+              if (std::abs(v[j]) >= 1e-10)
+                {
+                  cutoff = j != v.size() - 1 ? r[j] : 1.0e10;
+                  break;
+                }
+            if (fname == "rhoatom")
+              cutoff = r.back();
+            if (fname != "beta" && fname != "pswfc")
+              for (double &x : v)
+                x *= invY00;
+            std::copy(e.second.begin(), e.second.end(), F.q);
+            F.cutoff     = cutoff;
+            F.smoothness = 1.0e10;
+            if (fname == "vlocal")
+              {
+                std::vector<double> xs, ys;
+                for (size_t i = 0; i < v.size(); ++i)
+                  if (r[i] <= 10.0001)
+                    {
+                      xs.push_back(r[i]);
+                      ys.push_back(v[i]);
+                    }
+                DFX_REQUIRE(xs.size() >= 2, "fewer than two vlocal points within 10 bohr: " + file);
+                const double left  = (ys[1] - ys[0]) / (xs[1] - xs[0]);
+                const double right = -1.0 * (ys.back() / xs.back());
+                F.x                = xs;
+                F.y                = ys;
+                F.mixed            = true;
+                F.tail             = -1.0 * invY00 * std::abs(zv);
+                cubicSpline(F.x, F, true, left, false, right);
+              }
+            else
+              cubicSpline(r, F);
+            fun.push_back(std::move(F));
+          }
+        A->fields[fname] = std::move(fun);
+      }
+    if (!A->text.count("dij"))
+      A->text["dij"] = {""};
+    *out = A.release();
+    return 0;
+  });
+}
+
+int dfx_atom_function_knots(const dfx_atom *a, const char *field, uint32_t k, uint32_t *n,
+                            double *knots, int *mixed, double *tail, double *cLeft)
+{
+  return guard(nullptr, [&] {
+    const dfx_atom::Function &F  = functionOf(a, field, k);
+    const std::vector<double> &x = F.knots(a->r);
+    if (n)
+      *n = (uint32_t)x.size();
+    if (knots)
+      std::copy(x.begin(), x.end(), knots);
+    if (mixed)
+      *mixed = F.mixed ? 1 : 0;
+    if (tail)
+      *tail = F.tail;
+    if (cLeft)
+      *cLeft = F.cLeft;
     return 0;
   });
 }
@@ -575,20 +936,22 @@ int dfx_atom_evaluate(dfx_ctx *ctx, const dfx_atom *a, const char *field, uint32
                 "gradients: FP64, 3 columns, one row per point");
     if (F.q[1] > 1)
       throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
-    const size_t n  = a->r.size();
+    const std::vector<double> &xs = F.knots(a->r);
+    const size_t n  = xs.size();
     DevBuf      &sb = work(ctx, "atom.spline", 5 * n * sizeof(double));
     double      *d  = sb.as<double>();
     hipStream_t  s  = ctx->stream;
-    const std::vector<double> *parts[5] = {&a->r, &F.y, &F.b, &F.c, &F.d};
+    const std::vector<double> *parts[5] = {&xs, &F.y, &F.b, &F.c, &F.d};
     for (int p = 0; p < 5; ++p)
       DFX_HIP(hipMemcpyAsync(d + p * n, parts[p]->data(), n * sizeof(double),
                              hipMemcpyHostToDevice, s));
-    const SplineDev S{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, (u32)n};
+    const SplineDev S{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, (u32)n, F.cLeft};
     const u32       nPts = points->N;
     if (nPts)
       {
         atomEvalK<<<(nPts + 255) / 256, 256, 0, s>>>(
-          S, F.q[1], F.q[2], F.cutoff, F.smoothness, origin[0], origin[1], origin[2],
+          S, F.q[1], F.q[2], F.cutoff, F.smoothness, F.mixed ? 1 : 0, F.tail, origin[0],
+          origin[1], origin[2],
           static_cast<const double *>(points->d), nPts, static_cast<double *>(values->d),
           gradients ? static_cast<double *>(gradients->d) : nullptr);
         checkLaunch("atomEval");

@@ -14,7 +14,17 @@ struct dfx_atom
   {
     int                 q[3]; // n, l, m
     double              cutoff, smoothness;
-    std::vector<double> y, b, c, d; // spline: y + b h + c h^2 + d h^3 on [r_i, r_i+1)
+    std::vector<double> x;          // knots (empty: the file's radial grid)
+    std::vector<double> y, b, c, d; // spline: y + b h + c h^2 + d h^3 on [x_i, x_i+1)
+    double              cLeft = 0;  // left of x_0: y_0 + b_0 h + cLeft h^2
+    // mixed (UPF local potential): Y00 (spline(r) for r <= x_last, tail / r
+    // beyond), no cutoff function
+    bool                mixed = false;
+    double              tail  = 0;
+    const std::vector<double> &knots(const std::vector<double> &grid) const
+    {
+      return x.empty() ? grid : x;
+    }
   };
   std::string                                     file;
   std::map<std::string, std::vector<std::string>> text; // direct children of <atom>
@@ -29,6 +39,7 @@ namespace dfx
   {
     const double *x, *y, *b, *c, *d;
     u32           n;
+    double        cl = 0.0; // left extrapolation y_0 + b_0 h + cl h^2
   };
 
   // value and slope at r (left of x_0 linear with slope b_0, right of
@@ -38,8 +49,9 @@ namespace dfx
   {
     if (!(r >= S.x[0])) // also NaN
       {
-        v  = S.b[0] * (r - S.x[0]) + S.y[0];
-        dv = S.b[0];
+        const double h = r - S.x[0];
+        v              = (S.cl * h + S.b[0]) * h + S.y[0];
+        dv             = 2.0 * S.cl * h + S.b[0];
         return;
       }
     u32 lo = 0, hi = S.n; // x[lo] <= r < x[hi] (x[n] = +inf)
@@ -97,7 +109,14 @@ namespace dfx
           for (int k = 0; k < 3; ++k)
             g[k] = y00 * du * dx[k] / r;
       }
-    else if (r > 0.0)
+    else if (r == 0.0)
+      {
+        // at the centre the reference takes theta = phi = 0: Y_10 = sqrt(3 / (4 pi)),
+        // Y_1,+-1 = 0 (its gradient is not defined there: 0)
+        if (m == 0)
+          f = 0.48860251190291992 * u;
+      }
+    else
       {
         // Y_1m = sqrt(3 / (4 pi)) (y, z, x)_m / r for m = -1, 0, 1
         const double c1   = 0.48860251190291992;

@@ -532,7 +532,7 @@ namespace
                 if (it == splAt.end())
                   {
                     it = splAt.emplace(key, (u32)P->splines.size()).first;
-                    for (const auto *v : {&S.atomData[a]->r, &F.y, &F.b, &F.c, &F.d})
+                    for (const auto *v : {&F.knots(S.atomData[a]->r), &F.y, &F.b, &F.c, &F.d})
                       P->splines.insert(P->splines.end(), v->begin(), v->end());
                   }
                 E.splOff = it->second;
@@ -1133,8 +1133,11 @@ extern "C"
               const auto &F = atomFunctions(P->spec, a);
               DFX_REQUIRE(!F.empty(), "atom data with no enrichment function");
               for (const auto &f : F)
-                if (f.q[1] > 1)
-                  throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
+                {
+                  if (f.q[1] > 1)
+                    throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
+                  DFX_REQUIRE(!f.mixed, "a mixed (local potential) field cannot enrich the basis");
+                }
             }
         }
       else

@@ -118,6 +118,8 @@ _sig("dfx_problem_create_operators", _vp, _u32, _pp, _pp, _pp, _pp, _pp, _pp)
 _sig("dfx_problem_density", _vp, _vp, _vp, _vp)
 _sig("dfx_problem_destroy", _vp)
 _sig("dfx_atom_read", C.c_char_p, _vp, _u32, _pp)
+_sig("dfx_atom_read_upf", C.c_char_p, _vp, _u32, _pp)
+_sig("dfx_atom_function_knots", _vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_atom_metadata", _vp, C.c_char_p, C.c_char_p, C.c_uint64, _vp)
 _sig("dfx_atom_radial_grid", _vp, _vp, _vp)
 _sig("dfx_atom_num_functions", _vp, C.c_char_p, _vp)
@@ -612,7 +614,15 @@ class SphericalDataNumerical:
 
     def __init__(self, atom, field, k):
         self.atom, self.field, self.k = atom, field, k
-        NR = atom.NR
+        nk, mixed = C.c_uint32(), C.c_int()
+        tail, cl = C.c_double(), C.c_double()
+        _host_check(_lib.dfx_atom_function_knots(atom.h, field.encode(), k, C.byref(nk), None,
+                                                 C.byref(mixed), C.byref(tail), C.byref(cl)))
+        NR = nk.value
+        self.knots = np.zeros(NR)
+        _host_check(_lib.dfx_atom_function_knots(atom.h, field.encode(), k, None,
+                                                 _ptr(self.knots), None, None, None))
+        self.mixed, self.tail, self.cLeft = bool(mixed.value), tail.value, cl.value
         q = np.zeros(3, dtype=np.int32)
         cut, sm = C.c_double(), C.c_double()
         self.values = np.zeros(NR)
@@ -650,12 +660,14 @@ class AtomSphericalDataEnrichment:
     src/atoms/AtomSphericalDataEnrichment.h:41-89): metadata elements, the
     radial grid and, per field, its numerical spherical functions."""
 
+    _reader = "dfx_atom_read"
+
     def __init__(self, fileName, fieldNames, metadataNames=()):
         names = [f.encode() for f in fieldNames]
         arr = (C.c_char_p * max(1, len(names)))(*names)
         h = C.c_void_p()
-        _host_check(_lib.dfx_atom_read(os.fspath(fileName).encode(), arr, len(names),
-                                       C.byref(h)))
+        _host_check(getattr(_lib, self._reader)(os.fspath(fileName).encode(), arr, len(names),
+                                                C.byref(h)))
         self.h = h
         self.fileName = os.fspath(fileName)
         nr = C.c_uint32()
@@ -696,6 +708,14 @@ class AtomSphericalDataEnrichment:
             if f.qNumbers == list(qNumbers):
                 return f
         raise KeyError(f"no function with quantum numbers {qNumbers} in {fieldName}")
+
+
+class AtomSphericalDataPSP(AtomSphericalDataEnrichment):
+    """A UPF v2 pseudopotential file (atoms::AtomSphericalDataPSP(fileName,
+    fieldNames, metadataNames), src/atoms/AtomSphericalDataPSP.h:47-140):
+    PP_HEADER metadata and the fields vlocal (mixed), beta, pswfc, nlcc,
+    rhoatom."""
+    _reader = "dfx_atom_read_upf"
 
 
 class Problem:

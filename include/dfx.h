@@ -355,8 +355,28 @@ int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *b, dfx_mv *x,
  * strictly increasing or has fewer than two points (the spline needs it). */
 int dfx_atom_read(const char *path, const char *const *fieldNames, uint32_t nFields,
                   dfx_atom **out);
+/* Pseudopotential file, UPF v2 (AtomSphericalDataPSP, src/atoms/
+ * AtomSphericalDataPSP.h:47-140, .cpp:291-830): PP_HEADER attributes become
+ * the metadata (plus "dij": D_ij in Ha when "beta" was read), PP_MESH/PP_R the
+ * radial grid; fields "vlocal", "beta", "pswfc", "nlcc", "rhoatom" with the
+ * reference's unit changes, divisions by r / 4 pi r^2 and Y_00 scaling;
+ * quantum numbers {0,0,0} for vlocal / nlcc / rhoatom, {p, l, m} per
+ * projector (p counts projectors of equal l in a run), {n, l, m} per
+ * pseudo-wavefunction (n from its label).  vlocal is mixed: a spline of the
+ * points within 10.0001 bohr (second-derivative condition on the left, slope
+ * -v/r on the right) and -|z_valence| / r beyond.  Errors as the reference's
+ * checks (header values, counts, repeated quantum numbers, l_max vs the
+ * projectors, unknown field names). */
+int dfx_atom_read_upf(const char *path, const char *const *fieldNames, uint32_t nFields,
+                      dfx_atom **out);
+/* The spline knots of function k (the radial grid except for a mixed
+ * function), whether it is mixed, its tail charge (value tail / r beyond the
+ * last knot, before the Y_00 factor) and its left quadratic coefficient. */
+int dfx_atom_function_knots(const dfx_atom *a, const char *field, uint32_t k, uint32_t *n,
+                            double *knots, int *mixed, double *tail, double *cLeft);
 /* Trimmed text of the single child element `name` of <atom> (symbol, Z,
- * charge, NR, ...): AtomSphericalDataEnrichment::getMetadata (.h:74-75).
+ * charge, NR, ...): AtomSphericalDataEnrichment::getMetadata (.h:74-75); for
+ * a UPF file the PP_HEADER attribute `name` (AtomSphericalDataPSP::getMetadata).
  * Writes at most len bytes including the terminating 0; *needed = length + 1. */
 int dfx_atom_metadata(const dfx_atom *a, const char *name, char *buf, uint64_t len,
                       uint64_t *needed);
@@ -367,16 +387,19 @@ int dfx_atom_num_functions(const dfx_atom *a, const char *field, uint32_t *count
 /* Function k of a field: quantum numbers, cutoff info, radial values and the
  * spline coefficients (value y_i + b_i h + c_i h^2 + d_i h^3 at h = r - r_i on
  * [r_i, r_i+1); left of r_0 y_0 + b_0 h; right of r_{NR-1} quadratic).  Any
- * output may be NULL; arrays hold NR values. */
+ * output may be NULL; arrays hold one value per knot (NR for atom XML files;
+ * dfx_atom_function_knots gives the count). */
 int dfx_atom_function(const dfx_atom *a, const char *field, uint32_t k, int *qNumbers,
                       double *cutoff, double *smoothness, double *values, double *splineB,
                       double *splineC, double *splineD);
 /* On the device: f(x) = R(r) Y_lm(theta, phi) f_c(r) of function k at every
  * point (points: FP64 nPts x 3), relative to origin (host, 3 values), and its
  * gradient (FP64 nPts x 3, may be NULL): SphericalDataNumerical::getValue /
- * getGradientValue (.cpp:330-420).  Zero past cutoff (1 + 1/smoothness); at the
- * origin itself the gradient is 0 (the reference asserts r > 0 there).
- * l > 1: DFX_E_UNSUPPORTED. */
+ * getGradientValue (.cpp:330-420); a mixed function (UPF vlocal) as
+ * SphericalDataMixed::getValue (src/atoms/SphericalDataMixed.cpp:43-69).
+ * Zero past cutoff (1 + 1/smoothness); at the origin itself the value uses
+ * theta = phi = 0 as the reference does and the gradient is 0 (the reference
+ * asserts r > 0 there).  l > 1: DFX_E_UNSUPPORTED. */
 int dfx_atom_evaluate(dfx_ctx *ctx, const dfx_atom *a, const char *field, uint32_t k,
                       const double *origin, dfx_mv *points, dfx_mv *values, dfx_mv *gradients);
 int dfx_atom_destroy(dfx_atom *a);

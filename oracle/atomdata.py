@@ -210,3 +210,129 @@ def evaluate(fun, r_grid, origin, points, polar_tol=1e-12):
                   dVdr * ct - dth * st], axis=1)
     g = np.where((inside & (r > 0))[:, None], g, 0.0)
     return val, g
+
+
+# ------------------------------------------------------------------ UPF
+# AtomSphericalDataPSP (src/atoms/AtomSphericalDataPSP.cpp:291-830) restated
+# with xml.etree and scipy (test infrastructure only).
+Y00 = math.sqrt(1.0 / 2.0) * (1.0 / math.sqrt(2 * math.pi)) * 1.0
+UPF_ELEMENT = {"vlocal": "PP_LOCAL", "beta": "PP_NONLOCAL", "pswfc": "PP_PSWFC",
+               "nlcc": "PP_NLCC", "rhoatom": "PP_RHOATOM"}
+
+
+class SplineBC(Spline):
+    """utils::Spline with a second-derivative condition at the left end and a
+    slope condition at the right (the UPF local potential)."""
+
+    def __init__(self, x, y, left_second, right_slope):
+        self.x, self.y = np.asarray(x, float), np.asarray(y, float)
+        self.cs = CubicSpline(self.x, self.y, bc_type=((2, left_second), (1, right_slope)))
+        self.b0 = float(self.cs(self.x[0], 1))
+        self.c0 = float(self.cs(self.x[0], 2)) / 2.0
+        self.bn = float(self.cs(self.x[-1], 1))
+        self.cn = float(self.cs(self.x[-1], 2)) / 2.0
+
+    def __call__(self, r, nu=0):
+        r = np.asarray(r, float)
+        v = np.asarray(self.cs(r, nu), float)
+        hl, hr = r - self.x[0], r - self.x[-1]
+        left, right = r < self.x[0], r > self.x[-1]
+        if nu == 0:
+            v = np.where(left, self.y[0] + (self.b0 + self.c0 * hl) * hl, v)
+            v = np.where(right, self.y[-1] + (self.bn + self.cn * hr) * hr, v)
+        else:
+            v = np.where(left, self.b0 + 2 * self.c0 * hl, v)
+            v = np.where(right, self.bn + 2.0 * self.cn * hr, v)
+        return v
+
+
+def read_upf(path, fields):
+    root = ET.parse(path).getroot()
+    if root.tag != "UPF":
+        raise OracleAtomError("XPath expression /UPF not found")
+    hdr = root.find("PP_HEADER").attrib
+    meta = {k: v.strip() for k, v in hdr.items()}
+    NR = int(float(meta["mesh_size"]))
+    zv = float(meta["z_valence"])
+    lmax = int(meta.get("l_max", "0"))
+    core = meta.get("core_correction", "F").lower() in ("t", "true", ".true.", "1")
+    r = np.array([float(t) for t in root.find("PP_MESH/PP_R").text.split()])
+    if r.size != NR:
+        raise OracleAtomError("Mismatch in number of radial points")
+    out = {}
+    for f in fields:
+        if f not in UPF_ELEMENT:
+            raise OracleAtomError(f"No element name '{f}' found in UPF_PARENT_PATH_LOOKUP")
+        entries = []
+        if f == "beta":
+            p = lprev = lmx = 0
+            for i, c in enumerate(root.find("PP_NONLOCAL")):
+                if "PP_BETA" in c.tag:
+                    l = int(c.attrib["angular_momentum"])
+                    if i != 0:
+                        p = 0 if l != lprev else p + 1
+                        lprev = l
+                    entries += [(c.text, [p, l, m]) for m in range(-l, l + 1)]
+                    lmx = max(lmx, l)
+            if lmx != lmax:
+                raise OracleAtomError("The lmax from PSP HEADER does not match")
+        elif f == "pswfc":
+            for c in root.find("PP_PSWFC"):
+                if "PP_CHI" in c.tag:
+                    l, n = int(c.attrib["l"]), int(c.attrib["label"][0])
+                    entries += [(c.text, [n, l, m]) for m in range(-l, l + 1)]
+        elif f == "nlcc" and not core:
+            entries.append((" ".join(["0"] * NR), [0, 0, 0]))
+        else:
+            entries.append((root.find(UPF_ELEMENT[f]).text, [0, 0, 0]))
+        qs = [tuple(q) for _, q in entries]
+        if len(set(qs)) != len(qs):
+            raise OracleAtomError("Trying to enter more than one of same quantum number vector")
+        funs = []
+        for text, q in entries:
+            v = np.array([float(t) for t in text.split()])
+            if v.size != NR:
+                raise OracleAtomError("Mismatch in number of points")
+            if f == "vlocal":
+                v = v * 0.5
+            elif f in ("beta", "pswfc"):
+                v[1:] = v[1:] / r[1:]
+                v[0] = v[1]
+            elif f == "rhoatom":
+                v[1:] = v[1:] / (4 * math.pi * r[1:] * r[1:])
+                v[0] = v[1]
+            cutoff = 1.0e10
+            for j in range(NR - 1, 0, -1):
+                if abs(v[j]) >= 1e-10:
+                    cutoff = r[j] if j != NR - 1 else 1.0e10
+                    break
+            if f == "rhoatom":
+                cutoff = r[-1]
+            if f not in ("beta", "pswfc"):
+                v = v * (1.0 / Y00)
+            fun = {"q": q, "cutoff": cutoff, "smoothness": 1.0e10, "values": v, "mixed": False}
+            if f == "vlocal":
+                keep = r <= 10.0001
+                xs, ys = r[keep], v[keep]
+                fun.update(values=ys, knots=xs, mixed=True, tail=-1.0 * (1.0 / Y00) * abs(zv),
+                           left=(ys[1] - ys[0]) / (xs[1] - xs[0]), right=-(ys[-1] / xs[-1]))
+            funs.append(fun)
+        out[f] = funs
+    return {"meta": meta, "r": r, "fields": out, "z_valence": zv}
+
+
+def evaluate_upf(fun, r_grid, origin, points):
+    """Value and gradient of a UPF function: the mixed local potential, or a
+    numerical function as `evaluate` (l <= 1)."""
+    if not fun["mixed"]:
+        return evaluate(fun, r_grid, origin, points)
+    S = SplineBC(fun["knots"], fun["values"], fun["left"], fun["right"])
+    X = np.asarray(points, float) - np.asarray(origin, float)
+    r = np.sqrt((X ** 2).sum(axis=1))
+    safe = np.where(r > 0, r, 1.0)
+    inside = r <= fun["knots"][-1]
+    R = np.where(inside, S(r), fun["tail"] / safe)
+    dR = np.where(inside, S(r, 1), -fun["tail"] / safe ** 2)
+    val = Y00 * R
+    g = np.where((r > 0)[:, None], (Y00 * dR / safe)[:, None] * X, 0.0)
+    return val, g

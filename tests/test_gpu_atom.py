@@ -149,3 +149,38 @@ def test_hydrogen_file_enrichment_lowers_the_ground_state(ctx):
     assert np.all(ev[1] <= ev[0] + 1e-10)
     assert ev[0][0] - ev[1][0] > 0.01          # the orbital carries the cusp region
     assert -0.5 < ev[1][0] < -0.3              # above -1/2 Ha (smeared nucleus)
+
+
+def test_upf_functions_match_oracle(ctx):
+    """Cu.upf on the device: the mixed local potential (spline within 10 bohr,
+    -z_valence / r beyond), the valence and core-correction densities and
+    the s / p projectors and pseudo-wavefunctions against the oracle; d
+    functions are refused."""
+    import dfx
+    path = os.path.join(os.path.dirname(GOLD), "upf", "Cu.upf")
+    fields = ["vlocal", "rhoatom", "nlcc", "beta", "pswfc"]
+    a = dfx.AtomSphericalDataPSP(path, fields)
+    o = A.read_upf(path, fields)
+    origin = np.array([0.2, -0.1, 0.4])
+    rng = np.random.default_rng(17)
+    pts = np.vstack([origin + rng.uniform(-12, 12, (3000, 3)),
+                     origin + rng.uniform(-2, 2, (1000, 3)),
+                     [origin, origin + [0, 0, 0.005], origin + [10.0, 0, 0],
+                      origin + [0, 10.5, 0]]])
+    for f in fields:
+        for fun, ofun in zip(a.getSphericalData(f), o["fields"][f]):
+            if fun.qNumbers[1] > 1:
+                with pytest.raises(dfx.DfxError, match="l > 1"):
+                    fun.evaluate(ctx, pts[:4], origin)
+                continue
+            v, g = fun.evaluate(ctx, pts, origin)
+            vw, gw = A.evaluate_upf(ofun, o["r"], origin, pts)
+            sv, sg = np.abs(vw).max(), np.abs(gw).max()
+            print(f"{f} {fun.qNumbers}: |dv| {np.abs(v - vw).max() / sv:.2e} "
+                  f"|dg| {np.abs(g - gw).max() / max(sg, 1e-300):.2e}", flush=True)
+            assert np.abs(v - vw).max() <= 1e-12 * sv
+            assert np.abs(g - gw).max() <= 1e-10 * max(sg, 1e-300)
+    vl = a.getSphericalData("vlocal")[0]
+    far = origin + np.array([[25.0, 0, 0], [0, -40.0, 0]])
+    assert np.allclose(vl.evaluate(ctx, far, origin, gradient=False), -19.0 / np.array([25.0, 40.0]),
+                       rtol=1e-15)
