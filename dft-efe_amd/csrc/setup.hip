@@ -143,6 +143,18 @@ namespace
     int    pad;
   };
 
+  // the local potential of one atom: the smeared Coulomb -Z erf(r / s) / r,
+  // or (pseudo set) a pseudopotential's mixed local part Y_00 (spline(r) up to
+  // the last knot, tail / r beyond)
+  struct PotDev
+  {
+    double r[3];
+    int    Z;
+    int    pseudo;
+    u32    splOff, splN;
+    double tail, cl;
+  };
+
   __device__ double cutoffDev(double r, double rc, double d, double &dv)
   {
     auto f1 = [](double x) { return x <= 0.0 ? 0.0 : exp(-1.0 / x); };
@@ -220,7 +232,7 @@ namespace
   // (point, function):  F, FW = W F, FV = W V F  (nQ x n) and G, GW (3 nQ x n).
   __global__ void tabulateK(const TabCell *cells, const u32 *map, u32 nClassical,
                             const double *tabL, const double *tabD, const double *tabX,
-                            const double *tabW, int np, const AtomDev *atoms, u32 nAtoms,
+                            const double *tabW, int np, const PotDev *atoms, u32 nAtoms,
                             double smear, const EnrFun *funs, const double *spl,
                             const double *alpha,
                             double *F, double *FW, double *FV, double *G, double *GW)
@@ -245,8 +257,23 @@ namespace
             const double dx = x[0] - atoms[a].r[0], dy = x[1] - atoms[a].r[1],
                          dz = x[2] - atoms[a].r[2];
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
-            v += r < 1e-12 ? -atoms[a].Z * 2.0 / (smear * 1.7724538509055160)
-                           : -atoms[a].Z * erf(r / smear) / r;
+            if (atoms[a].pseudo)
+              {
+                // SphericalDataMixed::getValue (src/atoms/SphericalDataMixed.cpp:43-69)
+                const PotDev &A  = atoms[a];
+                const double *b  = spl + A.splOff;
+                const SplineDev S{b, b + A.splN, b + 2 * (u64)A.splN, b + 3 * (u64)A.splN,
+                                  b + 4 * (u64)A.splN, A.splN, A.cl};
+                double R, dR;
+                if (r <= S.x[S.n - 1])
+                  splineEval(S, r, R, dR);
+                else
+                  R = A.tail / r;
+                v += 0.28209479177387814 * R;
+              }
+            else
+              v += r < 1e-12 ? -atoms[a].Z * 2.0 / (smear * 1.7724538509055160)
+                             : -atoms[a].Z * erf(r / smear) / r;
           }
         auto put = [&](u32 j, double f, double gx, double gy, double gz) {
           const u64 o = C.bufOff;
@@ -415,6 +442,8 @@ struct dfx_problem
   std::vector<u32>    funOff;               // atom a: enriched ids funOff[a] .. funOff[a+1]-1
   std::vector<EnrFun> funs;                 // per enriched id
   std::vector<double> splines;              // numerical functions' knots and coefficients
+  std::vector<const dfx_atom *> localPot;  // pseudopotential local parts (empty: Coulomb)
+  std::vector<PotDev> pots;                 // per atom
   DevBuf H, M, K, alpha;                    // raw (caller-layout) cell matrices
 };
 
@@ -541,6 +570,30 @@ namespace
             P->funs.push_back(E);
           }
       }
+    // local potentials: smeared Coulomb, or the atom's pseudopotential vlocal
+    P->pots.assign(S.nAtoms, PotDev{});
+    for (u32 a = 0; a < S.nAtoms; ++a)
+      {
+        PotDev &V = P->pots[a];
+        for (int d = 0; d < 3; ++d)
+          V.r[d] = S.atomXYZ[3 * a + d];
+        V.Z = S.atomZ[a];
+        if (S.localPotential && S.localPotential[a])
+          {
+            const auto it = S.localPotential[a]->fields.find("vlocal");
+            DFX_REQUIRE(it != S.localPotential[a]->fields.end() && it->second.size() == 1 &&
+                          it->second[0].mixed,
+                        "local potential of an atom: a UPF file read with the field vlocal");
+            const dfx_atom::Function &F = it->second[0];
+            V.pseudo = 1;
+            V.splOff = (u32)P->splines.size();
+            V.splN   = (u32)F.x.size();
+            V.tail   = F.tail;
+            V.cl     = F.cLeft;
+            for (const auto *v : {&F.x, &F.y, &F.b, &F.c, &F.d})
+              P->splines.insert(P->splines.end(), v->begin(), v->end());
+          }
+      }
     P->nEnriched    = P->funOff[nA];
     P->nDofs        = P->nClassical + P->nEnriched;
     P->atomCells.assign(nA, {});
@@ -646,10 +699,7 @@ namespace
     upload(dX, tabX, s);
     upload(dW, tabW, s);
     upload(dMap, P->cellMap, s);
-    std::vector<AtomDev> at(S.nAtoms);
-    for (u32 a = 0; a < S.nAtoms; ++a)
-      at[a] = {{S.atomXYZ[3 * a], S.atomXYZ[3 * a + 1], S.atomXYZ[3 * a + 2]}, S.atomZ[a], 0};
-    upload(dAtoms, at, s);
+    upload(dAtoms, P->pots, s);
     DevBuf dFuns, dSpl;
     upload(dFuns, P->funs, s);
     upload(dSpl, P->splines, s);
@@ -704,7 +754,7 @@ namespace
         DevBuf &bGW = work(ctx, "setup.GW", (size_t)cnt * 3 * nQ * n * 8);
         tabulateK<<<cnt, 256, 0, s>>>(dTc.as<TabCell>(), dMap.as<u32>(), P->nClassical,
                                       dL.as<double>(), dD.as<double>(), dX.as<double>(),
-                                      dW.as<double>(), (int)np, dAtoms.as<AtomDev>(),
+                                      dW.as<double>(), (int)np, dAtoms.as<PotDev>(),
                                       S.nAtoms, S.vSmear, dFuns.as<EnrFun>(),
                                       dSpl.as<double>(), alpha,
                                       bF.as<double>(), bFW.as<double>(), bFV.as<double>(),
@@ -1121,6 +1171,13 @@ extern "C"
       P->atomZ.assign(spec->atomZ, spec->atomZ + spec->nAtoms);
       P->spec.atomXYZ = P->atomXYZ.data();
       P->spec.atomZ   = P->atomZ.data();
+      if (spec->nAtoms && spec->localPotential)
+        {
+          P->localPot.assign(spec->localPotential, spec->localPotential + spec->nAtoms);
+          P->spec.localPotential = P->localPot.data();
+        }
+      else
+        P->spec.localPotential = nullptr;
       if (spec->enrich && spec->nAtoms && spec->atomData)
         {
           P->atomData.assign(spec->atomData, spec->atomData + spec->nAtoms);

@@ -584,7 +584,8 @@ class _ProblemSpec(C.Structure):
                 ("nAtoms", C.c_uint32), ("atomXYZ", C.c_void_p), ("atomZ", C.c_void_p),
                 ("vSmear", C.c_double), ("enrich", C.c_int), ("rcut", C.c_double),
                 ("smooth", C.c_double), ("orthogonalize", C.c_int),
-                ("atomData", C.c_void_p), ("enrichField", C.c_char_p)]
+                ("atomData", C.c_void_p), ("enrichField", C.c_char_p),
+                ("localPotential", C.c_void_p)]
 
 
 _PROB_ARRAYS = {  # name: (id, dtype, size key)
@@ -724,11 +725,14 @@ class Problem:
 
     def __init__(self, ctx, p=3, n=(8, 8, 8), lo=(-10., -10., -10.), hi=(10., 10., 10.),
                  atoms=(), nq=0, nq_enr=0, v_smear=0.5, enrich=False, rcut=4.0,
-                 smooth=2.0, orthogonalize=True, atom_data=None, enrich_field="orbital"):
+                 smooth=2.0, orthogonalize=True, atom_data=None, enrich_field="orbital",
+                 local_potential=None):
         """atom_data: per atom an AtomSphericalDataEnrichment whose field
         `enrich_field` supplies that atom's enrichment functions (numerical
         radial data, each with its own cutoff); None: the analytic Slater set
-        with cutoff rcut and smoothness smooth."""
+        with cutoff rcut and smoothness smooth.  local_potential: per atom an
+        AtomSphericalDataPSP read with "vlocal" (its local part replaces the
+        smeared Coulomb of that atom) or None."""
         self.ctx = ctx
         xyz = np.ascontiguousarray([a[:3] for a in atoms], dtype=np.float64).reshape(-1)
         Z = np.ascontiguousarray([int(a[3]) for a in atoms], dtype=np.int32)
@@ -737,7 +741,13 @@ class Problem:
             if len(atom_data) != len(atoms):
                 raise ValueError("atom_data needs one entry per atom")
             handles = (C.c_void_p * max(1, len(atoms)))(*[d.h.value for d in atom_data])
-        self._keep = (xyz, Z, handles, list(atom_data or ()))
+        pots = None
+        if local_potential is not None:
+            if len(local_potential) != len(atoms):
+                raise ValueError("local_potential needs one entry per atom")
+            pots = (C.c_void_p * max(1, len(atoms)))(
+                *[d.h.value if d is not None else None for d in local_potential])
+        self._keep = (xyz, Z, handles, list(atom_data or ()), pots, list(local_potential or ()))
         s = _ProblemSpec()
         s.p = p
         s.n = (C.c_uint32 * 3)(*[int(v) for v in n])
@@ -750,6 +760,7 @@ class Problem:
         s.orthogonalize = 1 if orthogonalize else 0
         s.atomData = C.cast(handles, C.c_void_p) if handles is not None else None
         s.enrichField = enrich_field.encode() if handles is not None else None
+        s.localPotential = C.cast(pots, C.c_void_p) if pots is not None else None
         h = C.c_void_p()
         ctx.check(_lib.dfx_problem_build(ctx.h, C.byref(s), C.byref(h)))
         self.h = h

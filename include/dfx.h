@@ -437,6 +437,11 @@ typedef struct dfx_problem_spec
    * NULL: the analytic Slater set above.  The handles must outlive the build. */
   const dfx_atom *const *atomData;
   const char   *enrichField;
+  /* local potential: per atom a UPF file read with the field "vlocal" (its
+   * local part, ElectrostaticLocalFE with pseudopotentials, src/ksdft/
+   * ElectrostaticLocalFE.t.cpp) or NULL for the smeared Coulomb above; the
+   * array itself may be NULL.  The handles must outlive the build. */
+  const dfx_atom *const *localPotential;
 } dfx_problem_spec;
 
 #define DFX_PROB_CELL_NDOFS 0  /* uint32 [nCells] */

@@ -184,3 +184,39 @@ def test_upf_functions_match_oracle(ctx):
     far = origin + np.array([[25.0, 0, 0], [0, -40.0, 0]])
     assert np.allclose(vl.evaluate(ctx, far, origin, gradient=False), -19.0 / np.array([25.0, 40.0]),
                        rtol=1e-15)
+
+
+def test_pseudopotential_local_part_in_the_hamiltonian(ctx):
+    """A Cu atom whose local potential comes from Cu.upf: beyond 10 bohr it is
+    -z_valence / r, so every cell farther than that from the atom has the H of
+    a point charge 19 (smeared Coulomb, erf = 1 there) while M and K do not
+    depend on the potential at all; near the atom V_loc is finite and H
+    differs."""
+    import dfx
+    upf = dfx.AtomSphericalDataPSP(os.path.join(os.path.dirname(GOLD), "upf", "Cu.upf"),
+                                   ["vlocal"])
+    kw = dict(p=2, n=(8, 8, 8), lo=(-16.,) * 3, hi=(16.,) * 3, atoms=[(0.3, -0.2, 0.1, 19)],
+              nq=5)
+    Pc = dfx.Problem(ctx, **kw)
+    Pp = dfx.Problem(ctx, local_potential=[upf], **kw)
+    for k in ("cellMap", "cellNdofs"):
+        assert np.array_equal(Pc.get(k), Pp.get(k))
+    assert np.array_equal(Pc.get("M"), Pp.get("M")) and np.array_equal(Pc.get("K"), Pp.get("K"))
+    Hc, Hp = Pc.get("H"), Pp.get("H")
+    n = Pc.get("cellNdofs")
+    h = 4.0
+    far = near = 0
+    off = 0
+    for c in range(Pc.nCells):
+        i, j, k = c % 8, (c // 8) % 8, c // 64
+        lo = np.array([-16 + h * i, -16 + h * j, -16 + h * k])
+        d = np.maximum(0, np.maximum(lo - [0.3, -0.2, 0.1], [0.3, -0.2, 0.1] - (lo + h)))
+        a, b = Hc[off:off + n[c] ** 2], Hp[off:off + n[c] ** 2]
+        off += n[c] ** 2
+        if np.sqrt((d ** 2).sum()) > 10.0001:
+            far += 1
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+        elif np.sqrt((d ** 2).sum()) == 0.0:
+            near += 1
+            assert np.abs(a - b).max() > 1e-3 * np.abs(a).max()
+    assert far > 100 and near == 1
