@@ -341,7 +341,7 @@ namespace
   }
 
   // f(x) = R(r) Y_lm(x / r) f_c(r) and its gradient at every point (rows of
-  // P, 3 values each) relative to the origin; l <= 1
+  // P, 3 values each) relative to the origin; l <= 2
   __global__ void atomEvalK(SplineDev S, int l, int m, double rc, double sm, int mixed,
                             double tail, double ox, double oy, double oz, const double *P,
                             u32 nPts, double *val, double *grad)
@@ -934,8 +934,8 @@ int dfx_atom_evaluate(dfx_ctx *ctx, const dfx_atom *a, const char *field, uint32
     DFX_REQUIRE(!gradients || (gradients->dtype == DFX_F64 && gradients->B == 3 &&
                                gradients->N == points->N),
                 "gradients: FP64, 3 columns, one row per point");
-    if (F.q[1] > 1)
-      throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
+    if (F.q[1] > 2)
+      throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 2 are not supported");
     const std::vector<double> &xs = F.knots(a->r);
     const size_t n  = xs.size();
     DevBuf      &sb = work(ctx, "atom.spline", 5 * n * sizeof(double));

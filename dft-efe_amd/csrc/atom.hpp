@@ -87,7 +87,7 @@ namespace dfx
   }
 
   // f = R(r) Y_lm(dx / r) f_c(r) and grad f at offset dx from the centre,
-  // l <= 1 (SphericalDataNumerical.cpp:40-140): zero past rc (1 + 1/sm);
+  // l <= 2 (SphericalDataNumerical.cpp:40-140): zero past rc (1 + 1/sm);
   // at r = 0 the gradient is 0
   __device__ inline void radialYlm(const SplineDev &S, int l, int m, double rc, double sm,
                                    const double *dx, double &f, double *g)
@@ -111,10 +111,36 @@ namespace dfx
       }
     else if (r == 0.0)
       {
-        // at the centre the reference takes theta = phi = 0: Y_10 = sqrt(3 / (4 pi)),
-        // Y_1,+-1 = 0 (its gradient is not defined there: 0)
+        // at the centre the reference takes theta = phi = 0: Y_l0 = sqrt((2l + 1)
+        // / (4 pi)), Y_lm = 0 for m != 0 (the gradient is not defined there: 0)
         if (m == 0)
-          f = 0.48860251190291992 * u;
+          f = (l == 1 ? 0.48860251190291992 : 0.6307831305050401) * u;
+      }
+    else if (l == 2)
+      {
+        // Y_2m = k_m S_m(x) / r^2 with S = xy, yz, 2z^2 - x^2 - y^2, xz, x^2 - y^2
+        // for m = -2 .. 2 (B_2|m| D_m P_2^|m| Q_m without the Condon-Shortley phase)
+        const double x = dx[0], y = dx[1], z = dx[2];
+        double       k, S, dS[3];
+        switch (m)
+          {
+            case -2: k = 1.0925484305920792, S = x * y, dS[0] = y, dS[1] = x, dS[2] = 0.0; break;
+            case -1: k = 1.0925484305920792, S = y * z, dS[0] = 0.0, dS[1] = z, dS[2] = y; break;
+            case 0:
+              k = 0.31539156525252005, S = 2.0 * z * z - x * x - y * y;
+              dS[0] = -2.0 * x, dS[1] = -2.0 * y, dS[2] = 4.0 * z;
+              break;
+            case 1: k = 1.0925484305920792, S = x * z, dS[0] = z, dS[1] = 0.0, dS[2] = x; break;
+            default:
+              k = 0.5462742152960396, S = x * x - y * y;
+              dS[0] = 2.0 * x, dS[1] = -2.0 * y, dS[2] = 0.0;
+              break;
+          }
+        const double r2 = r * r;
+        const double q = u / r2, dq = du / r2 - 2.0 * u / (r2 * r); // f = k S q(r)
+        f              = k * S * q;
+        for (int c = 0; c < 3; ++c)
+          g[c] = k * (q * dS[c] + S * dq * dx[c] / r);
       }
     else
       {

@@ -1191,8 +1191,8 @@ extern "C"
               DFX_REQUIRE(!F.empty(), "atom data with no enrichment function");
               for (const auto &f : F)
                 {
-                  if (f.q[1] > 1)
-                    throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 1 are not supported");
+                  if (f.q[1] > 2)
+                    throw Error(DFX_E_UNSUPPORTED, "radial functions with l > 2 are not supported");
                   DFX_REQUIRE(!f.mixed, "a mixed (local potential) field cannot enrich the basis");
                 }
             }

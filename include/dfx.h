@@ -340,7 +340,7 @@ int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *b, dfx_mv *x,
  * numerical radial functions.  Replaces AtomSphericalDataEnrichment (src/atoms/
  * AtomSphericalDataEnrichment.h:41-89, .cpp:511-640: libxml2 XPath reads of
  * /dft-efe:atom/dft-efe:<name>) and SphericalDataNumerical (src/atoms/
- * SphericalDataNumerical.h:40-140, .cpp:40-140, 323-420) for l <= 1.
+ * SphericalDataNumerical.h:40-140, .cpp:40-140, 323-420) for l <= 2.
  * Host-side reads; only dfx_atom_evaluate touches the GPU.
  * ==================================================================== */
 
@@ -399,7 +399,7 @@ int dfx_atom_function(const dfx_atom *a, const char *field, uint32_t k, int *qNu
  * SphericalDataMixed::getValue (src/atoms/SphericalDataMixed.cpp:43-69).
  * Zero past cutoff (1 + 1/smoothness); at the origin itself the value uses
  * theta = phi = 0 as the reference does and the gradient is 0 (the reference
- * asserts r > 0 there).  l > 1: DFX_E_UNSUPPORTED. */
+ * asserts r > 0 there).  l > 2: DFX_E_UNSUPPORTED. */
 int dfx_atom_evaluate(dfx_ctx *ctx, const dfx_atom *a, const char *field, uint32_t k,
                       const double *origin, dfx_mv *points, dfx_mv *values, dfx_mv *gradients);
 int dfx_atom_destroy(dfx_atom *a);

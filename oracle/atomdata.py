@@ -166,19 +166,26 @@ def _dqm(m, phi):
 
 def _plm(l, m, th):
     """P_l^m(cos th) without the Condon-Shortley phase and its first two theta
-    derivatives, l <= 1."""
+    derivatives, l <= 2."""
+    c, s = np.cos(th), np.sin(th)
     if l == 0:
         return np.ones_like(th), np.zeros_like(th), np.zeros_like(th)
+    if l == 1:
+        if m == 0:
+            return c, -s, -c
+        return s, c, -s
     if m == 0:
-        return np.cos(th), -np.sin(th), -np.cos(th)
-    return np.sin(th), np.cos(th), -np.sin(th)
+        return (3 * c * c - 1) / 2, -3 * c * s, -3 * (c * c - s * s)
+    if m == 1:
+        return 3 * c * s, 3 * (c * c - s * s), -12 * s * c
+    return 3 * s * s, 6 * s * c, 6 * (c * c - s * s)
 
 
 def evaluate(fun, r_grid, origin, points, polar_tol=1e-12):
     """Value and gradient of fun (a read_atom entry) at points (n x 3)."""
     n_, l, m = fun["q"]
-    if l > 1:
-        raise NotImplementedError("l > 1")
+    if l > 2:
+        raise NotImplementedError("l > 2")
     rc, sm = fun["cutoff"], fun["smoothness"]
     S = Spline(r_grid, fun["values"])
     X = np.asarray(points, float) - np.asarray(origin, float)
@@ -323,7 +330,7 @@ def read_upf(path, fields):
 
 def evaluate_upf(fun, r_grid, origin, points):
     """Value and gradient of a UPF function: the mixed local potential, or a
-    numerical function as `evaluate` (l <= 1)."""
+    numerical function as `evaluate` (l <= 2)."""
     if not fun["mixed"]:
         return evaluate(fun, r_grid, origin, points)
     S = SplineBC(fun["knots"], fun["values"], fun["left"], fun["right"])

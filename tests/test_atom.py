@@ -211,3 +211,39 @@ def test_oracle_spherical_harmonics_pins(tmp_path):
             assert np.abs(Y - math.sqrt(3 / (4 * math.pi)) * dirs[:, axis]).max() < 1e-13
         else:
             assert np.abs(Y - 0.5 / math.sqrt(math.pi)).max() < 1e-15
+
+
+def test_oracle_d_shell_pins(tmp_path):
+    """d shell: normalized on the sphere, Y_2m = k (xy, yz, 2z^2 - x^2 - y^2,
+    xz, x^2 - y^2) / r^2 for m = -2 .. 2, gradients by finite differences."""
+    r = np.linspace(0.0, 8.0, 801)
+    path = write_atom(tmp_path / "d.xml", r, [([3, 2, m], (3.0, 0.6), r * r * np.exp(-r))
+                                              for m in range(-2, 3)])
+    o = A.read_atom(path, ["orbital"])
+    xg, wg = np.polynomial.legendre.leggauss(24)
+    ph = (np.arange(48) + 0.5) * 2 * math.pi / 48
+    ct, P = np.meshgrid(xg, ph, indexing="ij")
+    W = np.meshgrid(wg, np.full(48, 2 * math.pi / 48), indexing="ij")
+    w = (W[0] * W[1]).ravel()
+    st = np.sqrt(1 - ct ** 2)
+    d = np.stack([st * np.cos(P), st * np.sin(P), ct], axis=-1).reshape(-1, 3)
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    want = [math.sqrt(15 / (4 * math.pi)) * x * y, math.sqrt(15 / (4 * math.pi)) * y * z,
+            math.sqrt(5 / (16 * math.pi)) * (2 * z * z - x * x - y * y),
+            math.sqrt(15 / (4 * math.pi)) * x * z, math.sqrt(15 / (16 * math.pi)) * (x * x - y * y)]
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-2, 2, (100, 3))
+    for fun, Yw in zip(o["fields"]["orbital"], want):
+        S = A.Spline(o["r"], fun["values"])
+        v, _ = A.evaluate(fun, o["r"], np.zeros(3), d)
+        Y = v / (S(np.array([1.0]))[0] * A.cutoff(np.array([1.0]), 3.0, 0.6)[0][0])
+        assert abs((w * Y * Y).sum() - 1.0) < 1e-12
+        assert np.abs(Y - Yw).max() < 1e-13
+        _, g = A.evaluate(fun, o["r"], np.zeros(3), pts)
+        fd = np.zeros_like(g)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-6
+            fd[:, k] = (A.evaluate(fun, o["r"], np.zeros(3), pts + e)[0] -
+                        A.evaluate(fun, o["r"], np.zeros(3), pts - e)[0]) / 2e-6
+        assert np.abs(g - fd).max() < 1e-6 * max(1.0, np.abs(g).max())

@@ -64,13 +64,24 @@ def test_p_shell_matches_oracle(ctx, tmp_path):
         assert np.abs(g - gw).max() <= 1e-10 * np.abs(gw).max()
 
 
-def test_d_shell_is_refused(ctx, tmp_path):
+def test_d_shell_matches_oracle_and_f_shell_is_refused(ctx, tmp_path):
     import dfx
-    r = np.linspace(0.0, 4.0, 41)
-    path = write_atom(tmp_path / "d.xml", r, [([3, 2, m], (2.0, 0.6), np.exp(-r))
-                                              for m in range(-2, 3)])
+    r = np.linspace(0.0, 8.0, 801)
+    R = r * r * np.exp(-1.1 * r)
+    path = write_atom(tmp_path / "d.xml", r, [([3, 2, m], (3.0, 0.6), R) for m in range(-2, 3)])
     a = dfx.AtomSphericalDataEnrichment(path, ["orbital"])
-    with pytest.raises(dfx.DfxError, match="l > 1"):
+    o = A.read_atom(path, ["orbital"])
+    origin = np.array([0.25, 0.0, -0.5])
+    pts = _points(np.random.default_rng(21), origin, 4.0, 0.01)
+    for fun, ofun in zip(a.getSphericalData("orbital"), o["fields"]["orbital"]):
+        v, g = fun.evaluate(ctx, pts, origin)
+        vw, gw = A.evaluate(ofun, o["r"], origin, pts)
+        assert np.abs(v - vw).max() <= 1e-12 * np.abs(vw).max()
+        assert np.abs(g - gw).max() <= 1e-10 * np.abs(gw).max()
+    path = write_atom(tmp_path / "f.xml", r, [([4, 3, m], (2.0, 0.6), np.exp(-r))
+                                              for m in range(-3, 4)])
+    a = dfx.AtomSphericalDataEnrichment(path, ["orbital"])
+    with pytest.raises(dfx.DfxError, match="l > 2"):
         a.getSphericalData("orbital")[0].evaluate(ctx, np.zeros((4, 3)))
 
 
@@ -154,8 +165,7 @@ def test_hydrogen_file_enrichment_lowers_the_ground_state(ctx):
 def test_upf_functions_match_oracle(ctx):
     """Cu.upf on the device: the mixed local potential (spline within 10 bohr,
     -z_valence / r beyond), the valence and core-correction densities and
-    the s / p projectors and pseudo-wavefunctions against the oracle; d
-    functions are refused."""
+    the s / p / d projectors and pseudo-wavefunctions against the oracle."""
     import dfx
     path = os.path.join(os.path.dirname(GOLD), "upf", "Cu.upf")
     fields = ["vlocal", "rhoatom", "nlcc", "beta", "pswfc"]
@@ -169,10 +179,6 @@ def test_upf_functions_match_oracle(ctx):
                       origin + [0, 10.5, 0]]])
     for f in fields:
         for fun, ofun in zip(a.getSphericalData(f), o["fields"][f]):
-            if fun.qNumbers[1] > 1:
-                with pytest.raises(dfx.DfxError, match="l > 1"):
-                    fun.evaluate(ctx, pts[:4], origin)
-                continue
             v, g = fun.evaluate(ctx, pts, origin)
             vw, gw = A.evaluate_upf(ofun, o["r"], origin, pts)
             sv, sg = np.abs(vw).max(), np.abs(gw).max()
