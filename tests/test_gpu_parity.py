@@ -487,3 +487,53 @@ def test_mixed_precision_matches_fp64_eigenvalues(ctx, probs):
     occ = out[dfx.PREC_FP64][1]
     err = np.abs(out[dfx.PREC_MIXED][0][:occ] - out[dfx.PREC_FP64][0][:occ]).max()
     assert err < 1e-6
+
+
+# ------------------------------------------------ the largest size class
+def _crowded(n_atoms, seed=8):
+    """p = 3 cells (64 classical dofs) carrying the five Slater functions of
+    up to n_atoms overlapping atoms: cells of more than 192 dofs, the
+    largest size class (four 16-row M tiles per wave)."""
+    rng = np.random.default_rng(seed)
+    atoms = [tuple(float(v) for v in rng.uniform(-2.5, 2.5, 3)) + (1,) for _ in range(n_atoms)]
+    return O.Problem.build(p=3, n=(3, 3, 3), lo=(-6.,) * 3, hi=(6.,) * 3, atoms=atoms,
+                           enrich=True, nq_enr=4)
+
+
+def _cell_ops(ctx, P):
+    d = P.arrays()
+    cons = dfx.Constraints(ctx, P.nDofs, d["rows"], d["rowSizes"], d["rowOffsets"], d["cols"],
+                           d["weights"], d["inhom"])
+    return d, cons, {k: dfx.CellOperator(ctx, d["cellNdofs"], d["cellMap"], d[k], P.nDofs,
+                                         P.nClassical, cons, keep_fp32=True)
+                     for k in ("H", "M", "K")}
+
+
+def test_largest_cells_match_oracle(ctx):
+    P = _crowded(30)
+    d, cons, ops = _cell_ops(ctx, P)
+    nd = d["cellNdofs"]
+    assert 192 < nd.max() <= 256 and (nd > 192).sum() >= 8
+    for B in (8, 64):
+        X = guess(P, B, seed=B + 1)
+        for kind, key in ((O.OP_H, "H"), (O.OP_M, "M"), (O.OP_K, "K")):
+            Xg, Yg = mv(ctx, X), dfx.MultiVector(ctx, P.nDofs, B)
+            ops[key].apply(Xg, Yg)
+            ref = P.apply(kind, X.copy())
+            assert rel(Yg.to_numpy(), ref) < TOL_APPLY64
+    X = guess(P, 24, seed=5)
+    Xg = mv(ctx, X.astype(np.float32), dfx.F32)
+    Yg = dfx.MultiVector(ctx, P.nDofs, 24, dfx.F32)
+    ops["H"].apply(Xg, Yg)
+    ref = P.apply(O.OP_H, X.astype(np.float32).astype(np.float64))
+    assert rel(Yg.to_numpy().astype(np.float64), ref) < TOL_APPLY32
+
+
+def test_cells_above_256_dofs_are_refused(ctx):
+    P = _crowded(40, seed=9)
+    d = P.arrays()
+    assert d["cellNdofs"].max() > 256
+    cons = dfx.Constraints(ctx, P.nDofs, d["rows"], d["rowSizes"], d["rowOffsets"], d["cols"],
+                           d["weights"], d["inhom"])
+    with pytest.raises(dfx.DfxError, match="1..256"):
+        dfx.CellOperator(ctx, d["cellNdofs"], d["cellMap"], d["H"], P.nDofs, P.nClassical, cons)
