@@ -119,10 +119,13 @@ namespace
             return;
         }
     }
+    int depth = 0;
     XNode element()
     {
       if (!starts("<"))
         fail("expected an element");
+      if (++depth > 64)
+        fail("elements nested more than 64 deep");
       ++i;
       XNode n;
       n.name = name();
@@ -132,6 +135,7 @@ namespace
           if (starts("/>"))
             {
               i += 2;
+              --depth;
               return n;
             }
           if (starts(">"))
@@ -168,6 +172,7 @@ namespace
               if (!starts(">"))
                 fail("malformed end tag");
               ++i;
+              --depth;
               return n;
             }
           if (starts("<!--"))
@@ -591,7 +596,11 @@ int dfx_atom_read_upf(const char *path, const char *const *fieldNames, uint32_t 
       if (!toDoubles(v, d) || d.size() != 1)
         throw Error(DFX_E_ARG, std::string("Error while converting /UPF/PP_HEADER element in ") +
                                  file + " (" + what + ")");
-      dst = (std::remove_reference_t<decltype(dst)>)d[0];
+      using D = std::remove_reference_t<decltype(dst)>;
+      if (std::is_integral<D>::value && !(std::fabs(d[0]) < 2147483647.0))
+        throw Error(DFX_E_ARG, std::string("Error while converting /UPF/PP_HEADER element in ") +
+                                 file + " to int (" + what + ")");
+      dst = (D)d[0];
       if ((double)dst != d[0])
         throw Error(DFX_E_ARG, std::string("Error while converting /UPF/PP_HEADER element in ") +
                                  file + " to int (" + what + ")");

@@ -247,3 +247,11 @@ def test_oracle_d_shell_pins(tmp_path):
             fd[:, k] = (A.evaluate(fun, o["r"], np.zeros(3), pts + e)[0] -
                         A.evaluate(fun, o["r"], np.zeros(3), pts - e)[0]) / 2e-6
         assert np.abs(g - fd).max() < 1e-6 * max(1.0, np.abs(g).max())
+
+
+def test_reader_bounds_nesting(tmp_path):
+    import dfx
+    p = tmp_path / "deep.xml"
+    p.write_text(f'<atom xmlns="{A.NS}">' + "<a>" * 100 + "</a>" * 100 + "</atom>")
+    with pytest.raises(dfx.DfxError, match="nested more than 64"):
+        dfx.AtomSphericalDataEnrichment(str(p), [])
