@@ -166,16 +166,28 @@ namespace dfx
           }
       }
 
-      __global__ void colDotsFinalK(const double *partials, u32 nBlocks, u32 B,
-                                    double *out)
+      // one workgroup per column: thread t adds partials t, t + 256, ... in
+      // order, then a fixed-shape tree over the 256 sums (deterministic; the
+      // partial count depends on N only, so a column's result does not
+      // depend on how many columns share the call)
+      __global__ __launch_bounds__(256) void colDotsFinalK(const double *partials, u32 nBlocks,
+                                                           u32 B, double *out)
       {
-        const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-        if (j >= B)
-          return;
-        double s = 0.0;
-        for (u32 b = 0; b < nBlocks; ++b)
+        __shared__ double red[256];
+        const u32 j = blockIdx.x;
+        double    s = 0.0;
+        for (u32 b = threadIdx.x; b < nBlocks; b += 256)
           s += partials[(u64)b * B + j];
-        out[j] = s;
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (u32 w = 128; w > 0; w >>= 1)
+          {
+            if (threadIdx.x < w)
+              red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+          }
+        if (threadIdx.x == 0)
+          out[j] = red[0];
       }
 
       template <class T>
@@ -403,7 +415,7 @@ namespace dfx
       else
         colDotsK<T, 64><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
       checkLaunch("colDots");
-      colDotsFinalK<<<(B + 63) / 64, 64, 0, s>>>(partials, nBlocks, B, out);
+      colDotsFinalK<<<B, 256, 0, s>>>(partials, nBlocks, B, out);
       checkLaunch("colDotsFinal");
     }
 
