@@ -118,21 +118,25 @@ namespace dfx
           }
       }
 
-      // y_E[i, j] = sum_k E[i + nE k] x_E[k, j]   (k ascending)
+      // y_E[i, j] = sum_k E[i + nE k] x_E[k, j]: one workgroup per row i and
+      // 64 columns; four k lanes per column each add k = lane, lane + 4, ...
+      // in ascending order, combined in lane order (fixed, deterministic)
       template <class T>
-      __global__ void enrBlockK(T *y, u32 ldy, const double *E, const T *x,
-                                u32 ldx, u32 nC, u32 nE, u32 B)
+      __global__ __launch_bounds__(256) void enrBlockK(T *y, u32 ldy, const double *E,
+                                                       const T *x, u32 ldx, u32 nC, u32 nE,
+                                                       u32 B)
       {
-        const u64 n = (u64)nE * B;
-        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
-             t += (u64)gridDim.x * blockDim.x)
-          {
-            RC r   = rc(t, B);
-            T  acc = 0;
-            for (u32 kk = 0; kk < nE; ++kk)
-              acc += (T)E[r.i + (u64)nE * kk] * x[(u64)(nC + kk) * ldx + r.j];
-            y[(u64)(nC + r.i) * ldy + r.j] = acc;
-          }
+        __shared__ T part[4][64];
+        const u32 i = blockIdx.x, cj = threadIdx.x % 64, kl = threadIdx.x / 64;
+        const u32 j = blockIdx.y * 64 + cj;
+        T         acc = 0;
+        if (j < B)
+          for (u32 kk = kl; kk < nE; kk += 4)
+            acc += (T)E[i + (u64)nE * kk] * x[(u64)(nC + kk) * ldx + j];
+        part[kl][cj] = acc;
+        __syncthreads();
+        if (kl == 0 && j < B)
+          y[(u64)(nC + i) * ldy + j] = ((part[0][cj] + part[1][cj]) + part[2][cj]) + part[3][cj];
       }
 
       // pass 1 of the column dots: block b owns rows [b*rows, (b+1)*rows);
@@ -391,7 +395,8 @@ namespace dfx
       const u64 n = (u64)nE * x.B;
       if (!n)
         return;
-      enrBlockK<T><<<gridFor(n), TPB, 0, s>>>(y.p, y.ld, E, x.p, x.ld, nC, nE, x.B);
+      enrBlockK<T><<<dim3(nE, (x.B + 63) / 64), 256, 0, s>>>(y.p, y.ld, E, x.p, x.ld, nC, nE,
+                                                             x.B);
       checkLaunch("enrichedBlock");
     }
 
