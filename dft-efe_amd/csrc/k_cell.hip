@@ -689,27 +689,39 @@ namespace dfx
                                                         u32 nC, const T *__restrict__ yE,
                                                         u32 B, T *__restrict__ y, u32 ldy)
       {
-        constexpr int CB = 16, SL = 16, U = 8;
-        __shared__ T part[SL][CB];
+        // slot indices are staged in LDS per chunk of IDX (one coalesced
+        // load instead of a dependent index load per value); lane sl adds
+        // slots sl, sl + SL, ... in ascending order (IDX is a multiple of SL)
+        constexpr int CB = 16, SL = 16, U = 16, IDX = 2048;
+        __shared__ T   part[SL][CB];
+        __shared__ u32 sIdx[IDX];
         const u32 g  = blockIdx.x;
         const int cj = threadIdx.x % CB, sl = threadIdx.x / CB;
         const u32 j  = blockIdx.y * CB + cj;
         const u32 e0 = off[g], e1 = off[g + 1];
         T         acc = 0;
-        if (j < B)
-          for (u32 e = e0 + sl; e < e1; e += SL * U)
-            {
-              T v[U];
-#pragma unroll
-              for (int u = 0; u < U; ++u)
+        for (u32 c0 = e0; c0 < e1; c0 += IDX)
+          {
+            const u32 cn = min((u32)IDX, e1 - c0);
+            __syncthreads(); // the previous chunk's indices are no longer read
+            for (u32 t = threadIdx.x; t < cn; t += blockDim.x)
+              sIdx[t] = slots[c0 + t];
+            __syncthreads();
+            if (j < B)
+              for (u32 e = sl; e < cn; e += SL * U)
                 {
-                  const u32 ee = e + u * SL;
-                  v[u]         = ee < e1 ? yE[(u64)slots[ee] * B + j] : (T)0;
-                }
+                  T v[U];
 #pragma unroll
-              for (int u = 0; u < U; ++u)
-                acc += v[u];
-            }
+                  for (int u = 0; u < U; ++u)
+                    {
+                      const u32 ee = e + u * SL;
+                      v[u]         = ee < cn ? yE[(u64)sIdx[ee] * B + j] : (T)0;
+                    }
+#pragma unroll
+                  for (int u = 0; u < U; ++u)
+                    acc += v[u];
+                }
+          }
         part[sl][cj] = acc;
         __syncthreads();
         if (sl == 0 && j < B)
