@@ -122,7 +122,9 @@ def c3_run(ctx, max_passes=40):
     """C3: enriched FE ChFSI (4 atoms, 16^3 cells at p = 3, 40 eigenpairs,
     degree 40) solved to a 1e-6 residual once in FP64 and once with the FP32
     filter, from the same start: passes, time per pass, eigenvalue
-    agreement."""
+    agreement.  Both replay their captured degree loops (the operator is
+    small enough for launch gaps to matter); each precision is solved once
+    untimed first (first-use allocations and the captures)."""
     import numpy as np
     import dfx
     from dfx import configs
@@ -133,23 +135,24 @@ def c3_run(ctx, max_passes=40):
     N, n = prob.nDofs, cfg["states"]
     out = {}
     for prec in (dfx.PREC_FP64, dfx.PREC_MIXED):
-        G = dfx.MultiVector(ctx, N, n)
-        G.fill_random(1, -1.0, 1.0)
-        cons.distribute_parent_to_child(G)
-        L = dfx.MultiVector(ctx, N, 1)
-        L.fill_random(2, -1.0, 1.0)
-        cons.distribute_parent_to_child(L)
-        W = dfx.MultiVector(ctx, N, n)
-        ks = dfx.KohnShamEigenSolver(ctx, n, 1e-6, max_passes, G, L,
-                                     polynomialDegree=cfg["degree"],
-                                     waveFunctionBatchSize=cfg["block"], precision=prec,
-                                     numElectrons=cfg["electrons"])
-        e = np.zeros(n)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        rc = ks.solve(H, e, W, M, Minv, Mlump)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
+        for timed in (False, True):
+            G = dfx.MultiVector(ctx, N, n)
+            G.fill_random(1, -1.0, 1.0)
+            cons.distribute_parent_to_child(G)
+            L = dfx.MultiVector(ctx, N, 1)
+            L.fill_random(2, -1.0, 1.0)
+            cons.distribute_parent_to_child(L)
+            W = dfx.MultiVector(ctx, N, n)
+            ks = dfx.KohnShamEigenSolver(ctx, n, 1e-6, max_passes, G, L,
+                                         polynomialDegree=cfg["degree"],
+                                         waveFunctionBatchSize=cfg["block"], precision=prec,
+                                         numElectrons=cfg["electrons"], useGraph=True)
+            e = np.zeros(n)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            rc = ks.solve(H, e, W, M, Minv, Mlump)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
         out[prec] = (rc, e, ks.result, dt)
     occ = int(out[dfx.PREC_FP64][2]["numOccupied"])
     f64, mix = out[dfx.PREC_FP64], out[dfx.PREC_MIXED]
@@ -159,7 +162,7 @@ def c3_run(ctx, max_passes=40):
             "fp64_ms_per_pass": f64[3] * 1e3 / max(int(f64[2]["passes"]), 1),
             "mixed_rc": int(mix[0]), "mixed_passes": int(mix[2]["passes"]),
             "mixed_ms_per_pass": mix[3] * 1e3 / max(int(mix[2]["passes"]), 1),
-            "occupied_states": occ,
+            "occupied_states": occ, "degree_loop": "captured graph, replayed every pass",
             "max_eig_err_mixed_vs_fp64_Ha": float(np.abs(mix[1][:occ] - f64[1][:occ]).max())}
 
 
