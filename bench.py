@@ -412,7 +412,7 @@ def main():
             e = np.zeros(n)
             t0 = time.time()
             rc = ks.solve(H, e, W, M, Minv, Mlump)
-            out[prec] = (rc, e, ks.result, ks.residualNorms, time.time() - t0)
+            out[prec] = (rc, e, ks.result, ks.residualNorms, time.time() - t0, W)
             log(f"KS solve {'mixed' if prec else 'FP64'}: rc {rc} passes "
                 f"{ks.result['passes']} occupied {ks.result['numOccupied']} "
                 f"max occ. residual {ks.residualNorms[:ks.result['numOccupied']].max():.2e} "
@@ -424,6 +424,24 @@ def main():
                 "mixed_rc": int(out[1][0]), "mixed_passes": int(out[1][2]["passes"]),
                 "residual_tol": 1e-6,
                 "fp64_solve_s": round(out[0][4], 2), "mixed_solve_s": round(out[1][4], 2)}
+        # the electron density of the occupied states (occupation 1 each)
+        # from both converged subspaces: rho = sum_i 2 f_i psi_i^2
+        try:
+            fo = np.zeros(n)
+            fo[:occ] = 1.0
+            r64 = prob.density(fo, out[0][5]).to_numpy().ravel()
+            rmx = prob.density(fo, out[1][5]).to_numpy().ravel()
+            wq = prob.get("quadW")
+            conv["density_err_max_rel"] = float(np.abs(rmx - r64).max() / np.abs(r64).max())
+            conv["density_err_integrated_electrons"] = float(np.sum(wq * np.abs(rmx - r64)))
+            conv["electrons_from_fp64_density"] = float(np.sum(wq * r64))
+            log(f"density mixed vs FP64: max rel {conv['density_err_max_rel']:.2e}, "
+                f"integrated |drho| {conv['density_err_integrated_electrons']:.2e} e "
+                f"(of {conv['electrons_from_fp64_density']:.6f})")
+            del r64, rmx, wq
+        except dfx.DfxError as ex:
+            log(f"density comparison skipped: {ex}")
+        del out
 
     # electron density of the 200-state subspace at every quadrature point
     # (next to the eigensolve: DensityCalculator::computeRho), first 70
