@@ -235,6 +235,28 @@ namespace dfx
           }
       }
 
+      // residual filter set-up from the FP64 seed Y (ChebyshevFilter.t.cpp:
+      // 330-359): r0 = Y, rn = a Y + b Y (a = alpha1, b = 0, the reference's
+      // axpby), res = 0, in T; the same arithmetic as the separate
+      // convert / axpby / convert / fill launches
+      template <class T>
+      __global__ void resSeedK(T *r0, T *rn, T *res, u32 ld, const double *Y, u32 ldy,
+                               double a, double b, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC           r = rc(t, B);
+            const double y = Y[(u64)r.i * ldy + r.j];
+            const u64    o = (u64)r.i * ld + r.j;
+            const double v = a * y + b * y;
+            r0[o]          = (T)y;
+            rn[o]          = (T)v;
+            res[o]         = (T)0.0;
+          }
+      }
+
       // ------------------------------------------------ constraints
       // ConstraintsInternal.cpp:38-107 restated per (row, column): the row's
       // value is inhom + sum_j w_j x[col_j] accumulated with fma in entry order.
@@ -451,6 +473,19 @@ namespace dfx
     }
 
     template <class T>
+    void resSeed(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> Y,
+                 double alpha1)
+    {
+      const u64 n = (u64)Y.N * Y.B;
+      if (!n)
+        return;
+      DFX_REQUIRE(r0.ld == rn.ld && rn.ld == res.ld, "residual seed: one leading dimension");
+      resSeedK<T><<<gridFor(n), TPB, 0, s>>>(r0.p, rn.p, res.p, r0.ld, Y.p, Y.ld, alpha1, 0.0,
+                                             Y.N, Y.B);
+      checkLaunch("resSeed");
+    }
+
+    template <class T>
     void p2c(hipStream_t s, const dfx_constraints *c, View<T> x)
     {
       if (!c || !c->nRows || !x.B)
@@ -535,6 +570,8 @@ namespace dfx
                             double);                                               \
   template void resStep<T>(hipStream_t, View<T>, View<const T>, View<const T>,     \
                            View<const T>, const double *, const double *);         \
+  template void resSeed<T>(hipStream_t, View<T>, View<T>, View<T>,                 \
+                           View<const double>, double);                            \
   template void p2c<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void c2p<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void setRows<T>(hipStream_t, const dfx_constraints *, View<T>, double); \

@@ -54,6 +54,11 @@ namespace dfx
     template <class T>
     void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
                  View<const T> r0, const double *mu, const double *coef);
+    // residual filter set-up from the FP64 seed Y: r0 = (T) Y, rn = (T)(alpha1 Y
+    // + 0 Y), res = 0 (r0, rn, res share one leading dimension)
+    template <class T>
+    void resSeed(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> Y,
+                 double alpha1);
 
     // -------------------------------------------------------- constraints
     template <class T>

@@ -264,12 +264,8 @@ namespace dfx
       View<T> rs = workView<T>(ctx, "rf.res", N, nb);
       View<T> s1 = workView<T>(ctx, "rf.s1", N, nb);
       View<T> s2 = workView<T>(ctx, "rf.s2", N, nb);
-      // ResidualNew = alpha1 * Y ; Residual = 0
-      k::convert<T, double>(s, r0, cv(Y));
-      View<double> rn64 = workView<double>(ctx, "rf.rn64", N, nb);
-      k::axpby<double>(s, rn64, rnScale, cv(Y), 0.0, cv(Y));
-      k::convert<T, double>(s, rn, cv(rn64));
-      k::fill<T>(s, rs, 0.0);
+      // ResidualNew = alpha1 * Y ; Residual = 0 (one fused launch)
+      k::resSeed<T>(s, r0, rn, rs, cv(Y), rnScale);
       auto step = [&](u32 deg) {
         if (!applyInvThenCellwise<T>(BInv, A, rn, s1, s2))
           {
