@@ -257,6 +257,24 @@ namespace dfx
           }
       }
 
+      // residual filter result (ChebyshevFilter.t.cpp:424-441): z = a1 a[j]
+      // (double) x + b1 b[j] y with x the last residual in T; the arithmetic of
+      // a conversion followed by the FP64 blocked axpby
+      template <class T>
+      __global__ void resFinalK(double *z, u32 ldz, double a1, const double *a, const T *x,
+                                u32 ldx, double b1, const double *b, const double *y, u32 ldy,
+                                u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC           r  = rc(t, B);
+            const double xv = (double)x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
+            z[(u64)r.i * ldz + r.j] = a1 * (double)a[r.j] * xv + b1 * (double)b[r.j] * yv;
+          }
+      }
+
       // ------------------------------------------------ constraints
       // ConstraintsInternal.cpp:38-107 restated per (row, column): the row's
       // value is inhom + sum_j w_j x[col_j] accumulated with fma in entry order.
@@ -486,6 +504,18 @@ namespace dfx
     }
 
     template <class T>
+    void resFinal(hipStream_t s, View<double> z, double a1, const double *a, View<const T> x,
+                  double b1, const double *b, View<const double> y)
+    {
+      const u64 n = (u64)z.N * z.B;
+      if (!n)
+        return;
+      resFinalK<T><<<gridFor(n), TPB, 0, s>>>(z.p, z.ld, a1, a, x.p, x.ld, b1, b, y.p, y.ld,
+                                              z.N, z.B);
+      checkLaunch("resFinal");
+    }
+
+    template <class T>
     void p2c(hipStream_t s, const dfx_constraints *c, View<T> x)
     {
       if (!c || !c->nRows || !x.B)
@@ -572,6 +602,9 @@ namespace dfx
                            View<const T>, const double *, const double *);         \
   template void resSeed<T>(hipStream_t, View<T>, View<T>, View<T>,                 \
                            View<const double>, double);                            \
+  template void resFinal<T>(hipStream_t, View<double>, double, const double *,     \
+                            View<const T>, double, const double *,                 \
+                            View<const double>);                                   \
   template void p2c<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void c2p<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void setRows<T>(hipStream_t, const dfx_constraints *, View<T>, double); \

@@ -59,6 +59,11 @@ namespace dfx
     template <class T>
     void resSeed(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> Y,
                  double alpha1);
+    // z[i,j] = a1 a[j] (double) x[i,j] + b1 b[j] y[i,j] (x in T; a, b device
+    // arrays of length B): the residual filter's result
+    template <class T>
+    void resFinal(hipStream_t s, View<double> z, double a1, const double *a, View<const T> x,
+                  double b1, const double *b, View<const double> y);
 
     // -------------------------------------------------------- constraints
     template <class T>

@@ -297,10 +297,7 @@ namespace dfx
           std::swap(rn, rs);
       });
       applyOp<T>(BInv, rn, rs);
-      View<double> rs64 = workView<double>(ctx, "rf.res64", N, nb);
-      k::convert<double, T>(s, rs64, cv(rs));
-      k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(rs64), 1.0,
-                              dtab + (size_t)(m + 1) * nb, cv(Xin));
+      k::resFinal<T>(s, Y, 1.0, dOnes, cv(rs), 1.0, dtab + (size_t)(m + 1) * nb, cv(Xin));
     }
 
     // --------------------------------------------------- CholGS and RR
