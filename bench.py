@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the Kohn-Sham eigensolver hot path on one MI355X.
 
-A step is one ChFSI pass (Chebyshev filter over all blocks + Cholesky-Gram-
-Schmidt + Rayleigh-Ritz, the reference's "Chebyshev Filter" / "OrthoNormalization"
-/ "RR Step" split, src/linearAlgebra/ChebyshevFilteredEigenSolver.t.cpp:222-397)
-on the 32-atom enriched workload (c4: 32^3 cells at p=4, ~2.3M dofs,
-200 states, block 64, degree 30), mixed precision (FP32 filter bulk, FP64
-orthonormalization / Rayleigh-Ritz).  All inputs live in HBM before the timed
-region; the problem setup and the Lanczos bounds are not timed.
+A step is one pass of the Kohn-Sham eigensolver's loop: one ChFSI pass
+(Chebyshev filter over all blocks + Cholesky-Gram-Schmidt + Rayleigh-Ritz, the
+reference's "Chebyshev Filter" / "OrthoNormalization" / "RR Step" split,
+src/linearAlgebra/ChebyshevFilteredEigenSolver.t.cpp:222-397) followed by the
+residual norms of every state (KohnShamEigenSolver::getLinearEigenSolveResidual,
+src/ksdft/KohnShamEigenSolver.t.cpp:574-684), on the 32-atom enriched workload
+(c4: 32^3 cells at p=4, ~2.1M dofs, 200 states, block 64, degree 30), mixed
+precision (FP32 filter bulk, FP64 orthonormalization / Rayleigh-Ritz /
+residuals).  All inputs live in HBM before the timed region; the problem
+setup and the Lanczos bounds are not timed.
 
 Prints one JSON line (see the keys below).  Progress goes to stderr.
 """
@@ -46,51 +49,70 @@ def traffic_from_profiles(kind):
         return None
 
 
-def cpu_baseline(prob, n_states, block, degree, budget_cells=4096, threads=None):
-    """The reference's CPU apply (oracle: gather, per-cell product, scatter,
-    constraints) on a sample of the workload's cells, extrapolated to one pass.
-    Test infrastructure (oracle/) is used here as the CPU baseline only."""
+def cpu_baseline(prob, n_states, block, degree, threads=16, sample_every=8, reps=5):
+    """The reference's CPU apply (the oracle's threaded cell-wise apply: gather,
+    per-cell product, scatter into a slab-local vector, parallel merge, plus
+    the constraint work) on the C4 operator in FP64 at `block` columns:
+      leg 1: `threads` threads, the full mesh, median of `reps` after a warm-up;
+      leg 2: 1 thread (the reference's per-rank behaviour), every
+             `sample_every`-th cell, median of `reps`, the cell part scaled by
+             the sample's share of sum n_e^2.
+    A pass is (degree + 5) column-applies per state (residual seed H and M,
+    degree - 1 filter applies, CholGS M, RR H, residual norms H and M); the
+    O(N B) vector updates, Gram matrices and dense factorizations are not
+    counted.  Test infrastructure (oracle/) is used here as the CPU baseline
+    only."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nd = prob.get("cellNdofs").astype(np.int64)
     cmap = prob.get("cellMap")
+    cons = dict(rows=prob.get("rows"), rowSizes=prob.get("rowSizes"),
+                rowOffsets=prob.get("rowOffsets"), cols=prob.get("cols"),
+                weights=prob.get("weights"), inhom=prob.get("inhom"))
+    X = O.rng_uniform(9, -1, 1, prob.nDofs * block).reshape(prob.nDofs, block)
+    col_applies = n_states * (degree + 5)
     H = prob.get("H")
-    stride = max(1, len(nd) // budget_cells)
-    sample = np.arange(0, len(nd), stride)
+    P = O.Problem.from_arrays(nd, cmap, prob.nDofs, prob.nClassical, H=H, p=prob.p, **cons)
+    del H
+    t_full, t_cells, t_vec, _ = P.time_apply(O.OP_H, X, reps=reps, threads=threads)
+    del P
+    # 1-thread leg on a sample of cells
     off = np.concatenate([[0], np.cumsum(nd)])
     moff = np.concatenate([[0], np.cumsum(nd * nd)])
+    sample = np.arange(0, len(nd), sample_every)
     smap = np.concatenate([cmap[off[e]:off[e + 1]] for e in sample])
+    H = prob.get("H")
     sH = np.concatenate([H[moff[e]:moff[e + 1]] for e in sample])
     del H
-    P = O.Problem.from_arrays(nd[sample], smap, prob.nDofs, prob.nClassical, H=sH,
-                              rows=prob.get("rows"), rowSizes=prob.get("rowSizes"),
-                              rowOffsets=prob.get("rowOffsets"), cols=prob.get("cols"),
-                              weights=prob.get("weights"), inhom=prob.get("inhom"),
-                              p=prob.p)
-    X = O.rng_uniform(9, -1, 1, prob.nDofs * block).reshape(prob.nDofs, block)
-    T = threads or min(16, os.cpu_count() or 1)
-    _, t_cells, t_vec, _ = P.time_apply(O.OP_H, X, reps=3, threads=T)
+    Ps = O.Problem.from_arrays(nd[sample], smap, prob.nDofs, prob.nClassical, H=sH, p=prob.p,
+                               **cons)
+    _, t1_cells, t1_vec, _ = Ps.time_apply(O.OP_H, X, reps=reps, threads=1)
     frac = float((nd[sample] ** 2).sum()) / float((nd ** 2).sum())
-    # full-mesh apply of one block: sampled cell work scaled by the share of
-    # sum n_e^2 it covers, plus the full-size vector work measured as is
-    t_apply = t_cells / frac + t_vec
-    # column-applies per pass: filter (degree + 1 per column: the residual seed
-    # H and M applies and degree-1 H applies), CholGS (M), RR (H)
-    col_applies = n_states * (degree + 1) + 2 * n_states
-    t_pass = t_apply * col_applies / block
-    return {"value": 1.0 / t_pass, "unit": "passes/s", "cores": T, "kind": "port",
-            "sample": (f"oracle cell-wise apply, FP64, B={block}: {len(sample)} of {len(nd)} "
-                       f"cells (every {stride}th; gather, vectorized per-cell product, "
-                       f"slab-local scatter, parallel merge) {t_cells * 1e3:.1f} ms, "
-                       f"scaled by 1/{frac:.4f} -> {t_cells / frac:.2f} s, plus full-vector "
-                       f"constraints and zeroing {t_vec * 1e3:.1f} ms = {t_apply:.2f} s per "
-                       f"block apply, x{col_applies / block:.1f} block applies per pass, "
-                       f"{T} threads; vector updates, Gram, Cholesky and the dense "
-                       f"eigensolve are not counted")}
+    t1_apply = t1_cells / frac + t1_vec
+    t_pass = t_full * col_applies / block
+    t1_pass = t1_apply * col_applies / block
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                 if l.startswith("model name")][0]
+    except (OSError, IndexError):
+        model = "unknown"
+    return {"value": 1.0 / t_pass, "unit": "passes/s", "cores": threads, "kind": "port",
+            "value_1_thread": 1.0 / t1_pass,
+            "apply_s": {str(threads) + "_threads_full_mesh": t_full,
+                        "1_thread_extrapolated": t1_apply},
+            "cpu": model,
+            "sample": (f"oracle cell-wise apply of the C4 Hamiltonian, FP64, B={block}: "
+                       f"{threads} threads on the full mesh ({len(nd)} cells), median of {reps} "
+                       f"= {t_full:.3f} s (cells {t_cells:.3f} s, constraints and zeroing "
+                       f"{t_vec:.3f} s); 1 thread on every {sample_every}th cell, median of "
+                       f"{reps}, cells {t1_cells:.3f} s / {frac:.4f} + vector work "
+                       f"{t1_vec:.3f} s = {t1_apply:.2f} s; a pass = {col_applies / block:.1f} "
+                       f"such block applies (filter, CholGS, RR, residual norms); vector "
+                       f"updates, Gram matrices and dense factorizations not counted")}
 
 
-def serving_run(n_requests, rate, slots, seed=5, use_graph=True):
+def serving_run(n_requests, rate, slots, seed=5, use_graph=True, saturate=False):
     """The C5 stream (or its first n_requests): Poisson arrivals, continuous
     batching on `slots` streams, FP32 filter / FP64 RR.  Latency = result time
     minus arrival time."""
@@ -98,6 +120,8 @@ def serving_run(n_requests, rate, slots, seed=5, use_graph=True):
     import dfx
     from dfx import configs
     reqs = configs.serving_stream(seed=seed, n_requests=n_requests, rate=rate)
+    if saturate:   # every request waiting at t = 0: the sustained capacity
+        reqs = [dict(r, arrival=0.0) for r in reqs]
     # first-use costs (code objects, library handles) outside the measurement
     dfx.serve([dict(reqs[0], arrival=0.0)], max_slots=1)
     t0 = time.perf_counter()
@@ -107,7 +131,9 @@ def serving_run(n_requests, rate, slots, seed=5, use_graph=True):
     lat = np.array([x["doneSeconds"] - r["arrival"] for x, r in zip(res, reqs)])
     ok = sum(1 for x in res if x["status"] == 0)
     last = max(x["doneSeconds"] for x in res)
-    return {"requests": n_requests, "arrival_rate_per_s": rate, "slots": slots,
+    return {"requests": n_requests, "arrival_rate_per_s": None if saturate else rate,
+            "arrivals": "all at t = 0 (saturation)" if saturate else "Poisson process",
+            "slots": slots,
             "solves_per_s": n_requests / last, "wall_s": round(wall, 3),
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
@@ -154,6 +180,34 @@ def c3_run(ctx, max_passes=40):
             ctx.synchronize()
             dt = time.perf_counter() - t0
         out[prec] = (rc, e, ks.result, dt)
+    # the FP32 cell kernel on C3's operator at the filter's width with M^-1
+    # folded in (the degree loop's apply), 20 eager applies profiled per
+    # colour step (the solves above replay graphs, which are not profiled)
+    info = H.info()
+    X = dfx.MultiVector(ctx, N, n, dfx.F32)
+    X.fill_random(3, -1.0, 1.0)
+    Y = dfx.MultiVector(ctx, N, n, dfx.F32)
+    H.apply_after_inverse(Minv, X, Y)
+    H.set_profiling(True, dfx.F32)
+    H.profile(reset=True)
+    for _ in range(20):
+        H.apply_after_inverse(Minv, X, Y)
+    ctx.synchronize()
+    pr = H.profile(reset=True)
+    H.set_profiling(False)
+    nrows = len(prob.get("rows"))
+    fl = 2.0 * n * info["sumNe2"] / info["nColors"]
+    by = (4.0 * info["sumNe2"] + 4.0 * (info["sumNe"] + info["nCells"]) + 2.0 * 4 * N * n +
+          4.0 * 2 * nrows + 4.0 * nrows * n) / info["nColors"]
+    ms = pr["kernelMs"] / max(pr["launches"], 1)
+    t_m, t_h = fl / (PEAK_FP32_MFMA * 1e12), by / (PEAK_HBM * 1e9)
+    ach = fl / (ms * 1e-3) / 1e12 if ms else 0.0
+    c3_roof = {"bound": "mfma" if t_m > t_h else "hbm", "kernel": "cellApplyBufK<float> "
+               "(M^-1 folded), one colour step per launch unit",
+               "achieved_tflops": ach, "frac_of_fp32_mfma_peak": ach / PEAK_FP32_MFMA,
+               "achieved_gbps": by / (ms * 1e-3) / 1e9 if ms else 0.0,
+               "frac_of_hbm_peak": (by / (ms * 1e-3) / 1e9 / PEAK_HBM) if ms else 0.0,
+               "avg_launch_ms": ms, "ideal_us_mfma_hbm": [t_m * 1e6, t_h * 1e6]}
     occ = int(out[dfx.PREC_FP64][2]["numOccupied"])
     f64, mix = out[dfx.PREC_FP64], out[dfx.PREC_MIXED]
     return {"workload": f"c3: {len(cfg['atoms'])} atoms, 16^3 cells, p=3, {N} dofs, "
@@ -163,15 +217,18 @@ def c3_run(ctx, max_passes=40):
             "mixed_rc": int(mix[0]), "mixed_passes": int(mix[2]["passes"]),
             "mixed_ms_per_pass": mix[3] * 1e3 / max(int(mix[2]["passes"]), 1),
             "occupied_states": occ, "degree_loop": "captured graph, replayed every pass",
-            "max_eig_err_mixed_vs_fp64_Ha": float(np.abs(mix[1][:occ] - f64[1][:occ]).max())}
+            "max_eig_err_mixed_vs_fp64_Ha": float(np.abs(mix[1][:occ] - f64[1][:occ]).max()),
+            "roofline": c3_roof}
 
 
 def poisson_run(ctx):
     """C2: batched Poisson CG on the electrostatics path -- 16^3 cells at p=3
     in a 20-bohr cube, zero Dirichlet, Jacobi preconditioner, relative
-    tolerance 1e-10, 8 right-hand sides 4 pi M rho_k with rho_k four
-    normalized Gaussians (width 0.3 bohr, charges 1-6, centres from seed 100+k)
-    sampled at the nodes.  The same cell kernel applies K."""
+    tolerance 1e-10, 8 right-hand sides 4 pi rho_k with rho_k four normalized
+    Gaussians (width 0.3 bohr, charges 1-6, centres from seed 100+k) at the
+    quadrature points, integrated against the basis by the library's Poisson
+    function (PoissonLinearSolverFunctionFE); CGLinearSolver::solve(function)
+    is timed.  The same cell kernel applies K."""
     import numpy as np
     import dfx
     from dfx import configs
@@ -180,42 +237,45 @@ def poisson_run(ctx):
     prob = dfx.Problem(ctx, p=cfg["p"], n=cfg["n"], lo=cfg["lo"], hi=cfg["hi"], atoms=[],
                        enrich=False, orthogonalize=False)
     H, M, K, Minv, Mlump, cons = prob.operators(keep_fp32=False)
-    setup = time.perf_counter() - t0
     N, nr, w = prob.nDofs, cfg["rhs"], cfg["width"]
-    nd = prob.get("cellNdofs").astype(np.int64)
-    cmap = prob.get("cellMap").astype(np.int64)
-    Kc = prob.get("K")
-    n0 = int(nd[0])
-    assert np.all(nd == n0)
-    diag = np.zeros(N)
-    np.add.at(diag, cmap, Kc.reshape(-1, n0, n0).diagonal(axis1=1, axis2=2).reshape(-1))
-    diag[prob.get("rows")] = 1.0
-    pc = dfx.DiagonalOperator(ctx, 1.0 / diag)
-    xyz = prob.get("nodeXYZ").reshape(-1, 3)
-    rho = np.zeros((N, nr))
+    xq = prob.get("quadXYZ").reshape(-1, 3)
+    f = np.zeros((xq.shape[0], nr))
     for k in range(nr):
         cen, q = configs.poisson_rhs_charges(100 + k)
         for c, qq in zip(cen, q):
-            r2 = ((xyz - c) ** 2).sum(axis=1)
-            rho[:, k] += qq * (2 * np.pi * w * w) ** -1.5 * np.exp(-r2 / (2 * w * w))
-    R = dfx.MultiVector.from_numpy(ctx, 4 * np.pi * rho)
-    b = dfx.MultiVector(ctx, N, nr)
-    M.apply(R, b)
-    x = dfx.MultiVector(ctx, N, nr)
-    x.set(0.0)
+            r2 = ((xq - c) ** 2).sum(axis=1)
+            f[:, k] += 4 * np.pi * qq * (2 * np.pi * w * w) ** -1.5 * np.exp(-r2 / (2 * w * w))
+    fn = dfx.PoissonLinearSolverFunctionFE(prob, K, cons, dfx.MultiVector.from_numpy(ctx, f))
+    setup = time.perf_counter() - t0
+    b = fn.getRhs()
     cg = dfx.CGLinearSolver(ctx, 10000, 1e-30, cfg["rel_tol"], 1e10)
     ctx.synchronize()
     t0 = time.perf_counter()
-    rc = cg.solve(K, pc, b, x)
+    rc = cg.solve(fn)
     ctx.synchronize()
     t = time.perf_counter() - t0
     bn = b.l2norms()
+    info = K.info()
+    # one CG iteration: about 17 N B FP64 vector accesses (PCG's dots,
+    # updates and Jacobi scaling) plus the minimum bytes of one K apply
+    # (cell matrices once, the map, X read and Y written once)
+    ncons = len(prob.get("rows"))
+    bytes_apply = (8.0 * info["sumNe2"] + 4.0 * (info["sumNe"] + info["nCells"]) +
+                   2.0 * 8 * N * nr + 4.0 * 2 * ncons + 8.0 * ncons)
+    bytes_iter = 17.0 * 8 * N * nr + bytes_apply
+    t_iter = t / max(cg.iterations, 1)
+    achieved = bytes_iter / t_iter / 1e9
     return {"workload": "c2: 16^3 cells, p=3, %d dofs, %d right-hand sides, Jacobi, rel. tol %g"
                         % (N, nr, cfg["rel_tol"]),
             "rc": int(rc), "iterations": int(cg.iterations), "solve_ms": t * 1e3,
-            "ms_per_iteration": t * 1e3 / max(cg.iterations, 1),
+            "ms_per_iteration": t_iter * 1e3,
             "max_relative_residual": float(np.max(cg.residualNorms / bn)),
-            "setup_s": round(setup, 3)}
+            "rhs": "integral of 4 pi rho against the basis at the quadrature points",
+            "setup_s": round(setup, 3),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM, "algorithmic_bytes_per_iteration": bytes_iter,
+                         "note": "17 N B FP64 vector accesses + one K apply's minimum bytes "
+                                 "per iteration, over the measured ms per iteration"}}
 
 
 def batch_widths(n, bs):
@@ -243,9 +303,11 @@ def main():
                          "launches in the trace)")
     ap.add_argument("--no-convergence", action="store_true")
     ap.add_argument("--convergence-passes", type=int, default=25)
-    ap.add_argument("--serve-requests", type=int, default=64,
+    ap.add_argument("--serve-requests", type=int, default=256,
                     help="requests of the C5 stream served after the C4 passes (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=6.0)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--no-serve-saturation", dest="serve_saturation", action="store_false",
+                    help="skip the run with every request waiting at t = 0")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy 5 times (the reference kernel for "
@@ -344,10 +406,16 @@ def main():
         return dfx.ChebyshevFilteredEigenSolver(ctx, bounds[0], bounds[1], c, m, 1e-14, guess,
                                                 True, B, prec)
 
+    res_ms = []
+
     def one_pass(solver, guess, ev, out, bnd):
         rc = solver.solve(H, ev, out, True, M, Minv)
         if rc != 0:
             raise RuntimeError(f"ChFSI pass failed: {dfx.EIG_CODES.get(rc, rc)}")
+        # the pass's convergence test: residual norms of every state (FP64)
+        tr = time.perf_counter()
+        dfx.eigen_residual_norms(ctx, H, M, out, ev)
+        res_ms.append((time.perf_counter() - tr) * 1e3)
         guess.copy_from(out)
         bnd[0], bnd[1] = ev[0], ev[-1]
         solver.reinit(bnd[0], bnd[1], c, m, 1e-14, guess)
@@ -364,6 +432,7 @@ def main():
     log(f"warm-up done ({args.warmup} passes); timing {args.steps} passes")
     H.set_profiling(True, dfx.F32)
     H.profile(reset=True)
+    res_ms.clear()
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -372,9 +441,11 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     prof32 = H.profile(reset=True)
     H.set_profiling(False)
-    tim = mixed.timings()
+    tim = dict(mixed.timings())
+    tim["residual_norms_ms"] = float(np.median(res_ms))
     log(f"mixed pass {t_mixed * 1e3:.1f} ms (filter {tim['filter_ms']:.1f} / ortho "
-        f"{tim['ortho_ms']:.1f} / RR {tim['rr_ms']:.1f} ms)")
+        f"{tim['ortho_ms']:.1f} / RR {tim['rr_ms']:.1f} / residual norms "
+        f"{tim['residual_norms_ms']:.1f} ms)")
 
     # ---- FP64 passes from the same start (reported beside; not the timed region)
     fp64 = dfx.ChebyshevFilteredEigenSolver(ctx, bnd64[0], bnd64[1], c, m, 1e-14, G64, True,
@@ -484,6 +555,13 @@ def main():
         serving = serving_run(args.serve_requests, args.serve_rate, args.serve_slots)
         log(f"served: {serving['solves_per_s']:.2f} solves/s, p50 "
             f"{serving['latency_p50_s']:.3f} s, p99 {serving['latency_p99_s']:.3f} s")
+        if args.serve_saturation:
+            log(f"saturation: {args.serve_requests} requests waiting at t = 0 ...")
+            sat = serving_run(args.serve_requests, args.serve_rate, args.serve_slots,
+                              saturate=True)
+            serving = {"stream": serving, "saturation": sat}
+            log(f"saturation: {sat['solves_per_s']:.2f} solves/s sustained, p50 "
+                f"{sat['latency_p50_s']:.3f} s, p99 {sat['latency_p99_s']:.3f} s")
 
     base = None
     if not args.no_cpu_baseline:
@@ -491,9 +569,26 @@ def main():
         base = cpu_baseline(prob, n, B, m)
         log(f"CPU baseline {base['value']:.4g} passes/s on {base['cores']} threads")
 
-    per_launch_flops = prof32["flops"] / max(prof32["launches"], 1)
-    per_launch_bytes = prof32["bytes"] / max(prof32["launches"], 1)
-    per_launch_ms = prof32["kernelMs"] / max(prof32["launches"], 1)
+    # ---- roofline of the dominant kernel: the FP32 cell apply of the filter's
+    # degree loop.  A launch unit is one colour step (its size-class launches
+    # side by side), timed with HIP events on the library's stream over the
+    # timed passes.  Algorithmic flops 2 B sum n_e^2 per apply; algorithmic
+    # bytes the fused minimum per apply, s sum n_e^2 + 4 (sum n_e + E) +
+    # 2 s N B + constraint bytes; both split evenly over the colours.
+    nrows = len(prob.get("rows"))
+    cons_bytes = 4.0 * (2 * nrows + prob.nEntries) + 8.0 * prob.nEntries
+    apply_widths = [bw for bw in batch_widths(n, B) for _ in range(m - 1)] * args.steps
+    flops_min = sum(2.0 * bw * info["sumNe2"] for bw in apply_widths)
+    bytes_min = sum(4.0 * info["sumNe2"] + 4.0 * (info["sumNe"] + info["nCells"]) +
+                    2.0 * 4 * N * bw + cons_bytes + 4.0 * nrows * bw for bw in apply_widths)
+    launches = prof32["launches"]
+    if launches != len(apply_widths) * info["nColors"] or \
+            abs(prof32["flops"] - flops_min) > 1e-9 * flops_min:
+        log(f"warning: profiled launches {launches} / flops {prof32['flops']:.4g} differ from "
+            f"the expected {len(apply_widths) * info['nColors']} / {flops_min:.4g}")
+    per_launch_flops = flops_min / max(launches, 1)
+    per_launch_bytes = bytes_min / max(launches, 1)
+    per_launch_ms = prof32["kernelMs"] / max(launches, 1)
     # the roofline that bounds the kernel is the one with the longer ideal time
     t_mfma = per_launch_flops / (PEAK_FP32_MFMA * 1e12)
     t_hbm = per_launch_bytes / (PEAK_HBM * 1e9)
@@ -543,10 +638,15 @@ def main():
                                 "launches running side by side"),
                      "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic,
+                     "traffic_over_algorithmic": (traffic / per_launch_bytes
+                                                  if traffic else None),
+                     "ideal_us_mfma_hbm": [t_mfma * 1e6, t_hbm * 1e6],
                      "launches": prof32["launches"],
                      "avg_launch_ms": per_launch_ms,
                      "algorithmic_flops_per_launch": per_launch_flops,
                      "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "achieved_gbps": per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+                                      if per_launch_ms else 0.0,
                      "achieved_tflops": (per_launch_flops / (per_launch_ms * 1e-3) / 1e12
                                          if per_launch_ms else 0.0),
                      "frac_of_fp32_mfma_peak": (per_launch_flops / (per_launch_ms * 1e-3)

@@ -36,12 +36,15 @@ namespace dfx
   // (src/ksdft/KohnShamOperatorContextFE.t.cpp:1326-1415)
   // dsc / xe: see applyInvThenCellwise (classical rows gathered as dsc o X,
   // enriched rows from xe; X already constrained)
+  // skipP2C: X is used as given (the caller has applied the parent->child
+  // step of other constraints, as LaplaceOperatorContextFE::apply does with
+  // its X manager's constraints, LaplaceOperatorContextFE.t.cpp:433-436)
   template <class T>
   void applyCellwise(dfx_op *op, View<T> X, View<T> Y, const double *dsc = nullptr,
-                     const T *xe = nullptr)
+                     const T *xe = nullptr, bool skipP2C = false)
   {
     hipStream_t s = op->ctx->stream;
-    if (!dsc)
+    if (!dsc && !skipP2C)
       k::p2c<T>(s, op->cons, X);
     // every classical row is stored by the first cell that reaches it and every
     // enriched row by the enriched reduction, so Y is cleared only when some
@@ -51,7 +54,8 @@ namespace dfx
     T *yE = nullptr;
     if (op->nEnrSlots)
       {
-        op->yE.ensure((size_t)op->nEnrSlots * X.B * sizeof(double));
+        if (op->yE.ensure((size_t)op->nEnrSlots * X.B * sizeof(double)))
+          ++op->gen;
         yE = op->yE.as<T>();
       }
     const T *frag = fragOf<T>(op);
@@ -151,7 +155,8 @@ namespace dfx
     T *yE = nullptr;
     if (op->nEnrSlots)
       {
-        op->yE.ensure((size_t)op->nEnrSlots * B * sizeof(double));
+        if (op->yE.ensure((size_t)op->nEnrSlots * B * sizeof(double)))
+          ++op->gen;
         yE = op->yE.as<T>();
       }
     View<const T> X{xb.as<T>(), N, B, B};
@@ -257,6 +262,7 @@ namespace dfx
         k::gatherDiag(s, Dinv->diag.as<double>(), A->map.as<u32>(), A->sumNe,
                       A->nClassical, A->foldedDiag.as<double>());
         A->foldedUid = Dinv->uid;
+        ++A->gen;
       }
     k::p2c<T>(s, Dinv->cons, X);
     if (Dinv->nEnriched)
@@ -319,6 +325,31 @@ namespace dfx
     dfx_ctx *ctx = c->ctx;
     delete c;
     ctxRelease(ctx);
+  }
+
+  template <class T>
+  void applyCellwiseNoP2C(dfx_op *op, View<T> X, View<T> Y)
+  {
+    DFX_REQUIRE(op->kind == OP_CELLWISE, "a cell-wise operator is required");
+    DFX_REQUIRE(X.N == op->nDofs && Y.N == op->nDofs, "apply: N does not match the operator");
+    DFX_REQUIRE(X.B == Y.B, "apply: X and Y have different widths");
+    applyCellwise<T>(op, X, Y, nullptr, nullptr, true);
+  }
+  template void applyCellwiseNoP2C<double>(dfx_op *, View<double>, View<double>);
+
+  // sum_e Q_e^T diag(A_e) (no constraint step): each cell's diagonal to its
+  // map slots, then added dof by dof in cell order
+  void cellDiagonal(dfx_op *op, double *out)
+  {
+    DFX_REQUIRE(op->kind == OP_CELLWISE, "a cell-wise operator is required");
+    hipStream_t s = op->ctx->stream;
+    DevBuf slots(op->sumNe * sizeof(double));
+    k::cellDiag(s, op, slots.as<double>());
+    View<double> o{out, op->nDofs, 1, 1};
+    k::fill<double>(s, o, 0.0);
+    k::scatterAdd<double>(s, op->dofSlotOffsets.as<u32>(), op->dofSlots.as<u32>(), op->nDofs,
+                          View<const double>{slots.as<double>(), (u32)op->sumNe, 1, 1}, o);
+    DFX_HIP(hipStreamSynchronize(s));
   }
 
   template void applyOp<double>(dfx_op *, View<double>, View<double>);
@@ -392,6 +423,25 @@ extern "C"
     if (!ctx || !stream)
       return DFX_E_ARG;
     *stream = (void *)ctx->stream;
+    return 0;
+  }
+
+  int dfx_ctx_graph_stats(const dfx_ctx *ctx, uint64_t *captures, uint64_t *replays,
+                          uint32_t *cached)
+  {
+    if (!ctx)
+      return DFX_E_ARG;
+    if (captures)
+      *captures = ctx->graphCaptures;
+    if (replays)
+      *replays = ctx->graphReplays;
+    if (cached)
+      {
+        uint32_t n = 0;
+        for (const auto &g : ctx->graphs)
+          n += g.second.exec != nullptr;
+        *cached = n;
+      }
     return 0;
   }
 
@@ -860,6 +910,36 @@ extern "C"
     });
   }
 
+  int dfx_op_apply_after_inverse(dfx_op *A, dfx_op *BInv, dfx_mv *X, dfx_mv *Y)
+  {
+    if (!A || !BInv || !X || !Y)
+      return DFX_E_ARG;
+    return guard(A->ctx, [&] {
+      DFX_REQUIRE(X->dtype == Y->dtype, "apply: X and Y dtypes differ");
+      DFX_REQUIRE(X->N == A->nDofs && Y->N == A->nDofs && BInv->nDofs == A->nDofs,
+                  "apply: N does not match the operators");
+      DFX_REQUIRE(X->B == Y->B, "apply: X and Y have different widths");
+      auto run = [&](auto tag) {
+        using T = decltype(tag);
+        // scratch1 of the reference (all of BInv X on the unfused path, its
+        // enriched rows on the folded one)
+        DevBuf &sb = work(A->ctx, "aai.s1", (size_t)X->N * X->B * sizeof(T));
+        View<T> S1{sb.as<T>(), X->N, X->B, X->B};
+        View<T> Xv = view<T>(X), Yv = view<T>(Y);
+        if (!applyInvThenCellwise<T>(BInv, A, Xv, S1, Yv))
+          {
+            applyOp<T>(BInv, Xv, S1);
+            applyOp<T>(A, S1, Yv);
+          }
+      };
+      if (X->dtype == DFX_F64)
+        run(double{});
+      else
+        run(float{});
+      return 0;
+    });
+  }
+
   int dfx_op_destroy(dfx_op *op)
   {
     if (!op)
@@ -868,6 +948,23 @@ extern "C"
     dfx_constraints *cons = op->cons;
     if (ctx && ctx->stream)
       (void)hipStreamSynchronize(ctx->stream);
+    if (ctx)
+      {
+        // captured graphs that launch on this operator hold its buffers
+        CaptureLock lk(captureMutex());
+        for (auto it = ctx->graphs.begin(); it != ctx->graphs.end();)
+          {
+            const auto &u = it->second.uids;
+            if (std::find(u.begin(), u.end(), op->uid) != u.end())
+              {
+                if (it->second.exec)
+                  (void)hipGraphExecDestroy(it->second.exec);
+                it = ctx->graphs.erase(it);
+              }
+            else
+              ++it;
+          }
+      }
     delete op;
     consRelease(cons);
     ctxRelease(ctx);

@@ -67,6 +67,15 @@ namespace dfx
   void applyOp(dfx_op *op, View<T> X, View<T> Y);
   template <class T>
   bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y);
+  // Y = C2P(sum_e Q_e^T A_e Q_e X) with X used as given (no parent->child step)
+  template <class T>
+  void applyCellwiseNoP2C(dfx_op *op, View<T> X, View<T> Y);
+  // out (device, nDofs) = sum_e Q_e^T diag(A_e), no constraint step
+  void cellDiagonal(dfx_op *op, double *out);
+  // FEBasisOperations::integrateWithBasisValues on the problem's basis
+  // (out = C2P(sum_e Q_e^T N_e^T (JxW o f_e)); cons NULL: no C2P)
+  void problemIntegrate(dfx_problem *P, const dfx_constraints *cons, View<const double> f,
+                        View<double> out);
 
   // column dots of two device multivectors -> host (FP64), deterministic
   void colDotsHost(dfx_ctx *ctx, const dfx_mv *X, const dfx_mv *Y, double *out);

@@ -106,10 +106,13 @@ namespace dfx
         }
       bytes = n;
     }
-    void ensure(size_t n)
+    // grows the buffer (contents lost); true when it was reallocated
+    bool ensure(size_t n)
     {
-      if (n > bytes)
-        alloc(n);
+      if (n <= bytes)
+        return false;
+      alloc(n);
+      return true;
     }
     void release()
     {
@@ -174,13 +177,17 @@ struct dfx_ctx
   dfx::DevBuf    reducePartials;
   std::map<std::string, dfx::DevBuf> work; // named solver scratch
   // captured degree loops of the Chebyshev filter, by name; `key` holds the
-  // buffers, sizes and scalars the graph was captured with
+  // operators (uid and buffer generation), buffers, sizes and scalars the
+  // graph was captured with; `uids` the operators it launches on, so that a
+  // destroyed operator's graphs are released with it
   struct Graph
   {
     hipGraphExec_t        exec = nullptr;
     std::vector<uint64_t> key;
+    std::vector<uint64_t> uids;
   };
   std::map<std::string, Graph> graphs;
+  uint64_t graphCaptures = 0, graphReplays = 0;
 };
 
 struct dfx_mv
@@ -256,6 +263,11 @@ struct dfx_op
   int          kind = dfx::OP_CELLWISE;
   // unique per operator object for the life of the process (cache keys)
   uint64_t     uid  = nextUid();
+  // bumped whenever a device buffer that the operator's launches read or
+  // write is (re)allocated (cell matrices, enriched partials, folded
+  // diagonal): a captured graph holds those addresses, so it is valid only
+  // for the generation it was captured in
+  uint64_t     gen  = 0;
   static uint64_t nextUid()
   {
     static std::atomic<uint64_t> n{1};

@@ -968,6 +968,34 @@ namespace dfx
       checkLaunch("enrichedReduce");
     }
 
+    namespace
+    {
+      // one block per cell (launch order): diagonal entry i of the cell's
+      // FP64 fragment-ordered matrix.  Entry (lane l, value v) of block
+      // (mt, kb) holds A_e^T[mt 16 + l % 16][kb 8 + 2 (l / 16) + v], so A_e(i, i)
+      // sits at mt = i / 16, kb = i / 8, l = 16 ((i % 8) / 2) + i % 16, v = i % 2
+      __global__ void cellDiagK(const CellMeta *__restrict__ meta,
+                                const double *__restrict__ frag, double *__restrict__ out)
+      {
+        const CellMeta cm  = meta[blockIdx.x];
+        const u32      nkb = (cm.n + 7) / 8;
+        for (u32 i = threadIdx.x; i < cm.n; i += blockDim.x)
+          {
+            const u32 mt = i / 16, kb = i / 8, l = 16 * ((i % 8) / 2) + i % 16, v = i % 2;
+            out[cm.mapOff + i] = frag[cm.fragOff64 + ((u64)(mt * nkb + kb) * 64 + l) * 2 + v];
+          }
+      }
+    } // namespace
+
+    void cellDiag(hipStream_t s, const dfx_op *op, double *out)
+    {
+      if (!op->nCells)
+        return;
+      cellDiagK<<<op->nCells, 256, 0, s>>>(op->meta.as<CellMeta>(), op->frag64.as<double>(),
+                                           out);
+      checkLaunch("cellDiag");
+    }
+
     template <class T>
     void repackCells(hipStream_t s, const dfx_op *op, const double *raw,
                      const u64 *rawOffsets, T *frag)

@@ -105,6 +105,9 @@ namespace dfx
     // y[nC + g, j] = sum over the enriched dof's slots of yE[slot, j]
     template <class T>
     void enrichedReduce(hipStream_t s, const dfx_op *op, const T *yE, View<T> y);
+    // out[mapOff_e + i] = A_e(i, i) from the FP64 fragment-ordered matrices
+    // (caller slot order)
+    void cellDiag(hipStream_t s, const dfx_op *op, double *out);
     // repack raw column-major cell matrices (device) into the fragment layout
     template <class T>
     void repackCells(hipStream_t s, const dfx_op *op, const double *raw,

@@ -302,6 +302,7 @@ namespace dfx
         dmats = raw.as<double>();
       }
     upload(rawOff, op->hMatOffset, st);
+    ++op->gen;
     if (!op->frag64.p)
       op->frag64.alloc(op->fragLen64 * sizeof(double));
     k::repackCells<double>(st, op, dmats, rawOff.as<u64>(), op->frag64.as<double>());

@@ -17,6 +17,9 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <exception>
+#include <new>
+#include <stdexcept>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -108,17 +111,14 @@ namespace dfx
       int                     firstError = 0;
     };
 
-    // one request on its own context: build, operators, seeded guess, KS solve
-    int runRequest(int device, int precision, bool useGraph, const dfx_request &r,
+    // one request on its slot's context: build, operators, seeded guess, KS
+    // solve; everything the request allocated (its objects and the
+    // context's named work buffers) is released at the end, the context
+    // itself (stream, rocBLAS handle) serves the slot's next request
+    int runRequest(dfx_ctx *ctx, int precision, bool useGraph, const dfx_request &r,
                    double *energies, dfx_request_result &out)
     {
-      dfx_ctx *ctx = nullptr;
-      int      rc  = dfx_ctx_create(device, &ctx);
-      if (rc != 0)
-        return rc;
-      // slots already run side by side: one stream per request keeps the
-      // process within the device's hardware queues
-      ctx->forkClasses = false;
+      int              rc = 0;
       dfx_problem     *P = nullptr;
       dfx_op          *H = nullptr, *M = nullptr, *K = nullptr, *Mi = nullptr, *Ml = nullptr;
       dfx_constraints *cons = nullptr;
@@ -181,8 +181,37 @@ namespace dfx
         dfx_constraints_destroy(cons);
       if (P)
         dfx_problem_destroy(P);
-      dfx_ctx_destroy(ctx);
+      (void)hipStreamSynchronize(ctx->stream);
+      {
+        CaptureLock lk(captureMutex());
+        ctx->work.clear();
+      }
       return rc;
+    }
+
+    // status of a request whose run threw (no exception crosses the ABI)
+    int statusOf(std::exception_ptr e)
+    {
+      try
+        {
+          std::rethrow_exception(e);
+        }
+      catch (const Error &x)
+        {
+          return x.code;
+        }
+      catch (const std::bad_alloc &)
+        {
+          return DFX_E_OOM;
+        }
+      catch (const std::invalid_argument &)
+        {
+          return DFX_E_ARG;
+        }
+      catch (...)
+        {
+          return DFX_E_STATE;
+        }
     }
   } // namespace
 } // namespace dfx
@@ -230,6 +259,14 @@ extern "C"
       return std::chrono::duration<double>(t - t0).count();
     };
     auto worker = [&](u32 slot) {
+      // the slot's context: one stream and rocBLAS handle for all its
+      // requests.  Slots already run side by side, so each uses one stream
+      // (no second stream for size classes), keeping the process within the
+      // device's hardware queues
+      dfx_ctx  *ctx   = nullptr;
+      const int ctxRc = dfx_ctx_create(p->device, &ctx);
+      if (ctxRc == 0)
+        ctx->forkClasses = false;
       for (;;)
         {
           u32 idx;
@@ -238,7 +275,12 @@ extern "C"
             for (;;)
               {
                 if (sh.next >= nRequests)
-                  return;
+                  {
+                    lk.unlock();
+                    if (ctx)
+                      dfx_ctx_destroy(ctx);
+                    return;
+                  }
                 idx = order[sh.next];
                 const auto arrive =
                   t0 + std::chrono::duration_cast<clk::duration>(
@@ -260,15 +302,18 @@ extern "C"
           out.slot                = slot;
           out.footprintBytes      = fp[idx];
           out.admitSeconds        = since(clk::now());
-          try
-            {
-              out.status = runRequest(p->device, p->precision, p->useGraph != 0, reqs[idx],
-                                      energies + energyOff[idx], out);
-            }
-          catch (...)
-            {
-              out.status = DFX_E_OOM;
-            }
+          if (!ctx)
+            out.status = ctxRc;
+          else
+            try
+              {
+                out.status = runRequest(ctx, p->precision, p->useGraph != 0, reqs[idx],
+                                        energies + energyOff[idx], out);
+              }
+            catch (...)
+              {
+                out.status = statusOf(std::current_exception());
+              }
           out.doneSeconds = since(clk::now());
           {
             std::lock_guard<std::mutex> lk(sh.mu);

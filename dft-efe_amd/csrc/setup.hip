@@ -825,8 +825,10 @@ namespace
   }
 
   // quadrature layout of the problem's cells: cell c's points are
-  // [off[c], off[c + 1]), x fastest; W (optional) = JxW of every point
-  void quadLayout(const dfx_problem *P, std::vector<u64> &off, std::vector<double> *W)
+  // [off[c], off[c + 1]), x fastest; W (optional) = JxW of every point,
+  // XYZ (optional) = coordinates of every point (x, y, z per point)
+  void quadLayout(const dfx_problem *P, std::vector<u64> &off, std::vector<double> *W,
+                  std::vector<double> *XYZ = nullptr)
   {
     const auto &S   = P->spec;
     const u32   np  = P->p + 1, ncl = np * np * np;
@@ -837,11 +839,27 @@ namespace
     off.assign(P->nCells + 1, 0);
     if (W)
       W->clear();
+    if (XYZ)
+      XYZ->clear();
     for (u32 c = 0; c < P->nCells; ++c)
       {
         const bool enr = P->cellNdofs[c] > ncl;
         const u32  nq  = enr ? nqE : nqC;
         off[c + 1]     = off[c] + (u64)nq * nq * nq;
+        if (XYZ)
+          {
+            // reference point x_q in [-1, 1] maps to lo + h (x_q + 1) / 2
+            const auto   &x  = enr ? xE : xC;
+            const double *lo = &P->cellLo[3 * (size_t)c], *h = &P->cellH[3 * (size_t)c];
+            for (u32 qc = 0; qc < nq; ++qc)
+              for (u32 qb = 0; qb < nq; ++qb)
+                for (u32 qa = 0; qa < nq; ++qa)
+                  {
+                    XYZ->push_back(lo[0] + h[0] * (0.5 * (x[qa] + 1.0)));
+                    XYZ->push_back(lo[1] + h[1] * (0.5 * (x[qb] + 1.0)));
+                    XYZ->push_back(lo[2] + h[2] * (0.5 * (x[qc] + 1.0)));
+                  }
+          }
         if (!W)
           continue;
         const auto  &w    = enr ? wE : wC;
@@ -929,6 +947,122 @@ namespace
                     });
   }
 
+  // fw[(t nQ + q) B + j] = JxW[q] f[(quadOff[c] + q) ldf + j] for the cells
+  // c = cells[t] of one batch: the inp o JxW product of
+  // FEBasisOperations::integrateWithBasisValues (FEBasisOperations.t.cpp:
+  // 2043-2059), JxW first
+  __global__ void weightQuadRowsK(const double *f, u32 ldf, u32 B, const double *W,
+                                  const u64 *quadOff, const u32 *cells, u32 nQ, double *fw)
+  {
+    const u32 t = blockIdx.y, c = cells[t];
+    const u64 q0 = quadOff[c];
+    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < nQ * B;
+         e += gridDim.x * blockDim.x)
+      {
+        const u32 q = e / B, j = e - q * B;
+        fw[((u64)t * nQ + q) * B + j] = W[q0 + q] * f[(q0 + q) * ldf + j];
+      }
+  }
+} // namespace
+
+namespace dfx
+{
+  // FEBasisOperations::integrateWithBasisValues (src/basis/FEBasisOperations.t.cpp:
+  // 1846-2190): out = C2P(sum_e Q_e^T N_e^T (JxW o f_e)) with the problem's
+  // basis (orthogonalized enrichment included) re-tabulated at its quadrature
+  // points.  Per batch of cells of one shape the weighted quadrature rows are
+  // formed and C_e (B x n_e) = fw_e (B x nQ) F_e (nQ x n_e) is one batched
+  // GEMM written straight into the cell's slots of the cell->dof map; the
+  // slots are then added into `out` dof by dof in cell order
+  // (addCellWiseDataToFieldData, FECellWiseDataOperations.t.cpp:89-151) and
+  // the constraints (when given) distribute child rows to their parents.
+  void problemIntegrate(dfx_problem *P, const dfx_constraints *cons, View<const double> f,
+                        View<double> out)
+  {
+    dfx_ctx    *ctx = P->ctx;
+    hipStream_t s   = ctx->stream;
+    const u32   B   = f.B;
+    std::vector<u64>    qoff;
+    std::vector<double> W;
+    quadLayout(P, qoff, &W);
+    DFX_REQUIRE(f.N == qoff.back(), "integrate: one row per quadrature point expected");
+    DFX_REQUIRE(out.N == P->nDofs && out.B == B, "integrate: output must be nDofs x B");
+    const u64 nSlots = P->cellMap.size();
+    // dof -> slots of the cell->dof map, in cell order
+    std::vector<u32> sOff(P->nDofs + 1, 0), sList(nSlots);
+    for (u64 t = 0; t < nSlots; ++t)
+      ++sOff[P->cellMap[t] + 1];
+    for (u32 d = 0; d < P->nDofs; ++d)
+      sOff[d + 1] += sOff[d];
+    {
+      std::vector<u32> fp(sOff.begin(), sOff.end() - 1);
+      for (u64 t = 0; t < nSlots; ++t)
+        sList[fp[P->cellMap[t]]++] = (u32)t;
+    }
+    DevBuf dQoff, dW, dSOff, dSList;
+    upload(dQoff, qoff, s);
+    upload(dW, W, s);
+    upload(dSOff, sOff, s);
+    upload(dSList, sList, s);
+    DevBuf &bSlots = work(ctx, "integ.slots", nSlots * B * sizeof(double));
+    u64 maxQ = 0;
+    for (u32 c = 0; c < P->nCells; ++c)
+      maxQ = std::max(maxQ, qoff[c + 1] - qoff[c]);
+    std::vector<u32> all(P->nCells);
+    std::iota(all.begin(), all.end(), 0u);
+    const double *alpha = P->alpha.p ? P->alpha.as<double>() : nullptr;
+    tabulateBatches(P, all, alpha, (size_t)maxQ * B * sizeof(double),
+                    [&](const u32 *cells, u32 cnt, u32 n, u64 nQ, DevBuf &bF, DevBuf &,
+                        DevBuf &, DevBuf &, DevBuf &) {
+                      DevBuf &bFw = work(ctx, "integ.fw", (size_t)cnt * nQ * B * 8);
+                      DevBuf  dCells;
+                      upload(dCells, cells, cnt, s);
+                      weightQuadRowsK<<<dim3((u32)((nQ * B + 255) / 256), cnt), 256, 0, s>>>(
+                        f.p, f.ld, B, dW.as<double>(), dQoff.as<u64>(), dCells.as<u32>(),
+                        (u32)nQ, bFw.as<double>());
+                      checkLaunch("weightQuadRows");
+                      std::vector<const double *> pA(cnt), pB(cnt);
+                      std::vector<double *>       pC(cnt);
+                      for (u32 t = 0; t < cnt; ++t)
+                        {
+                          pA[t] = bFw.as<double>() + (u64)t * nQ * B;
+                          pB[t] = bF.as<double>() + (u64)t * nQ * n;
+                          pC[t] = bSlots.as<double>() + (u64)P->mapOff[cells[t]] * B;
+                        }
+                      DevBuf dpA, dpB, dpC;
+                      upload(dpA, pA, s);
+                      upload(dpB, pB, s);
+                      upload(dpC, pC, s);
+                      const double one = 1.0, zero = 0.0;
+                      {
+                        CaptureLock lk(captureMutex());
+                        rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+                        // C (B x n, the cell's slot rows) = fw (B x nQ) F (nQ x n)
+                        if (rocblas_dgemm_batched(ctx->blas, rocblas_operation_none,
+                                                  rocblas_operation_none, (rocblas_int)B,
+                                                  (rocblas_int)n, (rocblas_int)nQ, &one,
+                                                  dpA.as<const double *>(), (rocblas_int)B,
+                                                  dpB.as<const double *>(), (rocblas_int)nQ,
+                                                  &zero, dpC.as<double *>(), (rocblas_int)B,
+                                                  (rocblas_int)cnt) != rocblas_status_success)
+                          throw Error(DFX_E_HIP, "rocblas_dgemm_batched failed");
+                      }
+                      // the batch's pointer arrays are released when it returns
+                      DFX_HIP(hipStreamSynchronize(s));
+                    });
+    k::fill<double>(s, out, 0.0);
+    k::scatterAdd<double>(s, dSOff.as<u32>(), dSList.as<u32>(), P->nDofs,
+                          View<const double>{bSlots.as<double>(), (u32)nSlots, B, B}, out);
+    if (cons)
+      k::c2p<double>(s, cons, out);
+    DFX_HIP(hipStreamSynchronize(s));
+    ctx->work.erase("integ.fw");
+    ctx->work.erase("integ.slots");
+  }
+} // namespace dfx
+
+namespace
+{
   // Per-atom L2 projection of the enrichment functions on the classical
   // nodes interior to the atom's support, with a cell-wise mass operator over
   // the support cells (non-interior nodes as zero Dirichlet rows) and PCG.
@@ -1299,6 +1433,13 @@ extern "C"
               quadLayout(P, off, &w);
               return cpu(w.data(), w.size() * 8);
             }
+          case DFX_PROB_QUAD_XYZ:
+            {
+              std::vector<u64>    off;
+              std::vector<double> xyz;
+              quadLayout(P, off, nullptr, &xyz);
+              return cpu(xyz.data(), xyz.size() * 8);
+            }
         }
       throw Error(DFX_E_ARG, "unknown problem array");
     });
@@ -1316,6 +1457,19 @@ extern "C"
       density(P, occupation,
               View<const double>{static_cast<const double *>(X->d), X->N, X->B, X->B},
               static_cast<double *>(rho->d));
+      return 0;
+    });
+  }
+
+  int dfx_problem_integrate(dfx_problem *P, dfx_constraints *cons, dfx_mv *f, dfx_mv *out)
+  {
+    if (!P || !f || !out || f->dtype != DFX_F64 || out->dtype != DFX_F64 || !f->B)
+      return DFX_E_ARG;
+    return guard(P->ctx, [&] {
+      DFX_REQUIRE(!cons || cons->nDofs == P->nDofs, "constraints built for another N");
+      problemIntegrate(P, cons,
+                       View<const double>{static_cast<const double *>(f->d), f->N, f->B, f->B},
+                       View<double>{static_cast<double *>(out->d), out->N, out->B, out->B});
       return 0;
     });
   }

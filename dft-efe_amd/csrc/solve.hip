@@ -20,6 +20,10 @@ struct dfx_chfsi
   dfx_chfsi_params p{};
   dfx_mv          *guess = nullptr;
   double           ms[3] = {0, 0, 0};
+  // storeIntermediateSubspaces: filtered and orthonormalized-filtered
+  // subspaces of the last solve (N x n FP64)
+  dfx::DevBuf      filtered, filteredOrtho;
+  dfx::u32         storedN = 0, storedB = 0;
 };
 
 namespace dfx
@@ -153,8 +157,8 @@ namespace dfx
     // graph is replayed without running body.
     template <class Body, class Replay>
     void degreeLoop(dfx_ctx *ctx, bool useGraph, const std::string &name,
-                    const std::vector<uint64_t> &key, u32 first, u32 last, Body body,
-                    Replay replayHost)
+                    const std::vector<uint64_t> &key, const std::vector<uint64_t> &uids,
+                    u32 first, u32 last, Body body, Replay replayHost)
     {
       if (!useGraph || first > last)
         {
@@ -168,6 +172,7 @@ namespace dfx
         {
           replayHost();
           DFX_HIP(hipGraphLaunch(g.exec, s));
+          ++ctx->graphReplays;
           return;
         }
       if (g.exec)
@@ -197,6 +202,8 @@ namespace dfx
       DFX_HIP(e);
       g.exec = exec;
       g.key  = key;
+      g.uids = uids;
+      ++ctx->graphCaptures;
       DFX_HIP(hipGraphLaunch(g.exec, s));
     }
 
@@ -284,15 +291,16 @@ namespace dfx
       const bool graph = useGraph && !A->profiling && !BInv->profiling;
       // the captured launches depend on operators, buffers and shapes only
       // (coefficients and mu come from dtab), so one graph per block width
-      // serves every pass of a solve
-      const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)A, (uint64_t)(uintptr_t)BInv,
+      // serves every pass of a solve; an operator whose buffers were
+      // reallocated since (its generation) is captured again
+      const std::vector<uint64_t> key = {A->uid, A->gen, BInv->uid, BInv->gen,
                                          (uint64_t)(uintptr_t)rn.p, (uint64_t)(uintptr_t)rs.p,
                                          (uint64_t)(uintptr_t)s1.p, (uint64_t)(uintptr_t)s2.p,
                                          (uint64_t)(uintptr_t)r0.p, (uint64_t)(uintptr_t)dtab,
                                          N, nb, m};
       const std::string gname = std::string(sizeof(T) == 8 ? "rf.f64." : "rf.f32.") +
                                 std::to_string(nb);
-      degreeLoop(ctx, graph, gname, key, 3, m, step, [&] {
+      degreeLoop(ctx, graph, gname, key, {A->uid, BInv->uid}, 3, m, step, [&] {
         for (u32 d = 3; d <= m; ++d)
           std::swap(rn, rs);
       });
@@ -448,9 +456,22 @@ namespace dfx
             }
         }
       cs->ms[0] = tf.stop();
+      const size_t subBytes = (size_t)XV.N * n * sizeof(double);
+      if (p.storeIntermediateSubspaces)
+        {
+          cs->filtered.ensure(subBytes);
+          cs->filteredOrtho.ensure(subBytes);
+          cs->storedN = XV.N;
+          cs->storedB = n;
+          DFX_HIP(hipMemcpyAsync(cs->filtered.p, XV.p, subBytes, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+        }
       Timer to(ctx->stream);
       int oc = cholGS(ctx, XV, G, Bop);
       cs->ms[1] = to.stop();
+      if (p.storeIntermediateSubspaces && oc == 0)
+        DFX_HIP(hipMemcpyAsync(cs->filteredOrtho.p, G.p, subBytes, hipMemcpyDeviceToDevice,
+                               ctx->stream));
       if (oc != 0)
         return DFX_EIG_CHFSI_ORTHONORMALIZATION_ERROR;
       Timer tr(ctx->stream);
@@ -636,6 +657,11 @@ extern "C"
     return guard(ctx, [&] {
       DFX_REQUIRE(p && guess && out, "null argument");
       DFX_REQUIRE(guess->dtype == DFX_F64, "the subspace guess must be FP64");
+      if (p->orthogonalizationType == DFX_ORTHO_MULTIPASS_LOWDIN)
+        throw Error(DFX_E_UNSUPPORTED, "Orthogonalization type not present");
+      if (p->orthogonalizationType != DFX_ORTHO_CHOLESKY_GRAMSCHMIDT)
+        throw Error(DFX_E_UNSUPPORTED,
+                    "only CHOLESKY_GRAMSCHMIDT orthogonalization is implemented");
       auto cs   = new dfx_chfsi();
       cs->ctx   = ctx;
       cs->p     = *p;
@@ -668,6 +694,25 @@ extern "C"
       return DFX_E_ARG;
     (void)computeEV;
     return guard(cs->ctx, [&] { return chfsiSolve(cs, A, eig, X, Bop, BInv); });
+  }
+
+  int dfx_chfsi_get_filtered_subspace(const dfx_chfsi *cs, int ortho, dfx_mv *out)
+  {
+    if (!cs || !out)
+      return DFX_E_ARG;
+    return guard(cs->ctx, [&] {
+      if (!cs->p.storeIntermediateSubspaces)
+        throw Error(DFX_E_STATE, "storeIntermediateSubspaces is false in CHFSI class. Cannot "
+                                 "return the filtered Subspace.");
+      DFX_REQUIRE(cs->storedN, "no solve has run yet");
+      DFX_REQUIRE(out->dtype == DFX_F64 && out->N == cs->storedN && out->B == cs->storedB,
+                  "output must be FP64 with the eigenvectors' shape");
+      const DevBuf &b = ortho ? cs->filteredOrtho : cs->filtered;
+      DFX_HIP(hipMemcpyAsync(out->d, b.p, (size_t)cs->storedN * cs->storedB * sizeof(double),
+                             hipMemcpyDeviceToDevice, cs->ctx->stream));
+      DFX_HIP(hipStreamSynchronize(cs->ctx->stream));
+      return 0;
+    });
   }
 
   int dfx_chfsi_timings(const dfx_chfsi *cs, double *ms3)
@@ -839,6 +884,19 @@ extern "C"
   // formed on the device from device-resident dots (the same IEEE divisions
   // as on the host), so one iteration synchronizes once, for the residual
   // norms of the convergence test.
+  int dfx_eigen_residual_norms(dfx_ctx *ctx, dfx_op *H, dfx_op *M, dfx_mv *X,
+                               const double *eigenValues, double *residualNorms)
+  {
+    if (!ctx || !H || !M || !X || !eigenValues || !residualNorms)
+      return DFX_E_ARG;
+    return guard(ctx, [&] {
+      View<double> x = mvView(X);
+      DFX_REQUIRE(H->nDofs == x.N && M->nDofs == x.N, "operator size does not match X");
+      eigenResidualNorms(ctx, H, M, x, eigenValues, residualNorms);
+      return 0;
+    });
+  }
+
   int dfx_cg_solve(dfx_ctx *ctx, dfx_op *A, dfx_op *pc, dfx_mv *bmv, dfx_mv *xmv,
                    double absTol, double relTol, double divTol, uint32_t maxIter,
                    uint32_t *iterations, double *residualNorms)

@@ -27,6 +27,7 @@ LIB_PATH = os.environ.get("DFX_LIBRARY") or os.path.join(_HERE, "libdfx.so")
 F64, F32 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 OP_KEEP_FP32 = 0x1
+PC_NONE, PC_JACOBI = 0, 1   # linearAlgebra::PreconditionerType
 
 EIG_CODES = {0: "SUCCESS", 1: "LAPACK_ERROR", 2: "LANCZOS_BETA_ZERO",
              3: "LANCZOS_SUBSPACE_INSUFFICIENT", 4: "CHFSI_ORTHONORMALIZATION_ERROR",
@@ -69,6 +70,7 @@ _sig("dfx_ctx_destroy", _vp)
 _sig("dfx_ctx_last_error", _vp, res=C.c_char_p)
 _sig("dfx_ctx_synchronize", _vp)
 _sig("dfx_ctx_stream", _vp, _pp)
+_sig("dfx_ctx_graph_stats", _vp, _vp, _vp, _vp)
 _sig("dfx_mv_create", _vp, _u32, _u32, C.c_int, _pp)
 _sig("dfx_mv_wrap", _vp, _vp, _u32, _u32, C.c_int, _pp)
 _sig("dfx_mv_destroy", _vp)
@@ -93,6 +95,18 @@ _sig("dfx_op_create_cellwise", _vp, _u32, _vp, _vp, _vp, _u32, _u32, _vp, _u32, 
 _sig("dfx_op_update_cells", _vp, _vp)
 _sig("dfx_op_create_diagonal", _vp, _u32, _vp, _u32, _vp, _vp, _pp)
 _sig("dfx_op_apply", _vp, _vp, _vp)
+_sig("dfx_op_apply_after_inverse", _vp, _vp, _vp, _vp)
+_sig("dfx_problem_integrate", _vp, _vp, _vp, _vp)
+_sig("dfx_poisson_create", _vp, _vp, _vp, _vp, C.c_int, _pp)
+_sig("dfx_poisson_reinit", _vp, _vp, _vp)
+_sig("dfx_poisson_get_ax", _vp, _pp)
+_sig("dfx_poisson_get_pc", _vp, _pp)
+_sig("dfx_poisson_get_rhs", _vp, _vp)
+_sig("dfx_poisson_get_initial_guess", _vp, _vp)
+_sig("dfx_poisson_set_solution", _vp, _vp)
+_sig("dfx_poisson_get_solution", _vp, _vp)
+_sig("dfx_poisson_solve", _vp, _u32, _dbl, _dbl, _dbl, _vp, _vp, _vp)
+_sig("dfx_poisson_destroy", _vp)
 _sig("dfx_op_destroy", _vp)
 _sig("dfx_op_get_info", _vp, _vp)
 _sig("dfx_op_export_map", _vp, _vp, _vp)
@@ -102,11 +116,13 @@ _sig("dfx_chfsi_create", _vp, _vp, _vp, _pp)
 _sig("dfx_chfsi_reinit", _vp, _dbl, _dbl, _dbl, _u32, _dbl, _vp)
 _sig("dfx_chfsi_solve", _vp, _vp, _vp, _vp, C.c_int, _vp, _vp)
 _sig("dfx_chfsi_timings", _vp, _vp)
+_sig("dfx_chfsi_get_filtered_subspace", _vp, C.c_int, _vp)
 _sig("dfx_chfsi_destroy", _vp)
 _sig("dfx_cholesky_gram_schmidt", _vp, _vp, _vp, _vp)
 _sig("dfx_rayleigh_ritz", _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_lanczos_extreme", _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp)
 _sig("dfx_ks_eigensolve", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_eigen_residual_norms", _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_cg_solve", _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _u32, _vp, _vp)
 _sig("dfx_op_set_profiling", _vp, C.c_int)
 _sig("dfx_op_autotune", _vp, C.c_int, _u32, _u32, _u32, _vp, _vp, _vp)
@@ -171,6 +187,12 @@ class Context:
         s = C.c_void_p()
         _lib.dfx_ctx_stream(self.h, C.byref(s))
         return s.value or 0
+
+    def graph_stats(self):
+        """(captures, replays, cached graphs) of the captured degree loops."""
+        c, r, n = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self.check(_lib.dfx_ctx_graph_stats(self.h, C.byref(c), C.byref(r), C.byref(n)))
+        return int(c.value), int(r.value), int(n.value)
 
     def close(self):
         if getattr(self, "h", None):
@@ -333,6 +355,11 @@ class _Operator:
         reference OperatorContext::apply)."""
         self.ctx.check(_lib.dfx_op_apply(self.h, X.h, Y.h))
 
+    def apply_after_inverse(self, BInv, X, Y):
+        """Y = Op (BInv X), the residual filter's BInv.apply then A.apply
+        (BInv folded into this operator's gather when eligible)."""
+        self.ctx.check(_lib.dfx_op_apply_after_inverse(self.h, BInv.h, X.h, Y.h))
+
     def info(self):
         i = _OpInfo()
         _lib.dfx_op_get_info(self.h, C.byref(i))
@@ -436,7 +463,12 @@ class _ChfsiParams(C.Structure):
                 ("polynomialDegree", C.c_uint32), ("illConditionTolerance", C.c_double),
                 ("isResidualChebyshevFilter", C.c_int),
                 ("eigenVectorBatchSize", C.c_uint32), ("precision", C.c_int),
-                ("useGraph", C.c_int)]
+                ("useGraph", C.c_int), ("orthogonalizationType", C.c_int),
+                ("storeIntermediateSubspaces", C.c_int)]
+
+
+# linearAlgebra::OrthogonalizationType (ChebyshevFilteredEigenSolver.h:45-50)
+ORTHO_CHOLESKY_GRAMSCHMIDT, ORTHO_MULTIPASS_LOWDIN, ORTHO_MULTIPASS_CGS = 0, 1, 2
 
 
 class ChebyshevFilteredEigenSolver:
@@ -445,30 +477,50 @@ class ChebyshevFilteredEigenSolver:
     def __init__(self, ctx, wantedSpectrumLowerBound, wantedSpectrumUpperBound,
                  unWantedSpectrumUpperBound, polynomialDegree, illConditionTolerance,
                  eigenSubspaceGuess, isResidualChebyshevFilter=True,
-                 eigenVectorBatchSize=0, precision=PREC_FP64, useGraph=False):
+                 eigenVectorBatchSize=0, precision=PREC_FP64, useGraph=False,
+                 orthoType=ORTHO_CHOLESKY_GRAMSCHMIDT, storeIntermediateSubspaces=False):
         self.ctx = ctx
         p = _ChfsiParams(wantedSpectrumLowerBound, wantedSpectrumUpperBound,
                          unWantedSpectrumUpperBound, int(polynomialDegree),
                          illConditionTolerance, 1 if isResidualChebyshevFilter else 0,
-                         int(eigenVectorBatchSize), int(precision), 1 if useGraph else 0)
+                         int(eigenVectorBatchSize), int(precision), 1 if useGraph else 0,
+                         int(orthoType), 1 if storeIntermediateSubspaces else 0)
         h = C.c_void_p()
         ctx.check(_lib.dfx_chfsi_create(ctx.h, C.byref(p), eigenSubspaceGuess.h, C.byref(h)))
         self.h = h
         self.guess = eigenSubspaceGuess
 
     def reinit(self, a, b, c, degree, tol, guess=None):
+        self.ctx.check(_lib.dfx_chfsi_reinit(self.h, a, b, c, int(degree), tol,
+                                             guess.h if guess is not None else None))
         if guess is not None:
             self.guess = guess
-        _lib.dfx_chfsi_reinit(self.h, a, b, c, int(degree), tol,
-                              guess.h if guess is not None else None)
 
     def solve(self, A, eigenValues, eigenVectors, computeEigenVectors=True, B=None, BInv=None):
         """Returns the EigenSolverErrorCode; eigenValues (numpy, in/out) updated in place."""
         ev = eigenValues
-        assert ev.dtype == np.float64 and ev.flags.c_contiguous
+        if not (isinstance(ev, np.ndarray) and ev.dtype == np.float64 and ev.flags.c_contiguous
+                and ev.ndim == 1):
+            raise TypeError("eigenValues must be a contiguous 1-D float64 numpy array")
+        if ev.size != eigenVectors.B:
+            raise ValueError(f"eigenValues holds {ev.size} values for {eigenVectors.B} "
+                             "eigenvectors")
+        if B is None or BInv is None:
+            raise ValueError("B (overlap) and BInv (its inverse) operators are required")
         rc = _lib.dfx_chfsi_solve(self.h, A.h, _ptr(ev), eigenVectors.h,
                                   1 if computeEigenVectors else 0, B.h, BInv.h)
         return self.ctx.check(rc, allow_positive=True)
+
+    def _subspace(self, ortho):
+        out = MultiVector(self.ctx, self.guess.N, self.guess.B)
+        self.ctx.check(_lib.dfx_chfsi_get_filtered_subspace(self.h, ortho, out.h))
+        return out
+
+    def getFilteredSubspace(self):
+        return self._subspace(0)
+
+    def getOrthogonalizedFilteredSubspace(self):
+        return self._subspace(1)
 
     def timings(self):
         out = np.zeros(3)
@@ -495,6 +547,17 @@ def rayleigh_ritz(ctx, A, X, Xo):
     eig = np.zeros(X.B)
     rc = ctx.check(_lib.dfx_rayleigh_ritz(ctx.h, A.h, X.h, _ptr(eig), Xo.h), allow_positive=True)
     return rc, eig
+
+
+def eigen_residual_norms(ctx, H, M, X, eigenValues):
+    """||H x_j - lambda_j M x_j|| per column (KohnShamEigenSolver::
+    getLinearEigenSolveResidual)."""
+    ev = _f64a(eigenValues)
+    if ev.size != X.B:
+        raise ValueError(f"{ev.size} eigenvalues for {X.B} vectors")
+    out = np.zeros(X.B)
+    ctx.check(_lib.dfx_eigen_residual_norms(ctx.h, H.h, M.h, X.h, _ptr(ev), _ptr(out)))
+    return out
 
 
 def lanczos_extreme(ctx, A, B, BInv, guess, max_krylov=20):
@@ -566,7 +629,20 @@ class CGLinearSolver:
         self.iterations = 0
         self.residualNorms = None
 
-    def solve(self, A, pc, b, x):
+    def solve(self, *args):
+        """solve(linearSolverFunction) as the reference (CGLinearSolver.t.cpp:
+        68-313: the function's Ax / PC contexts, right-hand side and initial
+        guess; setSolution at the end), or solve(A, pc, b, x) with x the
+        initial guess on entry and the solution on return.  Returns the
+        LinearSolverErrorCode."""
+        if len(args) == 1:
+            fn = args[0]
+            b = fn.getRhs()
+            x = fn.getInitialGuess()
+            rc = self.solve(fn.getAxContext(), fn.getPCContext(), b, x)
+            fn.setSolution(x)
+            return rc
+        A, pc, b, x = args
         it = C.c_uint32()
         res = np.zeros(b.B)
         rc = _lib.dfx_cg_solve(self.ctx.h, A.h, pc.h, b.h, x.h, self.absTol, self.relTol,
@@ -575,6 +651,99 @@ class CGLinearSolver:
         self.iterations = it.value
         self.residualNorms = res
         return rc
+
+
+class _Borrowed(_Operator):
+    """An operator handle owned by another object (never destroyed here)."""
+
+    def __init__(self, ctx, h, owner):
+        self.ctx, self._hb, self._owner = ctx, h, owner
+
+    @property
+    def h(self):
+        return self._hb
+
+    def close(self):
+        pass
+
+
+class PoissonLinearSolverFunctionFE:
+    """electrostatics::PoissonLinearSolverFunctionFE on the GPU
+    (src/electrostatics/PoissonLinearSolverFunctionFE.t.cpp:38-660):
+    right-hand side = integral of the quadrature values against the basis
+    minus the stiffness applied to the Dirichlet lift, Jacobi diagonal
+    assembled from the stiffness cell matrices, solution = x + u_D.
+
+    problem: the Problem whose basis and quadrature the values live on;
+    stiffness: its cell-wise K operator (homogeneous constraints);
+    fieldConstraints: the field's constraints (Dirichlet values in the
+    inhomogeneities); rhs: MultiVector (quadrature points x components)."""
+
+    def __init__(self, problem, stiffness, fieldConstraints, rhs, pcType=PC_JACOBI):
+        self.ctx = problem.ctx
+        self._keep = [problem, stiffness, fieldConstraints]
+        h = C.c_void_p()
+        self.ctx.check(_lib.dfx_poisson_create(problem.h, stiffness.h, fieldConstraints.h,
+                                               rhs.h, int(pcType), C.byref(h)))
+        self.h = h
+        self.nDofs, self.nComponents = problem.nDofs, rhs.B
+        a, p = C.c_void_p(), C.c_void_p()
+        _lib.dfx_poisson_get_ax(self.h, C.byref(a))
+        _lib.dfx_poisson_get_pc(self.h, C.byref(p))
+        self._ax = _Borrowed(self.ctx, a, self)
+        self._pc = _Borrowed(self.ctx, p, self)
+
+    def reinit(self, fieldConstraints, rhs):
+        if fieldConstraints is not None:
+            self._keep.append(fieldConstraints)
+        self.ctx.check(_lib.dfx_poisson_reinit(
+            self.h, fieldConstraints.h if fieldConstraints is not None else None, rhs.h))
+
+    def getAxContext(self):
+        return self._ax
+
+    def getPCContext(self):
+        return self._pc
+
+    def _copy(self, fn):
+        out = MultiVector(self.ctx, self.nDofs, self.nComponents)
+        self.ctx.check(fn(self.h, out.h))
+        return out
+
+    def getRhs(self):
+        return self._copy(_lib.dfx_poisson_get_rhs)
+
+    def getInitialGuess(self):
+        return self._copy(_lib.dfx_poisson_get_initial_guess)
+
+    def setSolution(self, x):
+        self.ctx.check(_lib.dfx_poisson_set_solution(self.h, x.h))
+
+    def getSolution(self, solution=None):
+        out = solution or MultiVector(self.ctx, self.nDofs, self.nComponents)
+        self.ctx.check(_lib.dfx_poisson_get_solution(self.h, out.h))
+        return out
+
+    def solve(self, maxIter, absTol, relTol, divTol, solution):
+        """CG solve and getSolution in one library call; returns
+        (LinearSolverErrorCode, iterations, residual norms)."""
+        it = C.c_uint32()
+        res = np.zeros(self.nComponents)
+        rc = _lib.dfx_poisson_solve(self.h, int(maxIter), absTol, relTol, divTol, solution.h,
+                                    C.byref(it), _ptr(res))
+        self.ctx.check(rc, allow_positive=True)
+        return rc, it.value, res
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.dfx_poisson_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ----------------------------------------------------------------- problems
@@ -599,6 +768,7 @@ _PROB_ARRAYS = {  # name: (id, dtype, size key)
     "enrM": (14, np.float64, "nE2"), "alpha": (15, np.float64, "nEC"),
     "nodeXYZ": (16, np.float64, "nC3"),
     "quadOffsets": (17, np.uint64, "nCells1"), "quadW": (18, np.float64, "nQuad"),
+    "quadXYZ": (19, np.float64, "nQuad3"),
 }
 
 
@@ -774,8 +944,9 @@ class Problem:
         wid, dt, key = _PROB_ARRAYS[name]
         sizes = {"nE2": self.nEnriched ** 2, "nEC": self.nEnriched * self.nClassical,
                  "nC3": 3 * self.nClassical, "nCells1": self.nCells + 1}
-        if key == "nQuad":
+        if key in ("nQuad", "nQuad3"):
             sizes["nQuad"] = int(self.get("quadOffsets")[-1])
+            sizes["nQuad3"] = 3 * sizes["nQuad"]
         n = sizes.get(key, getattr(self, key, 0))
         out = np.zeros(n, dtype=dt)
         if n:
@@ -791,6 +962,17 @@ class Problem:
         rho = MultiVector(self.ctx, nq, 1)
         self.ctx.check(_lib.dfx_problem_density(self.h, _ptr(occ), X.h, rho.h))
         return rho
+
+    def integrate(self, f, constraints=None):
+        """FEBasisOperations::integrateWithBasisValues: the integrals of the
+        quadrature values f (MultiVector, points x components) against every
+        basis function, child rows distributed to their parents by
+        `constraints` (None: no distribution).  Returns a MultiVector
+        (nDofs x components)."""
+        out = MultiVector(self.ctx, self.nDofs, f.B)
+        self.ctx.check(_lib.dfx_problem_integrate(
+            self.h, constraints.h if constraints is not None else None, f.h, out.h))
+        return out
 
     def operators(self, keep_fp32=True):
         """(H, M, K, MInv, MLump, constraints) as operator objects."""

@@ -84,6 +84,13 @@ const char *dfx_ctx_last_error(const dfx_ctx *ctx);
 int         dfx_ctx_synchronize(dfx_ctx *ctx);
 /* the hipStream_t all work of this context is launched on */
 int         dfx_ctx_stream(dfx_ctx *ctx, void **stream);
+/* captured degree-loop graphs of this context: how many captures and replays
+ * have run, and how many graphs are cached now (a graph is captured again
+ * when an operator it launches on has reallocated a buffer, and released
+ * with that operator).  No reference counterpart: the reference has no graph
+ * capture; for tests and callers that want to see replays happen. */
+int         dfx_ctx_graph_stats(const dfx_ctx *ctx, uint64_t *captures, uint64_t *replays,
+                                uint32_t *cached);
 
 /* ======================================================================
  * Device multivector: N x B, row-major, vector index fastest.
@@ -183,6 +190,13 @@ int dfx_op_create_diagonal(dfx_ctx *ctx, uint32_t nDofs, const double *diag,
                            uint32_t nEnriched, const double *enrBlock,
                            dfx_constraints *cons, dfx_op **op);
 int dfx_op_apply(dfx_op *op, dfx_mv *X, dfx_mv *Y);
+/* Y = A (BInv X): the residual Chebyshev filter's pair of applies
+ * BInv.apply(ResidualNew, scratch1); A.apply(scratch1, scratch2)
+ * (src/linearAlgebra/ChebyshevFilter.t.cpp:355-356).  When BInv is diagonal
+ * and shares A's homogeneous Dirichlet constraints, BInv is folded into A's
+ * cell gather (the same values bit for bit); otherwise the two applies run
+ * in turn.  X receives BInv's parent->child update, as BInv.apply does. */
+int dfx_op_apply_after_inverse(dfx_op *A, dfx_op *BInv, dfx_mv *X, dfx_mv *Y);
 int dfx_op_destroy(dfx_op *op);
 
 typedef struct dfx_op_info
@@ -258,7 +272,18 @@ typedef struct dfx_chfsi_params
   uint32_t eigenVectorBatchSize;      /* 0 -> all vectors in one block */
   int      precision;                 /* DFX_PREC_* */
   int      useGraph;                  /* capture the degree loop in a hipGraph */
+  /* OrthogonalizationType (ChebyshevFilteredEigenSolver.h:45-50): only
+   * DFX_ORTHO_CHOLESKY_GRAMSCHMIDT is implemented; the others are refused at
+   * creation (DFX_E_UNSUPPORTED) */
+  int      orthogonalizationType;
+  /* keep copies of the filtered subspace and of its orthonormalized form
+   * (storeIntermediateSubspaces, ChebyshevFilteredEigenSolver.t.cpp:70-78,
+   * 336-381), read back with dfx_chfsi_get_filtered_subspace */
+  int      storeIntermediateSubspaces;
 } dfx_chfsi_params;
+#define DFX_ORTHO_CHOLESKY_GRAMSCHMIDT 0
+#define DFX_ORTHO_MULTIPASS_LOWDIN     1
+#define DFX_ORTHO_MULTIPASS_CGS        2
 
 int dfx_chfsi_create(dfx_ctx *ctx, const dfx_chfsi_params *params,
                      dfx_mv *eigenSubspaceGuess, dfx_chfsi **out);
@@ -270,6 +295,11 @@ int dfx_chfsi_reinit(dfx_chfsi *s, double wantedLower, double wantedUpper,
 int dfx_chfsi_solve(dfx_chfsi *s, dfx_op *A, double *eigenValues,
                     dfx_mv *eigenVectors, int computeEigenVectors, dfx_op *B,
                     dfx_op *BInv);
+/* getFilteredSubspace (orthonormalized = 0) / getOrthogonalizedFilteredSubspace
+ * (orthonormalized = 1) of the last solve (ChebyshevFilteredEigenSolver.t.cpp:
+ * 441-480), copied into `out` (FP64, the eigenvectors' shape); DFX_E_STATE
+ * when the solver was created without storeIntermediateSubspaces */
+int dfx_chfsi_get_filtered_subspace(const dfx_chfsi *s, int orthonormalized, dfx_mv *out);
 /* wall times (ms) of the last solve: filter, orthonormalization, RR */
 int dfx_chfsi_timings(const dfx_chfsi *s, double *ms3);
 int dfx_chfsi_destroy(dfx_chfsi *s);
@@ -320,6 +350,11 @@ typedef struct dfx_ks_result
   double   fermiEnergy;
   uint32_t numOccupied;               /* states with occupancy > tolerance */
 } dfx_ks_result;
+/* ||H x_j - lambda_j M x_j||_2 per column of X (FP64, host results):
+ * KohnShamEigenSolver::getLinearEigenSolveResidual (src/ksdft/
+ * KohnShamEigenSolver.t.cpp:574-684), the convergence test of every pass */
+int dfx_eigen_residual_norms(dfx_ctx *ctx, dfx_op *H, dfx_op *M, dfx_mv *X,
+                             const double *eigenValues, double *residualNorms);
 /* residualNorms: host, length numWantedEigenvalues (may be NULL) */
 int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,
                       dfx_op *M, dfx_op *MInv, dfx_op *MLanczos,
@@ -464,6 +499,8 @@ typedef struct dfx_problem_spec
 #define DFX_PROB_QUAD_OFFSETS 17 /* uint64 [nCells + 1] cell c's quadrature points are
                                    [off[c], off[c + 1]), x fastest within a cell */
 #define DFX_PROB_QUAD_W     18 /* double [off[nCells]] JxW of every quadrature point */
+#define DFX_PROB_QUAD_XYZ   19 /* double [3 off[nCells]] coordinates of every quadrature
+                                  point (x, y, z per point) */
 
 int dfx_problem_build(dfx_ctx *ctx, const dfx_problem_spec *spec, dfx_problem **out);
 /* sizes[8]: nCells, nDofs, nClassical, nEnriched, sum n_e, sum n_e^2, nRows, nEntries */
@@ -487,7 +524,56 @@ int dfx_problem_create_operators(dfx_problem *prob, uint32_t flags, dfx_op **H,
  * laid out by DFX_PROB_QUAD_OFFSETS. */
 int dfx_problem_density(dfx_problem *prob, const double *occupation, dfx_mv *X,
                         dfx_mv *rho);
+/* out = C2P(sum_e Q_e^T N_e^T (JxW o f_e)): the integral of quadrature values
+ * f against every basis function of the problem (orthogonalized enrichment
+ * included), added over cells in cell order, then the constraints' child ->
+ * parent distribution (cons NULL: none).  Replaces
+ * FEBasisOperations::integrateWithBasisValues (src/basis/
+ * FEBasisOperations.t.cpp:1846-2190).  f: FP64, (quadrature points) x nComp,
+ * laid out by DFX_PROB_QUAD_OFFSETS; out: FP64 nDofs x nComp. */
+int dfx_problem_integrate(dfx_problem *prob, dfx_constraints *cons, dfx_mv *f,
+                          dfx_mv *out);
 int dfx_problem_destroy(dfx_problem *prob);
+
+/* ======================================================================
+ * Poisson linear-solver function (electrostatics, one GPU).
+ * Replaces electrostatics::PoissonLinearSolverFunctionFE
+ * (src/electrostatics/PoissonLinearSolverFunctionFE.h, .t.cpp:38-660) and
+ * its LinearSolverFunction interface (src/linearAlgebra/
+ * LinearSolverFunction.h:79-105): getAxContext / getPCContext / getRhs /
+ * getInitialGuess / setSolution / getSolution.
+ *  - K: cell-wise stiffness operator on the homogeneous constraints (the
+ *    Ax context, LaplaceOperatorContextFE on the homogeneous basis manager);
+ *  - field: the field's constraints, Dirichlet values in the
+ *    inhomogeneities (feBasisManagerField);
+ *  - rhsQuad: FP64 (quadrature points of prob) x nComponents (inpRhs);
+ *  - pcType: DFX_PC_JACOBI (assembled, condensed diagonal; constrained rows
+ *    1) or DFX_PC_NONE.
+ * b = integrate(rhsQuad) - C2P_hom(K P2C_field(0)).  prob and K must outlive
+ * the object.
+ * ==================================================================== */
+typedef struct dfx_poisson dfx_poisson;
+#define DFX_PC_NONE   0
+#define DFX_PC_JACOBI 1
+int dfx_poisson_create(dfx_problem *prob, dfx_op *K, dfx_constraints *field,
+                       dfx_mv *rhsQuad, int pcType, dfx_poisson **out);
+/* new right-hand side (and optionally new field constraints): reinit */
+int dfx_poisson_reinit(dfx_poisson *p, dfx_constraints *field, dfx_mv *rhsQuad);
+/* borrowed handles, valid while p lives */
+int dfx_poisson_get_ax(dfx_poisson *p, dfx_op **A);
+int dfx_poisson_get_pc(dfx_poisson *p, dfx_op **pc);
+/* copies into caller multivectors (FP64, nDofs x nComponents) */
+int dfx_poisson_get_rhs(dfx_poisson *p, dfx_mv *b);
+int dfx_poisson_get_initial_guess(dfx_poisson *p, dfx_mv *x);
+int dfx_poisson_set_solution(dfx_poisson *p, const dfx_mv *x);
+/* solution = P2C_field(x + u_D); it becomes the next initial guess */
+int dfx_poisson_get_solution(dfx_poisson *p, dfx_mv *solution);
+/* CGLinearSolver::solve(function) (src/linearAlgebra/CGLinearSolver.t.cpp:
+ * 68-313) followed by getSolution: returns the LinearSolverErrorCode */
+int dfx_poisson_solve(dfx_poisson *p, uint32_t maxIter, double absTol, double relTol,
+                      double divTol, dfx_mv *solution, uint32_t *iterations,
+                      double *residualNorms);
+int dfx_poisson_destroy(dfx_poisson *p);
 
 /* ======================================================================
  * Serving: a stream of independent Kohn-Sham eigenproblems on one GPU with

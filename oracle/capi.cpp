@@ -111,6 +111,81 @@ extern "C"
       }
   }
 
+  int orc_interpolate(void *h, const double *X, uint32_t B, double *out)
+  {
+    return guarded([&] {
+      interpolate(*static_cast<Problem *>(h), X, B, out);
+      return 0;
+    });
+  }
+
+  // integrateWithBasisValues with the problem's constraints' child->parent
+  // step when c2p != 0
+  int orc_integrate(void *h, const double *f, uint32_t B, double *out, int c2p)
+  {
+    return guarded([&] {
+      const Problem &P = *static_cast<Problem *>(h);
+      integrate(P, f, B, out);
+      if (c2p)
+        distributeChildToParent(P.cons, out, B);
+      return 0;
+    });
+  }
+
+  // PoissonLinearSolverFunctionFE (src/electrostatics/
+  // PoissonLinearSolverFunctionFE.t.cpp) on the problem's K, with the
+  // problem's constraints as the homogeneous ones and the same rows carrying
+  // the inhomogeneities `inhom` (nRows) as the field constraints:
+  //   uD   = P2C_field(0)                               (ctor :188-190)
+  //   b    = C2P(integrate(f)) - C2P(sum_e Q_e^T K_e Q_e uD)  (reinit :396-480;
+  //          LaplaceOperatorContextFE::apply with X = field, Y = homogeneous
+  //          managers, LaplaceOperatorContextFE.t.cpp:397-467)
+  //   diag = C2P(sum_e Q_e^T diag K_e), constrained rows 1  (:38-87, :231-245)
+  int orc_poisson(void *h, const double *inhom, const double *f, uint32_t B, double *b,
+                  double *diag, double *uD)
+  {
+    return guarded([&] {
+      const Problem &P   = *static_cast<Problem *>(h);
+      Constraints    fld = P.cons;
+      std::copy(inhom, inhom + fld.inhom.size(), fld.inhom.begin());
+      const size_t n = (size_t)P.nDofs * B;
+      std::fill(uD, uD + n, 0.0);
+      distributeParentToChild(fld, uD, B);
+      integrate(P, f, B, b);
+      distributeChildToParent(P.cons, b, B);
+      // K uD cell by cell (X as given: the field's parent->child step is done)
+      std::vector<double> y(n, 0.0), xc, yc;
+      for (u32 e = 0; e < P.nCells; ++e)
+        {
+          const u32 ne = P.cellNdofs[e];
+          xc.resize((size_t)ne * B);
+          yc.resize((size_t)ne * B);
+          for (u32 i = 0; i < ne; ++i)
+            std::memcpy(&xc[(size_t)i * B], uD + (size_t)P.cellMap[P.cellMapOffset[e] + i] * B,
+                        sizeof(double) * B);
+          gemm('N', 'N', (int)B, (int)ne, (int)ne, 1.0, xc.data(), (int)B,
+               P.K.data() + P.cellMatOffset[e], (int)ne, 0.0, yc.data(), (int)B);
+          for (u32 i = 0; i < ne; ++i)
+            for (u32 j = 0; j < B; ++j)
+              y[(size_t)P.cellMap[P.cellMapOffset[e] + i] * B + j] += yc[(size_t)i * B + j];
+        }
+      distributeChildToParent(P.cons, y.data(), B);
+      for (size_t t = 0; t < n; ++t)
+        b[t] = 1.0 * b[t] + (-1.0) * y[t];
+      std::fill(diag, diag + P.nDofs, 0.0);
+      for (u32 e = 0; e < P.nCells; ++e)
+        {
+          const u32 ne = P.cellNdofs[e];
+          for (u32 i = 0; i < ne; ++i)
+            diag[P.cellMap[P.cellMapOffset[e] + i]] +=
+              P.K[P.cellMatOffset[e] + (size_t)i * ne + i];
+        }
+      distributeChildToParent(P.cons, diag, 1);
+      setConstrainedNodes(P.cons, diag, 1, 1.0);
+      return 0;
+    });
+  }
+
   // A problem assembled from caller arrays (maps, constraints, cell matrices
   // and diagonals), used to run the oracle on data produced elsewhere.
   void *orc_from_arrays(uint32_t nCells, uint32_t nDofs, uint32_t nClassical,

@@ -932,4 +932,65 @@ namespace oracle
           }
       }
   }
+
+  // FEBasisOperations::interpolate (src/basis/FEBasisOperations.t.cpp:996-1277)
+  // with no constraint step: out[(quadOff[e] + q) B + c] = sum_k N_k(x_q)
+  // X[map_e[k], c], k in order (the gather and per-cell gemm of density())
+  void interpolate(const Problem &P, const double *X, u32 B, double *out)
+  {
+    if (P.quadOff.empty())
+      throw std::runtime_error("interpolate needs the basis values (build with keepBasis)");
+    for (u32 e = 0; e < P.nCells; ++e)
+      {
+        const u32     ne  = P.cellNdofs[e];
+        const u64     q0  = P.quadOff[e], nQ = P.quadOff[e + 1] - q0;
+        const double *bf  = &P.basisF[P.basisOff[e]];
+        const u32    *map = &P.cellMap[P.cellMapOffset[e]];
+        for (u64 q = 0; q < nQ; ++q)
+          for (u32 c = 0; c < B; ++c)
+            {
+              double v = 0.0;
+              for (u32 k = 0; k < ne; ++k)
+                v += X[(size_t)map[k] * B + c] * bf[(size_t)k * nQ + q];
+              out[(q0 + q) * B + c] = v;
+            }
+      }
+  }
+
+  // FEBasisOperations::integrateWithBasisValues (src/basis/FEBasisOperations.t.cpp:
+  // 1846-2190) before its constraint step: per cell, inpJxW(q, c) = JxW(q) f(q, c)
+  // (the khatriRao product, :2043-2059), C(c, k) = sum_q inpJxW(q, c) N_k(x_q)
+  // in q order (the per-cell gemm, :2144-2163), added into out in cell order
+  // (addCellWiseDataToFieldData, FECellWiseDataOperations.t.cpp:89-151).
+  // out is overwritten (setValue(0), :1952).
+  void integrate(const Problem &P, const double *f, u32 B, double *out)
+  {
+    if (P.quadOff.empty())
+      throw std::runtime_error("integrate needs the basis values (build with keepBasis)");
+    std::fill(out, out + (size_t)P.nDofs * B, 0.0);
+    std::vector<double> fw, c;
+    for (u32 e = 0; e < P.nCells; ++e)
+      {
+        const u32     ne  = P.cellNdofs[e];
+        const u64     q0  = P.quadOff[e], nQ = P.quadOff[e + 1] - q0;
+        const double *bf  = &P.basisF[P.basisOff[e]];
+        const u32    *map = &P.cellMap[P.cellMapOffset[e]];
+        fw.resize((size_t)nQ * B);
+        for (u64 q = 0; q < nQ; ++q)
+          for (u32 j = 0; j < B; ++j)
+            fw[q * B + j] = P.quadW[q0 + q] * f[(q0 + q) * B + j];
+        c.assign((size_t)ne * B, 0.0);
+        for (u32 k = 0; k < ne; ++k)
+          for (u32 j = 0; j < B; ++j)
+            {
+              double v = 0.0;
+              for (u64 q = 0; q < nQ; ++q)
+                v += fw[q * B + j] * bf[(size_t)k * nQ + q];
+              c[(size_t)k * B + j] = v;
+            }
+        for (u32 k = 0; k < ne; ++k)
+          for (u32 j = 0; j < B; ++j)
+            out[(size_t)map[k] * B + j] += c[(size_t)k * B + j];
+      }
+  }
 } // namespace oracle

@@ -119,6 +119,13 @@ namespace oracle
   // rho[quadOff[e] + q] = sum_i 2 occ[i] psi_i(x_q)^2,
   // psi_i(x_q) = sum_k N_k(x_q) X[map_e[k], i]
   void density(const Problem &P, const double *X, u32 B, const double *occ, double *rho);
+  // integrals of quadrature values against the basis (needs keepBasis):
+  // out[map_e[k], c] += sum_q JxW(q) f(q, c) N_k(x_q), cells in order; no
+  // constraint step
+  void integrate(const Problem &P, const double *f, u32 B, double *out);
+  // values of nodal vectors at the quadrature points (needs keepBasis), no
+  // constraint step
+  void interpolate(const Problem &P, const double *X, u32 B, double *out);
 
   void gatherCells(const Problem &P, const double *x, u32 B, double *xCell);
   void scatterAddCells(const Problem &P, const double *yCell, u32 B, double *y);

@@ -40,7 +40,7 @@ def lib():
                      "orc_apply", "orc_chebyshev_filter", "orc_residual_filter",
                      "orc_cholgs", "orc_rr", "orc_chfsi", "orc_lanczos",
                      "orc_residual_norms", "orc_cg", "orc_time_apply", "orc_quadrature",
-                     "orc_density"):
+                     "orc_density", "orc_integrate", "orc_poisson", "orc_interpolate"):
             getattr(L, name).argtypes = None
         _LIB = L
     return _LIB
@@ -416,6 +416,47 @@ class Problem:
         w = np.zeros(int(off[-1]))
         lib().orc_quadrature(self.h, None, _p(w))
         return off, w
+
+    def interpolate(self, X):
+        """FEBasisOperations::interpolate of nodal vectors X (nDofs x B) to
+        the quadrature points (layout quadrature()), no constraint step."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        X2 = X.reshape(X.shape[0], -1)
+        off, _ = self.quadrature()
+        out = np.zeros((int(off[-1]), X2.shape[1]))
+        if lib().orc_interpolate(self.h, _p(X2), C.c_uint32(X2.shape[1]), _p(out)) != 0:
+            raise RuntimeError(lib().orc_error().decode())
+        return out
+
+    def integrate(self, f, c2p=True):
+        """FEBasisOperations::integrateWithBasisValues (src/basis/
+        FEBasisOperations.t.cpp:1846-2190) of quadrature values f
+        (points x components, layout quadrature()) on a problem built with
+        keep_basis; c2p: the constraints' child->parent step at the end."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        f2 = f.reshape(f.shape[0], -1)
+        out = np.zeros((self.nDofs, f2.shape[1]))
+        if lib().orc_integrate(self.h, _p(f2), C.c_uint32(f2.shape[1]), _p(out),
+                               C.c_int(1 if c2p else 0)) != 0:
+            raise RuntimeError(lib().orc_error().decode())
+        return out
+
+    def poisson(self, inhom, f):
+        """PoissonLinearSolverFunctionFE's set-up with this problem's
+        constraints as the homogeneous ones and `inhom` (one value per
+        constraint row) as the field's Dirichlet data: returns (b, Jacobi
+        diagonal, u_D) with b = integrate(f) - C2P(K u_D)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        f2 = f.reshape(f.shape[0], -1)
+        B = f2.shape[1]
+        inh = np.ascontiguousarray(inhom, dtype=np.float64)
+        assert inh.size == self.nRows
+        b = np.zeros((self.nDofs, B)); uD = np.zeros((self.nDofs, B))
+        diag = np.zeros(self.nDofs)
+        if lib().orc_poisson(self.h, _p(inh), _p(f2), C.c_uint32(B), _p(b), _p(diag),
+                             _p(uD)) != 0:
+            raise RuntimeError(lib().orc_error().decode())
+        return b, diag, uD
 
     def density(self, X, occupation):
         """rho at every quadrature point (cell-major): sum_i 2 f_i psi_i^2."""

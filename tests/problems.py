@@ -39,3 +39,39 @@ def enriched_orth(p=2, n=6, L=24.0):
 def guess(P, nvec, seed=1):
     X = O.rng_uniform(seed, -1.0, 1.0, P.nDofs * nvec).reshape(P.nDofs, nvec)
     return P.p2c(X)
+
+
+def quad_xyz(P):
+    """Coordinates of the quadrature points of an oracle problem built with
+    keep_basis, in its layout (cells in order, Gauss points x fastest within a
+    cell): lo + h (x_q + 1) / 2 per direction."""
+    off, _ = P.quadrature()
+    lo, h = P.cell_boxes()
+    out = np.zeros((int(off[-1]), 3))
+    for e in range(P.nCells):
+        nq = int(round((int(off[e + 1]) - int(off[e])) ** (1.0 / 3.0)))
+        x, _ = O.gauss(nq)
+        t = 0.5 * (x + 1.0)
+        zz, yy, xx = np.meshgrid(t, t, t, indexing="ij")
+        pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+        out[int(off[e]):int(off[e + 1])] = lo[e] + h[e] * pts
+    return out
+
+
+def assembled(P, key, d=None):
+    """The raw assembled matrix sum_e Q_e^T A_e Q_e (no constraints) as a
+    scipy sparse matrix, from the problem's cell matrices."""
+    import scipy.sparse as sp
+    d = d or P.arrays()
+    rows, cols, vals = [], [], []
+    off = moff = 0
+    for n in d["cellNdofs"]:
+        ids = d["cellMap"][off:off + n].astype(np.int64)
+        A = d[key][moff:moff + n * n].reshape(n, n, order="F")
+        rows.append(np.repeat(ids, n))
+        cols.append(np.tile(ids, n))
+        vals.append(A.ravel())
+        off += n
+        moff += n * n
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(P.nDofs, P.nDofs))
