@@ -1,6 +1,8 @@
 // C ABI of libdfx, part 1: context, multivectors, constraints, operators.
 #include <algorithm>
+#include <mutex>
 #include <numeric>
+#include <vector>
 
 #include "api_util.hpp"
 #include "kernels.hpp"
@@ -39,9 +41,12 @@ namespace dfx
   // skipP2C: X is used as given (the caller has applied the parent->child
   // step of other constraints, as LaplaceOperatorContextFE::apply does with
   // its X manager's constraints, LaplaceOperatorContextFE.t.cpp:433-436)
+  // yEOverride: caller-provided scratch for the enriched partial rows (at
+  // least nEnrSlots x B values of T): nothing is allocated and no profiling
+  // record is taken
   template <class T>
   void applyCellwise(dfx_op *op, View<T> X, View<T> Y, const double *dsc = nullptr,
-                     const T *xe = nullptr, bool skipP2C = false)
+                     const T *xe = nullptr, bool skipP2C = false, T *yEOverride = nullptr)
   {
     hipStream_t s = op->ctx->stream;
     if (!dsc && !skipP2C)
@@ -52,7 +57,9 @@ namespace dfx
     if (op->fillY || (op->nDofs > op->nClassical && !op->nEnrSlots))
       k::fill<T>(s, Y, 0.0);
     T *yE = nullptr;
-    if (op->nEnrSlots)
+    if (op->nEnrSlots && yEOverride)
+      yE = yEOverride;
+    else if (op->nEnrSlots)
       {
         if (op->yE.ensure((size_t)op->nEnrSlots * X.B * sizeof(double)))
           ++op->gen;
@@ -69,7 +76,7 @@ namespace dfx
       return !e || atoi(e) != 0;
     }();
     const bool fork = forkEnv && op->ctx->forkClasses;
-    const bool prof = op->profiling && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
+    const bool prof = op->profiling && !yEOverride && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
     ProfRec    r{};
     for (size_t li = 0; li < L.size(); ++li)
       {
@@ -358,6 +365,54 @@ namespace dfx
   template View<float> view<float>(dfx_mv *);
 } // namespace dfx
 
+// ----------------------------------------------- entry point without allocation
+// Scratch of dfx_op_apply_noalloc: the enriched partial rows of a cell-wise
+// apply, DFX_NOALLOC_SCRATCH_BYTES per GPU, held by every GPU the library
+// runs on until the process ends.  Host code never uses the array's name as
+// an address: each device's address comes from hipGetSymbolAddress, taken
+// once per device with that device current when a context is created there.
+namespace
+{
+  static __device__ __align__(256) unsigned char scratch[DFX_NOALLOC_SCRATCH_BYTES];
+
+  std::mutex &scratchMutex()
+  {
+    static std::mutex *m = new std::mutex; // never destroyed
+    return *m;
+  }
+  std::vector<void *> &scratchPtrs()
+  {
+    static std::vector<void *> *v = new std::vector<void *>;
+    return *v;
+  }
+
+  // the current device is `device`
+  void scratchSetUp(int device)
+  {
+    std::lock_guard<std::mutex> lk(scratchMutex());
+    auto &v = scratchPtrs();
+    if ((size_t)device < v.size() && v[device])
+      return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw Error(DFX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                               hipGetErrorString(e));
+    if ((size_t)device >= v.size())
+      v.resize(device + 1, nullptr);
+    v[device] = ptr;
+  }
+
+  void *scratchOn(int device)
+  {
+    std::lock_guard<std::mutex> lk(scratchMutex());
+    auto &v = scratchPtrs();
+    if ((size_t)device >= v.size() || !v[device])
+      throw Error(DFX_E_STATE, "no-allocation scratch not set up on this device");
+    return v[device];
+  }
+} // namespace
+
 // =================================================================== ABI
 extern "C"
 {
@@ -374,6 +429,7 @@ extern "C"
       DFX_REQUIRE(device >= 0 && device < n, "no such HIP device");
       DFX_HIP(hipSetDevice(device));
       ctx->device = device;
+      scratchSetUp(device);
       DFX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
       DFX_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
       DFX_HIP(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming));
@@ -906,6 +962,41 @@ extern "C"
         applyOp<double>(op, view<double>(X), view<double>(Y));
       else
         applyOp<float>(op, view<float>(X), view<float>(Y));
+      return 0;
+    });
+  }
+
+  int dfx_op_apply_noalloc(dfx_op *op, dfx_mv *X, dfx_mv *Y)
+  {
+    if (!op || !X || !Y)
+      return DFX_E_ARG;
+    return guard(op->ctx, [&] {
+      DFX_REQUIRE(X->dtype == Y->dtype, "apply: X and Y dtypes differ");
+      DFX_REQUIRE(X->N == op->nDofs && Y->N == op->nDofs, "apply: N does not match the operator");
+      DFX_REQUIRE(X->B == Y->B, "apply: X and Y have different widths");
+      if (op->kind != OP_CELLWISE)
+        {
+          // the diagonal apply uses no scratch
+          if (X->dtype == DFX_F64)
+            applyDiagonal<double>(op, view<double>(X), view<double>(Y));
+          else
+            applyDiagonal<float>(op, view<float>(X), view<float>(Y));
+          return 0;
+        }
+      const size_t sz   = X->dtype == DFX_F64 ? 8 : 4;
+      const size_t need = (size_t)op->nEnrSlots * X->B * sz;
+      if (need > (size_t)DFX_NOALLOC_SCRATCH_BYTES)
+        throw Error(DFX_E_ARG, "dfx_op_apply_noalloc: the enriched partial rows need " +
+                                 std::to_string(need) + " bytes, above the " +
+                                 std::to_string((size_t)DFX_NOALLOC_SCRATCH_BYTES) +
+                                 "-byte scratch (use dfx_op_apply or fewer columns)");
+      void *ptr = scratchOn(op->ctx->device);
+      if (X->dtype == DFX_F64)
+        applyCellwise<double>(op, view<double>(X), view<double>(Y), nullptr, nullptr, false,
+                              static_cast<double *>(ptr));
+      else
+        applyCellwise<float>(op, view<float>(X), view<float>(Y), nullptr, nullptr, false,
+                             static_cast<float *>(ptr));
       return 0;
     });
   }

@@ -28,6 +28,7 @@ F64, F32 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 OP_KEEP_FP32 = 0x1
 PC_NONE, PC_JACOBI = 0, 1   # linearAlgebra::PreconditionerType
+NOALLOC_SCRATCH_BYTES = 64 << 20   # DFX_NOALLOC_SCRATCH_BYTES
 
 EIG_CODES = {0: "SUCCESS", 1: "LAPACK_ERROR", 2: "LANCZOS_BETA_ZERO",
              3: "LANCZOS_SUBSPACE_INSUFFICIENT", 4: "CHFSI_ORTHONORMALIZATION_ERROR",
@@ -96,6 +97,7 @@ _sig("dfx_op_update_cells", _vp, _vp)
 _sig("dfx_op_create_diagonal", _vp, _u32, _vp, _u32, _vp, _vp, _pp)
 _sig("dfx_op_apply", _vp, _vp, _vp)
 _sig("dfx_op_apply_after_inverse", _vp, _vp, _vp, _vp)
+_sig("dfx_op_apply_noalloc", _vp, _vp, _vp)
 _sig("dfx_problem_integrate", _vp, _vp, _vp, _vp)
 _sig("dfx_poisson_create", _vp, _vp, _vp, _vp, C.c_int, _pp)
 _sig("dfx_poisson_reinit", _vp, _vp, _vp)
@@ -354,6 +356,12 @@ class _Operator:
         """Y = Op X (X's constrained rows are set by parent->child, as in the
         reference OperatorContext::apply)."""
         self.ctx.check(_lib.dfx_op_apply(self.h, X.h, Y.h))
+
+    def apply_noalloc(self, X, Y):
+        """apply() with no device allocation (library-owned per-GPU scratch of
+        NOALLOC_SCRATCH_BYTES for the enriched partial rows; larger applies
+        raise).  Overlapping calls on one GPU race."""
+        self.ctx.check(_lib.dfx_op_apply_noalloc(self.h, X.h, Y.h))
 
     def apply_after_inverse(self, BInv, X, Y):
         """Y = Op (BInv X), the residual filter's BInv.apply then A.apply

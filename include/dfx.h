@@ -190,6 +190,16 @@ int dfx_op_create_diagonal(dfx_ctx *ctx, uint32_t nDofs, const double *diag,
                            uint32_t nEnriched, const double *enrBlock,
                            dfx_constraints *cons, dfx_op **op);
 int dfx_op_apply(dfx_op *op, dfx_mv *X, dfx_mv *Y);
+/* dfx_op_apply with no device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  The enriched partial rows of a
+ * cell-wise apply (nEnrichedSlots x B values) go to a per-GPU scratch array
+ * of DFX_NOALLOC_SCRATCH_BYTES owned by the library (set up when a context
+ * is created on the device, held until the process ends); larger applies are
+ * refused with DFX_E_ARG.  Calls that overlap on one GPU share the array and
+ * race: serialize them.  Results equal dfx_op_apply's bit for bit. */
+#define DFX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dfx_op_apply_noalloc(dfx_op *op, dfx_mv *X, dfx_mv *Y);
 /* Y = A (BInv X): the residual Chebyshev filter's pair of applies
  * BInv.apply(ResidualNew, scratch1); A.apply(scratch1, scratch2)
  * (src/linearAlgebra/ChebyshevFilter.t.cpp:355-356).  When BInv is diagonal
