@@ -274,32 +274,72 @@ namespace dfx
             // registers into G 16-byte LDS stores (one per column, holding G
             // consecutive K values), at the swizzled column position swz().
             vec_t v[UPT][G];
+            // the rows a wave gathers for unit u form one contiguous block
+            // [kLo, kHi] (wave-uniform): wholly classical blocks read rX,
+            // wholly enriched blocks rXE, and the one block that straddles
+            // nc reads its classical rows here and its enriched rows in a
+            // second round of loads, so that every row's load is in flight
+            // at once instead of a load pair and a wait per row
+            bool strU[UPT];
 #pragma unroll
             for (int u = 0; u < UPT; ++u)
               {
                 const int  t    = threadIdx.x + u * 256;
                 const int  q    = t / VCL, c = (t - q * VCL) * G; // K group, first column
                 const bool live = t < UNITS;                     // wave-uniform
+                const int  t0   = u * 256 + 64 * w;
+                const int  kLo  = (t0 / VCL) * G, kHi = ((t0 + 63) / VCL) * G + G - 1;
+                const bool allE = !oneX && kLo >= nc;
+                strU[u]         = !oneX && kLo < nc && kHi >= nc;
+                const rsrc_t r1 = allE ? rXE : rX;
+                const u32    b1 = allE ? nC : lo;
 #pragma unroll
                 for (int r = 0; r < G; ++r)
                   {
                     const int kk = q * G + r;
                     const u32 m  = sMap[live ? kk : 0]; // kk < KMAX <= 256
-                    if (oneX) // every row of the cell through rX (block-uniform)
-                      v[u][r] = BF::load4(rX, (live & (kk < n) & (c < bw))
-                                                ? ((m - lo) * ldx + c0 + c) * S
-                                                : OFF_NONE);
-                    else
-                      {
-                        const vec_t vc = BF::load4(rX, (live & (kk < nc) & (c < bw))
-                                                         ? ((m - lo) * ldx + c0 + c) * S
-                                                         : OFF_NONE);
-                        const vec_t ve = BF::load4(rXE, (live & (kk >= nc) & (kk < n) & (c < bw))
-                                                          ? ((m - nC) * ldx + c0 + c) * S
-                                                          : OFF_NONE);
-                        v[u][r] = kk >= nc ? ve : vc;
-                      }
+                    v[u][r] = BF::load4(r1, (live & (kk < n) & (c < bw) & !(strU[u] & (kk >= nc)))
+                                              ? ((m - b1) * ldx + c0 + c) * S
+                                              : OFF_NONE);
                   }
+              }
+            if (!oneX)
+              {
+                // at most one unit of this wave straddles nc (blocks are
+                // disjoint): its enriched rows, in one round of G loads
+                int uS = -1;
+#pragma unroll
+                for (int u = 0; u < UPT; ++u)
+                  uS = strU[u] ? u : uS;
+                const int  tS    = threadIdx.x + (uS < 0 ? 0 : uS) * 256;
+                const int  qS    = tS / VCL, cS = (tS - qS * VCL) * G;
+                const bool liveS = uS >= 0 && tS < UNITS;
+                vec_t      e[G];
+#pragma unroll
+                for (int r = 0; r < G; ++r)
+                  {
+                    const int kk = qS * G + r;
+                    const u32 m  = sMap[liveS ? kk : 0];
+                    e[r] = BF::load4(rXE, (liveS & (kk >= nc) & (kk < n) & (cS < bw))
+                                            ? ((m - nC) * ldx + c0 + cS) * S
+                                            : OFF_NONE);
+                  }
+#pragma unroll
+                for (int u = 0; u < UPT; ++u)
+                  {
+                    const int q = (threadIdx.x + u * 256) / VCL;
+#pragma unroll
+                    for (int r = 0; r < G; ++r)
+                      v[u][r] = (u == uS & q * G + r >= nc) ? e[r] : v[u][r];
+                  }
+              }
+#pragma unroll
+            for (int u = 0; u < UPT; ++u)
+              {
+                const int  t    = threadIdx.x + u * 256;
+                const bool live = t < UNITS;
+                const int  q    = t / VCL;
+                (void)live;
                 if constexpr (DSC)
                   {
                     // M^-1 folded in: rows q G .. q G + G - 1 of the unit
