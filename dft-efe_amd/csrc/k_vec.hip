@@ -29,6 +29,11 @@ namespace dfx
       };
       __device__ inline RC rc(u64 t, u32 B)
       {
+        if ((t >> 32) == 0) // 32-bit division while the flat index fits
+          {
+            const u32 tt = (u32)t, i = tt / B;
+            return {i, tt - i * B};
+          }
         u32 i = (u32)(t / B);
         return {i, (u32)(t - (u64)i * B)};
       }
@@ -216,6 +221,17 @@ namespace dfx
           }
       }
 
+      // one element of the residual step (both kernels below use it, so a
+      // column gets the same bits on either path)
+      template <class T>
+      __device__ inline T resUpdate(T s2, T rn, T res, T r0, double mu, T alpha1, T alpha2,
+                                    T c)
+      {
+        const T s1 = alpha1 * s2 + (-c * alpha1) * rn;
+        const T v  = (T)1 * s1 + alpha2 * res;
+        return (T)1 * (T)1 * v + alpha1 * (T)mu * r0;
+      }
+
       template <class T>
       __global__ void resStepK(T *res, u32 ldr, const T *s2, u32 lds, const T *rn,
                                u32 ldn, const T *r0, u32 ld0, const double *mu,
@@ -226,12 +242,37 @@ namespace dfx
         for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
              t += (u64)gridDim.x * blockDim.x)
           {
-            RC r     = rc(t, B);
-            T  s1    = alpha1 * s2[(u64)r.i * lds + r.j] +
-                      (-c * alpha1) * rn[(u64)r.i * ldn + r.j];
-            T &o     = res[(u64)r.i * ldr + r.j];
-            T  v     = (T)1 * s1 + alpha2 * o;
-            o        = (T)1 * (T)1 * v + alpha1 * (T)mu[r.j] * r0[(u64)r.i * ld0 + r.j];
+            RC r = rc(t, B);
+            T &o = res[(u64)r.i * ldr + r.j];
+            o    = resUpdate<T>(s2[(u64)r.i * lds + r.j], rn[(u64)r.i * ldn + r.j], o,
+                                r0[(u64)r.i * ld0 + r.j], mu[r.j], alpha1, alpha2, c);
+          }
+      }
+
+      // resStepK on contiguous N x B arrays (every leading dimension B) with
+      // B a multiple of the 16-byte vector: one vector of V columns per
+      // thread, the same arithmetic per element
+      template <class T>
+      __global__ void resStepVecK(T *__restrict__ res, const T *__restrict__ s2,
+                                  const T *__restrict__ rn, const T *__restrict__ r0,
+                                  const double *__restrict__ mu,
+                                  const double *__restrict__ coef, u32 nVec, u32 B)
+      {
+        constexpr int V = 16 / sizeof(T);
+        typedef T vt __attribute__((ext_vector_type(V)));
+        const T alpha1 = (T)coef[0], alpha2 = (T)coef[1], c = (T)coef[2];
+        for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < nVec;
+             t += gridDim.x * blockDim.x)
+          {
+            const u32 j0 = (t * V) % B;
+            const vt  a  = reinterpret_cast<const vt *>(s2)[t];
+            const vt  b  = reinterpret_cast<const vt *>(rn)[t];
+            const vt  z  = reinterpret_cast<const vt *>(r0)[t];
+            vt        o  = reinterpret_cast<const vt *>(res)[t];
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              o[e] = resUpdate<T>(a[e], b[e], o[e], z[e], mu[j0 + e], alpha1, alpha2, c);
+            reinterpret_cast<vt *>(res)[t] = o;
           }
       }
 
@@ -485,6 +526,18 @@ namespace dfx
       const u64 n = (u64)res.N * res.B;
       if (!n)
         return;
+      constexpr u32 V       = 16 / sizeof(T);
+      auto          aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+      if (res.ld == res.B && s2.ld == res.B && rn.ld == res.B && r0.ld == res.B &&
+          res.B % V == 0 && n / V < (1ull << 32) && aligned(res.p) && aligned(s2.p) &&
+          aligned(rn.p) && aligned(r0.p))
+        {
+          const u64 nv = n / V;
+          resStepVecK<T><<<gridFor(nv), TPB, 0, s>>>(res.p, s2.p, rn.p, r0.p, mu, coef,
+                                                     (u32)nv, res.B);
+          checkLaunch("resStepVec");
+          return;
+        }
       resStepK<T><<<gridFor(n), TPB, 0, s>>>(res.p, res.ld, s2.p, s2.ld, rn.p, rn.ld,
                                              r0.p, r0.ld, mu, coef, res.N, res.B);
       checkLaunch("resStep");
