@@ -44,9 +44,13 @@ namespace dfx
   // yEOverride: caller-provided scratch for the enriched partial rows (at
   // least nEnrSlots x B values of T): nothing is allocated and no profiling
   // record is taken
+  // rf (with dsc): the residual Chebyshev step fused in; Y then holds
+  // partial sums on the rows the cells finish, and res is complete on
+  // return (the listed remaining rows by a separate step)
   template <class T>
   void applyCellwise(dfx_op *op, View<T> X, View<T> Y, const double *dsc = nullptr,
-                     const T *xe = nullptr, bool skipP2C = false, T *yEOverride = nullptr)
+                     const T *xe = nullptr, bool skipP2C = false, T *yEOverride = nullptr,
+                     const k::ResFuse<T> *rf = nullptr)
   {
     hipStream_t s = op->ctx->stream;
     if (!dsc && !skipP2C)
@@ -100,7 +104,7 @@ namespace dfx
             DFX_HIP(hipEventRecord(op->ctx->forkEv, s));
             DFX_HIP(hipStreamWaitEvent(op->ctx->stream2, op->ctx->forkEv, 0));
           }
-        k::cellApply<T>(ls, op, l, frag, Xc, Y, yE, dsc, xe);
+        k::cellApply<T>(ls, op, l, frag, Xc, Y, yE, dsc, xe, 0, rf);
         if (side)
           {
             DFX_HIP(hipEventRecord(op->ctx->joinEv, ls));
@@ -126,6 +130,10 @@ namespace dfx
     if (op->nEnrSlots)
       k::enrichedReduce<T>(s, op, yE, Y);
     k::c2p<T>(s, op->cons, Y);
+    if (rf)
+      k::resFix<T>(s, View<T>{rf->res, X.N, X.B, X.ld}, View<const T>{Y.p, Y.N, Y.B, Y.ld}, Xc,
+                   View<const T>{rf->r0, X.N, X.B, X.ld}, op->resRows.as<u32>(), op->nResRows,
+                   rf->mu, rf->coef);
   }
 
   // Y = C2P(diag o P2C(X) + [0; E X_E])
@@ -243,8 +251,12 @@ namespace dfx
   // same precision; constrained rows are zero either way) and X receives the
   // same parent->child update.  Returns false (nothing done) when the
   // operators or the multivectors do not qualify.
+  // rf: the residual Chebyshev step fused in (res and r0 N x B at X's
+  // leading dimension); res then equals resStep(Y, X, res, r0) bit for bit
+  // and Y holds partial sums on the rows the cells finish
   template <class T>
-  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y)
+  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y,
+                            const k::ResFuse<T> *rf)
   {
     if (Dinv->kind != OP_DIAGONAL || A->kind != OP_CELLWISE || Dinv->cons != A->cons)
       return false;
@@ -275,13 +287,13 @@ namespace dfx
     if (Dinv->nEnriched)
       k::enrichedBlock<T>(s, S1, Dinv->enrBlock.as<double>(),
                           View<const T>{X.p, X.N, X.B, X.ld}, A->nClassical, Dinv->nEnriched);
-    applyCellwise<T>(A, X, Y, A->foldedDiag.as<double>(), S1.p);
+    applyCellwise<T>(A, X, Y, A->foldedDiag.as<double>(), S1.p, false, nullptr, rf);
     return true;
   }
   template bool applyInvThenCellwise<double>(dfx_op *, dfx_op *, View<double>, View<double>,
-                                             View<double>);
+                                             View<double>, const k::ResFuse<double> *);
   template bool applyInvThenCellwise<float>(dfx_op *, dfx_op *, View<float>, View<float>,
-                                            View<float>);
+                                            View<float>, const k::ResFuse<float> *);
 
   template <class T>
   void applyOp(dfx_op *op, View<T> X, View<T> Y)

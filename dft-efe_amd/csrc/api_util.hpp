@@ -7,6 +7,15 @@
 
 namespace dfx
 {
+  namespace k
+  {
+    template <class T>
+    struct ResFuse;
+  }
+} // namespace dfx
+
+namespace dfx
+{
   inline std::string &last_global_error()
   {
     static thread_local std::string e;
@@ -66,7 +75,8 @@ namespace dfx
   template <class T>
   void applyOp(dfx_op *op, View<T> X, View<T> Y);
   template <class T>
-  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y);
+  bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y,
+                            const k::ResFuse<T> *rf = nullptr);
   // Y = C2P(sum_e Q_e^T A_e Q_e X) with X used as given (no parent->child step)
   template <class T>
   void applyCellwiseNoP2C(dfx_op *op, View<T> X, View<T> Y);

@@ -291,6 +291,12 @@ struct dfx_op
   dfx::u64 fragLen64 = 0, fragLen32 = 0;
   dfx::DevBuf map, meta, frag64, frag32;
   dfx::DevBuf first;     // per launch position, 8 words: first-touch bits of classical rows
+  // per launch position, 8 words: last-touch bits of the unconstrained
+  // classical rows (the residual step fused into the cell apply), and the
+  // rows such an apply leaves to the stand-alone step (constrained,
+  // enriched, in no cell)
+  dfx::DevBuf last, resRows;
+  dfx::u32    nResRows = 0;
   bool fillY  = false;   // some classical dof lies in no cell: Y needs a zero fill
   bool hasF32 = false;
   // enriched partial reduction

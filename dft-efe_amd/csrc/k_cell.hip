@@ -17,6 +17,7 @@
 // 1 KiB read.  Entry (lane l, value s) of block (mt, kb) is
 // A_e^T[mt*16 + l%16][kb*4G + G*(l/16) + s] = A_e(k, j) (column-major input).
 #include "kernels.hpp"
+#include "res_update.hpp"
 
 // Build-time tuning of the cell kernel (defaults measured on MI355X):
 // A-operand prefetch depth in K blocks, whether the read-modify-write loads of
@@ -54,6 +55,10 @@
 #endif
 #ifndef DFX_CELL_DEPTH1
 #define DFX_CELL_DEPTH1 1
+#endif
+// rows per request group of the fused residual step's operands (1, 2 or 4)
+#ifndef DFX_RES_ROWS
+#define DFX_RES_ROWS 1
 #endif
 
 
@@ -170,12 +175,18 @@ namespace dfx
         static constexpr bool early = small ? DFX_CELL_EARLY_SMALL : DFX_CELL_EARLY;
       };
 
-      template <class T, int NT, int MTW, bool DSC, bool TAIL>
+      // RES (with DSC): the residual Chebyshev step fused into the scatter.
+      // last[pos * 8 + i / 32] bit i % 32: this cell is the last (in launch
+      // order) to touch its classical row i and the row is unconstrained, so
+      // y's row is complete here; res = resUpdate(y, x, res, r0) is stored for
+      // it instead of y (x read back raw, res and r0 at x's leading dimension)
+      template <class T, int NT, int MTW, bool DSC, bool TAIL, bool RES>
       __global__ __launch_bounds__(256, (CellPolicy<T, NT, MTW>::minw)) void cellApplyBufK(
         const CellMeta *__restrict__ meta, const u32 *__restrict__ first, u32 begin,
         const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
         u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC,
-        const double *__restrict__ dsc, const T *xe)
+        const double *__restrict__ dsc, const T *xe, ResFuse<T> rf,
+        const u32 *__restrict__ last)
       {
         using M                 = Mfma<T>;
         using BF                = Buf<T>;
@@ -206,6 +217,7 @@ namespace dfx
         T *sX = reinterpret_cast<T *>(smemRaw);
         __shared__ u32 sMap[256];
         __shared__ u32 sFirst[8];
+        __shared__ u32 sLast[RES ? 8 : 1];
         __shared__ __align__(16) T sDsc[DSC ? 256 : 4];
 
         const u32      pos  = begin + blockIdx.x;
@@ -226,6 +238,11 @@ namespace dfx
         const rsrc_t rXE = rsrc(xe + (u64)nC * ldx, cap((u64)(nRowsX - nC) * ldx * S));
         const rsrc_t rY  = rsrc(y + (u64)lo * ldy, cap((u64)(nRowsY - lo) * ldy * S));
         const rsrc_t rE  = rsrc(yE, yEBytes);
+        // fused residual step: res and r0 addressed like x
+        const rsrc_t rRes = rsrc(RES ? rf.res + (u64)lo * ldx : nullptr,
+                                 RES ? cap((u64)(nRowsX - lo) * ldx * S) : 0u);
+        const rsrc_t rR0  = rsrc(RES ? rf.r0 + (u64)lo * ldx : nullptr,
+                                 RES ? cap((u64)(nRowsX - lo) * ldx * S) : 0u);
 
         auto loadA = [&](int kb, vec_t *dst) {
 #pragma unroll
@@ -252,7 +269,11 @@ namespace dfx
           sDsc[threadIdx.x] = threadIdx.x < (unsigned)nc ? (T)dsc[cm.mapOff + threadIdx.x]
                                                          : (T)1;
         if (threadIdx.x < 8)
-          sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
+          {
+            sFirst[threadIdx.x] = first[(u64)pos * 8 + threadIdx.x];
+            if constexpr (RES)
+              sLast[threadIdx.x] = last[(u64)pos * 8 + threadIdx.x];
+          }
         __syncthreads();
 
         const int cl = l & 15;
@@ -264,6 +285,14 @@ namespace dfx
         // for columns past B; a separate instantiation because the second
         // path costs registers
         auto chunk = [&](auto ntl, const u32 c0, const int bw) {
+            // RES: the thread index is made opaque per chunk, so the row
+            // offsets and masks of the gather and the scatter are formed here
+            // rather than hoisted out of the chunk loop and held (spilled)
+            // throughout
+            int lc = l, tx = threadIdx.x;
+            if constexpr (RES)
+              asm volatile("" : "+v"(lc), "+v"(tx));
+            auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(lc, v); };
             constexpr int NTL   = decltype(ntl)::value;
             constexpr int VCL   = 16 * NTL / G;
             constexpr int UNITS = (KMAX / G) * VCL; // G x G gather units of the tile
@@ -284,7 +313,7 @@ namespace dfx
 #pragma unroll
             for (int u = 0; u < UPT; ++u)
               {
-                const int  t    = threadIdx.x + u * 256;
+                const int  t    = tx + u * 256;
                 const int  q    = t / VCL, c = (t - q * VCL) * G; // K group, first column
                 const bool live = t < UNITS;                     // wave-uniform
                 const int  t0   = u * 256 + 64 * w;
@@ -311,7 +340,7 @@ namespace dfx
 #pragma unroll
                 for (int u = 0; u < UPT; ++u)
                   uS = strU[u] ? u : uS;
-                const int  tS    = threadIdx.x + (uS < 0 ? 0 : uS) * 256;
+                const int  tS    = tx + (uS < 0 ? 0 : uS) * 256;
                 const int  qS    = tS / VCL, cS = (tS - qS * VCL) * G;
                 const bool liveS = uS >= 0 && tS < UNITS;
                 vec_t      e[G];
@@ -327,7 +356,7 @@ namespace dfx
 #pragma unroll
                 for (int u = 0; u < UPT; ++u)
                   {
-                    const int q = (threadIdx.x + u * 256) / VCL;
+                    const int q = (tx + u * 256) / VCL;
 #pragma unroll
                     for (int r = 0; r < G; ++r)
                       v[u][r] = (u == uS & q * G + r >= nc) ? e[r] : v[u][r];
@@ -336,7 +365,7 @@ namespace dfx
 #pragma unroll
             for (int u = 0; u < UPT; ++u)
               {
-                const int  t    = threadIdx.x + u * 256;
+                const int  t    = tx + u * 256;
                 const bool live = t < UNITS;
                 const int  q    = t / VCL;
                 (void)live;
@@ -364,7 +393,7 @@ namespace dfx
 #pragma unroll
             for (int u = 0; u < UPT; ++u)
               {
-                const int t = threadIdx.x + u * 256;
+                const int t = tx + u * 256;
                 const int q = t / VCL, c = (t - q * VCL) * G;
                 if (t < UNITS)
 #pragma unroll
@@ -468,7 +497,22 @@ namespace dfx
                 loadA(d, ring[d]);
 
             // ---- scatter: classical rows to y (stored on first touch, else
-            // added to the old value), enriched rows to yE
+            // added to the old value), enriched rows to yE; with RES the rows
+            // this cell finishes get the residual step instead
+            T muT[RES ? NTL : 1];
+            T a1 = 0, a2 = 0, cc = 0;
+            if constexpr (RES)
+              {
+                a1 = (T)rf.coef[0];
+                a2 = (T)rf.coef[1];
+                cc = (T)rf.coef[2];
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt)
+                  {
+                    const int col = nt * 16 + cl;
+                    muT[nt]       = (T)rf.mu[c0 + (col < bw ? col : 0)];
+                  }
+              }
 #pragma unroll
             for (int t = 0; t < MTW; ++t)
               {
@@ -479,22 +523,61 @@ namespace dfx
                 if constexpr (!EARLY)
                   loadOld(t, oldLate);
                 const bool enr = (w + 4 * t) * 16 + 15 >= nc; // wave-uniform
+                // rows in groups of RG: the fused step's operands (0 on the
+                // rows it skips) are requested per group, then the group is
+                // stored
+                constexpr int RG = RES ? DFX_RES_ROWS : 4;
 #pragma unroll
-                for (int vv = 0; vv < 4; ++vv)
+                for (int vg = 0; vg < 4; vg += RG)
                   {
-                    const int row = rowOf(t, vv);
-                    const u32 yb  = ((sMap[row] - lo) * ldy + c0 + cl) * S;
-                    const u32 eb  = ((cm.enrOff + (u32)(row - nc)) * B + c0 + cl) * S;
+                    T    fx[RES ? NTL : 1][RG], fr[RES ? NTL : 1][RG], f0[RES ? NTL : 1][RG];
+                    bool fin[RG];
 #pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt)
+                    for (int g = 0; g < RG; ++g)
                       {
-                        const bool colOk = nt * 16 + cl < bw;
-                        BF::store(o[nt][vv] + acc[t][nt][vv], rY,
-                                  ((row < nc) & colOk) ? yb + nt * 16 * S : OFF_NONE);
-                        if (enr)
-                          BF::store(acc[t][nt][vv], rE,
-                                    ((row >= nc) & (row < n) & colOk) ? eb + nt * 16 * S
-                                                                      : OFF_NONE);
+                        const int row = rowOf(t, vg + g);
+                        fin[g] = RES && (row < nc) &&
+                                 ((sLast[RES ? row >> 5 : 0] >> (row & 31)) & 1u);
+                        if constexpr (RES)
+                          {
+                            const u32 xb = ((sMap[row] - lo) * ldx + c0 + cl) * S;
+#pragma unroll
+                            for (int nt = 0; nt < NTL; ++nt)
+                              {
+                                const u32 off = (fin[g] & (nt * 16 + cl < bw))
+                                                  ? xb + nt * 16 * S
+                                                  : OFF_NONE;
+                                fx[nt][g] = BF::load(rX, off);
+                                fr[nt][g] = BF::load(rRes, off);
+                                f0[nt][g] = BF::load(rR0, off);
+                              }
+                          }
+                      }
+#pragma unroll
+                    for (int g = 0; g < RG; ++g)
+                      {
+                        const int vv  = vg + g;
+                        const int row = rowOf(t, vv);
+                        const u32 yb  = ((sMap[row] - lo) * ldy + c0 + cl) * S;
+                        const u32 eb  = ((cm.enrOff + (u32)(row - nc)) * B + c0 + cl) * S;
+                        const u32 xb  = ((sMap[row] - lo) * ldx + c0 + cl) * S;
+#pragma unroll
+                        for (int nt = 0; nt < NTL; ++nt)
+                          {
+                            const bool colOk = nt * 16 + cl < bw;
+                            const T    yv    = o[nt][vv] + acc[t][nt][vv];
+                            BF::store(yv, rY,
+                                      ((row < nc) & colOk & !fin[g]) ? yb + nt * 16 * S
+                                                                     : OFF_NONE);
+                            if constexpr (RES)
+                              BF::store(resUpdate<T>(yv, fx[nt][g], fr[nt][g], f0[nt][g],
+                                                     (double)muT[nt], a1, a2, cc),
+                                        rRes, (fin[g] & colOk) ? xb + nt * 16 * S : OFF_NONE);
+                            if (enr)
+                              BF::store(acc[t][nt][vv], rE,
+                                        ((row >= nc) & (row < n) & colOk) ? eb + nt * 16 * S
+                                                                          : OFF_NONE);
+                          }
                       }
                   }
               }
@@ -824,7 +907,8 @@ namespace dfx
     {
       template <class T, int NT, int MTW>
       void launchCell(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                      View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe)
+                      View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe,
+                      const ResFuse<T> *rf)
       {
         constexpr int BC = 16 * NT;
         const u64     S  = sizeof(T);
@@ -837,18 +921,22 @@ namespace dfx
         // (NT = 5 serves 65..80 columns in one chunk, never with a tail)
         constexpr bool TAILOK = NT != 1 && NT != 5;
         const bool  tail  = TAILOK && buf && x.B % BC != 0 && x.B % BC <= 16;
-        const int   which = !buf ? 0 : (dsc ? 2 : 1) + (tail ? 2 : 0);
-        const void *fns[5] = {
+        DFX_REQUIRE(!rf || (dsc && buf && op->last.p),
+                    "the fused residual step needs the folded buffer-addressed cell kernel");
+        const int   which = !buf ? 0 : rf ? 5 + (tail ? 1 : 0) : (dsc ? 2 : 1) + (tail ? 2 : 0);
+        const void *fns[7] = {
           reinterpret_cast<const void *>(&cellApplyK<T, NT, MTW>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, false>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, false>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, TAILOK>),
-          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, TAILOK>)};
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, false, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, false, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, false, TAILOK, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, TAILOK, false>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, false, true>),
+          reinterpret_cast<const void *>(&cellApplyBufK<T, NT, MTW, true, TAILOK, true>)};
         const void *fn = fns[which];
         // raised once per kernel to the largest tile seen (several host
         // threads may launch concurrently: serving slots)
-        static std::atomic<size_t> attrSet[5] = {60 * 1024, 60 * 1024, 60 * 1024, 60 * 1024,
-                                                 60 * 1024};
+        static std::atomic<size_t> attrSet[7] = {60 * 1024, 60 * 1024, 60 * 1024, 60 * 1024,
+                                                 60 * 1024, 60 * 1024, 60 * 1024};
         size_t cur = attrSet[which].load();
         if (lds > cur)
           {
@@ -859,23 +947,30 @@ namespace dfx
               }
           }
         DFX_REQUIRE(buf || !dsc, "folded diagonal needs the buffer-addressed cell kernel");
-        auto bufLaunch = [&](auto dscTag, auto tailTag) {
+        auto bufLaunch = [&](auto dscTag, auto tailTag, auto resTag) {
           constexpr bool D = decltype(dscTag)::value, TL = decltype(tailTag)::value;
-          cellApplyBufK<T, NT, MTW, D, TL><<<L.end - L.begin, 256, lds, s>>>(
+          constexpr bool R = decltype(resTag)::value;
+          cellApplyBufK<T, NT, MTW, D, TL, R><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
             x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical,
-            D ? dsc : nullptr, D && xe ? xe : x.p);
+            D ? dsc : nullptr, D && xe ? xe : x.p, R ? *rf : ResFuse<T>{},
+            R ? op->last.as<u32>() : nullptr);
         };
-        using F = std::false_type;
+        using F  = std::false_type;
+        using Tr = std::true_type;
         using Tt = std::integral_constant<bool, TAILOK>;
         if (which == 1)
-          bufLaunch(F{}, F{});
+          bufLaunch(F{}, F{}, F{});
         else if (which == 2)
-          bufLaunch(std::true_type{}, F{});
+          bufLaunch(Tr{}, F{}, F{});
         else if (which == 3)
-          bufLaunch(F{}, Tt{});
+          bufLaunch(F{}, Tt{}, F{});
         else if (which == 4)
-          bufLaunch(std::true_type{}, Tt{});
+          bufLaunch(Tr{}, Tt{}, F{});
+        else if (which == 5)
+          bufLaunch(Tr{}, F{}, Tr{});
+        else if (which == 6)
+          bufLaunch(Tr{}, Tt{}, Tr{});
         else
           cellApplyK<T, NT, MTW><<<L.end - L.begin, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
@@ -886,14 +981,14 @@ namespace dfx
       template <class T, int NT>
       void launchCellMtw(hipStream_t s, const dfx_op *op, const Launch &L,
                          const T *frag, View<const T> x, View<T> y, T *yE,
-                         const double *dsc, const T *xe)
+                         const double *dsc, const T *xe, const ResFuse<T> *rf)
       {
         switch (L.mtw)
           {
-            case 1: return launchCell<T, NT, 1>(s, op, L, frag, x, y, yE, dsc, xe);
-            case 2: return launchCell<T, NT, 2>(s, op, L, frag, x, y, yE, dsc, xe);
-            case 3: return launchCell<T, NT, 3>(s, op, L, frag, x, y, yE, dsc, xe);
-            case 4: return launchCell<T, NT, 4>(s, op, L, frag, x, y, yE, dsc, xe);
+            case 1: return launchCell<T, NT, 1>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+            case 2: return launchCell<T, NT, 2>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+            case 3: return launchCell<T, NT, 3>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+            case 4: return launchCell<T, NT, 4>(s, op, L, frag, x, y, yE, dsc, xe, rf);
           }
         throw Error(DFX_E_UNSUPPORTED, "cell size above 256 dofs");
       }
@@ -951,7 +1046,8 @@ namespace dfx
 
     template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
-                   View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe, int nt)
+                   View<const T> x, View<T> y, T *yE, const double *dsc, const T *xe, int nt,
+                   const ResFuse<T> *rf)
     {
       if (L.end <= L.begin || !x.B)
         return;
@@ -963,11 +1059,11 @@ namespace dfx
         nt = chooseNT<T>(x.B, L.kpadMax);
       switch (nt)
         {
-          case 1: return launchCellMtw<T, 1>(s, op, L, frag, x, y, yE, dsc, xe);
-          case 2: return launchCellMtw<T, 2>(s, op, L, frag, x, y, yE, dsc, xe);
-          case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE, dsc, xe);
-          case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE, dsc, xe);
-          case 5: return launchCellMtw<T, 5>(s, op, L, frag, x, y, yE, dsc, xe);
+          case 1: return launchCellMtw<T, 1>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+          case 2: return launchCellMtw<T, 2>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+          case 4: return launchCellMtw<T, 4>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+          case 8: return launchCellMtw<T, 8>(s, op, L, frag, x, y, yE, dsc, xe, rf);
+          case 5: return launchCellMtw<T, 5>(s, op, L, frag, x, y, yE, dsc, xe, rf);
         }
       throw Error(DFX_E_ARG, "column-chunk width must be 1, 2, 4, 5 or 8 tiles of 16");
     }
@@ -1053,10 +1149,12 @@ namespace dfx
 
     template void cellApply<double>(hipStream_t, const dfx_op *, const Launch &,
                                     const double *, View<const double>, View<double>,
-                                    double *, const double *, const double *, int);
+                                    double *, const double *, const double *, int,
+                                    const ResFuse<double> *);
     template void cellApply<float>(hipStream_t, const dfx_op *, const Launch &,
                                    const float *, View<const float>, View<float>, float *,
-                                   const double *, const float *, int);
+                                   const double *, const float *, int,
+                                   const ResFuse<float> *);
     template int builtinNT<double>(u32, u32);
     template int builtinNT<float>(u32, u32);
     template void enrichedReduce<double>(hipStream_t, const dfx_op *, const double *,

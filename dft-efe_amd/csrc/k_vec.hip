@@ -5,6 +5,7 @@
 // are addressed in place, without the reference's per-dof copy loops,
 // src/linearAlgebra/ChebyshevFilteredEigenSolver.t.cpp:246-333).
 #include "kernels.hpp"
+#include "res_update.hpp"
 
 namespace dfx
 {
@@ -221,17 +222,6 @@ namespace dfx
           }
       }
 
-      // one element of the residual step (both kernels below use it, so a
-      // column gets the same bits on either path)
-      template <class T>
-      __device__ inline T resUpdate(T s2, T rn, T res, T r0, double mu, T alpha1, T alpha2,
-                                    T c)
-      {
-        const T s1 = alpha1 * s2 + (-c * alpha1) * rn;
-        const T v  = (T)1 * s1 + alpha2 * res;
-        return (T)1 * (T)1 * v + alpha1 * (T)mu * r0;
-      }
-
       template <class T>
       __global__ void resStepK(T *res, u32 ldr, const T *s2, u32 lds, const T *rn,
                                u32 ldn, const T *r0, u32 ld0, const double *mu,
@@ -273,6 +263,27 @@ namespace dfx
             for (int e = 0; e < V; ++e)
               o[e] = resUpdate<T>(a[e], b[e], o[e], z[e], mu[j0 + e], alpha1, alpha2, c);
             reinterpret_cast<vt *>(res)[t] = o;
+          }
+      }
+
+      // the residual step on the rows a fused cell apply leaves to it (listed
+      // in `rows`: constrained, enriched and cell-less rows), after the
+      // enriched reduction and the child->parent step have finished y there
+      template <class T>
+      __global__ void resFixK(T *res, const T *s2, u32 lds, const T *rn, const T *r0, u32 ld,
+                              const u32 *__restrict__ rows, u32 nRows,
+                              const double *mu, const double *coef, u32 B)
+      {
+        const u64 n      = (u64)nRows * B;
+        const T   alpha1 = (T)coef[0], alpha2 = (T)coef[1], c = (T)coef[2];
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            const RC  r = rc(t, B);
+            const u32 i = rows[r.i];
+            const u64 o = (u64)i * ld + r.j;
+            res[o]      = resUpdate<T>(s2[(u64)i * lds + r.j], rn[o], res[o], r0[o], mu[r.j],
+                                       alpha1, alpha2, c);
           }
       }
 
@@ -544,6 +555,20 @@ namespace dfx
     }
 
     template <class T>
+    void resFix(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
+                View<const T> r0, const u32 *rows, u32 nRows, const double *mu,
+                const double *coef)
+    {
+      const u64 n = (u64)nRows * res.B;
+      if (!n)
+        return;
+      DFX_REQUIRE(res.ld == rn.ld && rn.ld == r0.ld, "residual fix-up: one leading dimension");
+      resFixK<T><<<gridFor(n), TPB, 0, s>>>(res.p, s2.p, s2.ld, rn.p, r0.p, res.ld, rows, nRows,
+                                            mu, coef, res.B);
+      checkLaunch("resFix");
+    }
+
+    template <class T>
     void resSeed(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> Y,
                  double alpha1)
     {
@@ -653,6 +678,9 @@ namespace dfx
                             double);                                               \
   template void resStep<T>(hipStream_t, View<T>, View<const T>, View<const T>,     \
                            View<const T>, const double *, const double *);         \
+  template void resFix<T>(hipStream_t, View<T>, View<const T>, View<const T>,      \
+                          View<const T>, const u32 *, u32, const double *,         \
+                          const double *);                                         \
   template void resSeed<T>(hipStream_t, View<T>, View<T>, View<T>,                 \
                            View<const double>, double);                            \
   template void resFinal<T>(hipStream_t, View<double>, double, const double *,     \

@@ -54,6 +54,12 @@ namespace dfx
     template <class T>
     void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
                  View<const T> r0, const double *mu, const double *coef);
+    // resStep on the listed rows only (res, rn, r0 share one leading
+    // dimension): the rows a cell apply with the fused step leaves over
+    template <class T>
+    void resFix(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
+                View<const T> r0, const u32 *rows, u32 nRows, const double *mu,
+                const double *coef);
     // residual filter set-up from the FP64 seed Y: r0 = (T) Y, rn = (T)(alpha1 Y
     // + 0 Y), res = 0 (r0, rn, res share one leading dimension)
     template <class T>
@@ -88,10 +94,23 @@ namespace dfx
     // (M^-1 folded into the gather) and the enriched rows from xe
     // nt (optional): the column-chunk width in 16-column tiles (1, 2, 4, 5 or
     // 8) instead of the tuned or built-in choice
+    // the residual Chebyshev step fused into a folded cell apply: on the
+    // classical rows whose last touch is the launch's (and which are not
+    // constrained), res = resStep(y, x, res, r0) is stored instead of y; the
+    // other rows of y are stored as usual and finished by resFix
+    template <class T>
+    struct ResFuse
+    {
+      T            *res;  // leading dimension that of x
+      const T      *r0;   // leading dimension that of x
+      const double *mu;   // one per column
+      const double *coef; // {alpha1, alpha2, c}
+    };
+    // rf (optional, with dsc only): the residual step fused into the launch
     template <class T>
     void cellApply(hipStream_t s, const dfx_op *op, const Launch &L, const T *frag,
                    View<const T> x, View<T> y, T *yE, const double *dsc = nullptr,
-                   const T *xe = nullptr, int nt = 0);
+                   const T *xe = nullptr, int nt = 0, const ResFuse<T> *rf = nullptr);
     // the built-in column-chunk width for B columns and a class of padded K
     // up to kpadMax
     template <class T>

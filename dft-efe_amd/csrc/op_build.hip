@@ -254,6 +254,38 @@ namespace dfx
           }
     }
 
+    // last-touch bits in launch order (the residual step fused into the
+    // cell apply): the last cell to reach an unconstrained classical dof
+    // finishes it; every other row is listed for the stand-alone step
+    std::vector<u32> hLast((size_t)nCells * 8, 0u), hResRows;
+    {
+      // 0: in no cell, 1: finished by a cell, 2: constrained
+      std::vector<unsigned char> st(nClassical, 0);
+      if (cons)
+        for (u32 r : cons->hRows)
+          if (r < nClassical)
+            st[r] = 2;
+      for (u32 q = nCells; q-- > 0;)
+        {
+          const u32 e = op->hOrder[q];
+          for (u32 i = 0; i < nc[e]; ++i)
+            {
+              const u32 d = map[op->hMapOffset[e] + i];
+              if (!st[d])
+                {
+                  st[d] = 1;
+                  hLast[(size_t)q * 8 + i / 32] |= 1u << (i % 32);
+                }
+            }
+        }
+      for (u32 d = 0; d < nClassical; ++d)
+        if (st[d] != 1)
+          hResRows.push_back(d);
+      for (u32 d = nClassical; d < nDofs; ++d)
+        hResRows.push_back(d);
+      op->nResRows = (u32)hResRows.size();
+    }
+
     // enriched dof -> slots (cell order)
     const u32 nE = nDofs - nClassical;
     std::vector<u32> eOff(nE + 1, 0), eSlots(op->nEnrSlots);
@@ -285,6 +317,9 @@ namespace dfx
     upload(op->map, map, sumNe, st);
     upload(op->meta, op->hMeta, st);
     upload(op->first, hFirst, st);
+    upload(op->last, hLast, st);
+    if (!hResRows.empty())
+      upload(op->resRows, hResRows, st);
     upload(op->enrSlotOffsets, eOff, st);
     upload(op->enrSlots, eSlots, st);
     upload(op->dofSlotOffsets, sOff, st);

@@ -207,6 +207,14 @@ namespace dfx
       DFX_HIP(hipGraphLaunch(g.exec, s));
     }
 
+    // DFX_FUSE_RES=0: the residual degree step as a launch of its own after
+    // each apply (read on every filter call)
+    bool fuseResidualStep()
+    {
+      const char *e = getenv("DFX_FUSE_RES");
+      return !e || atoi(e) != 0;
+    }
+
     template <class T>
     void residualFilter(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv,
                         const double *lam, View<double> Xin, u32 m, double a,
@@ -273,14 +281,22 @@ namespace dfx
       View<T> s2 = workView<T>(ctx, "rf.s2", N, nb);
       // ResidualNew = alpha1 * Y ; Residual = 0 (one fused launch)
       k::resSeed<T>(s, r0, rn, rs, cv(Y), rnScale);
+      // the degree step fused into the folded cell apply where it qualifies
+      // (the same bits as the separate step)
+      const bool fuse = fuseResidualStep() && rs.ld == rn.ld && r0.ld == rn.ld;
       auto step = [&](u32 deg) {
-        if (!applyInvThenCellwise<T>(BInv, A, rn, s1, s2))
+        const k::ResFuse<T> rf{rs.p, r0.p, dtab + (size_t)deg * nb,
+                               dtab + coefOff + 3 * (size_t)deg};
+        if (!(fuse && applyInvThenCellwise<T>(BInv, A, rn, s1, s2, &rf)))
           {
-            applyOp<T>(BInv, rn, s1);
-            applyOp<T>(A, s1, s2);
+            if (!applyInvThenCellwise<T>(BInv, A, rn, s1, s2))
+              {
+                applyOp<T>(BInv, rn, s1);
+                applyOp<T>(A, s1, s2);
+              }
+            k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb,
+                          dtab + coefOff + 3 * (size_t)deg);
           }
-        k::resStep<T>(s, rs, cv(s2), cv(rn), cv(r0), dtab + (size_t)deg * nb,
-                      dtab + coefOff + 3 * (size_t)deg);
         std::swap(rn, rs);
       };
       // degree 2 runs directly (first-use allocations and kernel attributes);
@@ -297,7 +313,7 @@ namespace dfx
                                          (uint64_t)(uintptr_t)rn.p, (uint64_t)(uintptr_t)rs.p,
                                          (uint64_t)(uintptr_t)s1.p, (uint64_t)(uintptr_t)s2.p,
                                          (uint64_t)(uintptr_t)r0.p, (uint64_t)(uintptr_t)dtab,
-                                         N, nb, m};
+                                         N, nb, m, fuse ? 1u : 0u};
       const std::string gname = std::string(sizeof(T) == 8 ? "rf.f64." : "rf.f32.") +
                                 std::to_string(nb);
       degreeLoop(ctx, graph, gname, key, {A->uid, BInv->uid}, 3, m, step, [&] {

@@ -175,6 +175,30 @@ def test_c4_fp32_cell_shapes_match_oracle(ctx, c4cells, B):
     assert np.array_equal(Yf.to_numpy(), Yu.to_numpy())
 
 
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+def test_c4_residual_step_fused_is_bit_identical(ctx, c4cells, prec, monkeypatch):
+    """One ChFSI pass in C4's batches (64 + 72 columns, degree 30): the
+    residual step fused into the cell kernels of C4's classes gives the
+    bits of the separate step (DFX_FUSE_RES=0)."""
+    P = c4cells
+    ops = Ops(ctx, P)
+    nv = 136
+    X = guess(P, nv, seed=136)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.1 * (c - a)
+    outs = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("DFX_FUSE_RES", fuse)
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, 30, 1e-14, G, True, 64, prec)
+        eig = np.linspace(a, b, nv)
+        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        outs.append((eig.copy(), V.to_numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_c4_fp64_200_columns_match_oracle(ctx, c4cells):
     """The FP64 applies of C4's CholGS and Rayleigh-Ritz (all 200 states)."""
     P = c4cells

@@ -295,6 +295,43 @@ def test_residual_filter_folded_inverse_is_bit_identical(ctx, probs, name, prec,
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("name", ["c1", "enr"])
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+@pytest.mark.parametrize("nv,bs,use_graph", [(8, 0, False), (72, 0, False), (40, 16, True)])
+def test_residual_step_fused_into_the_apply_is_bit_identical(ctx, probs, name, prec, nv, bs,
+                                                             use_graph, monkeypatch):
+    """The residual filter's degree step runs in the cell kernel's scatter on
+    the rows each cell finishes (its last touch of an unconstrained classical
+    row) and in a pass of its own over the constrained, enriched and
+    cell-less rows; with DFX_FUSE_RES=0 it is a launch of its own after the
+    apply.  Both give the same bits: 8 columns, 72 (one 80-column FP32 chunk;
+    a 64-column FP64 chunk and a narrow tail) and batches of 16 with a
+    remainder of 8, in captured graphs replayed over two passes."""
+    P = probs[name]
+    ops = Ops(ctx, P)
+    X = guess(P, nv, seed=61)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    outs = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("DFX_FUSE_RES", fuse)
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, 9, 1e-14, G, True, bs, prec,
+                                             useGraph=use_graph)
+        eig = np.linspace(a, b, nv)
+        res = []
+        for _ in range(2):
+            assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+            res.append((eig.copy(), V.to_numpy()))
+            G.copy_from(V)
+            s.reinit(eig[0], eig[-1], c, 9, 1e-14, G)
+        outs.append(res)
+    for (e0, v0), (e1, v1) in zip(outs[0], outs[1]):
+        assert np.array_equal(e0, e1) and np.array_equal(v0, v1)
+
+
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
 def test_filter_graph_replays_match_eager(ctx, probs, prec):
     """The residual filter's degree loop captured in a hipGraph and replayed
