@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "dft-efe_amd"))
 PEAK_FP32_MFMA = 157.3   # TFLOP/s, MI355X dense FP32 matrix
 PEAK_FP64_MFMA = 78.6    # TFLOP/s
 PEAK_HBM = 8000.0        # GB/s
+PROF_EVERY = 4           # cell applies timed by events: every 4th of the kind
 
 
 def log(*a):
@@ -412,9 +413,10 @@ def main():
         rc = solver.solve(H, ev, out, True, M, Minv)
         if rc != 0:
             raise RuntimeError(f"ChFSI pass failed: {dfx.EIG_CODES.get(rc, rc)}")
-        # the pass's convergence test: residual norms of every state (FP64)
+        # the pass's convergence test: residual norms of every state (FP64),
+        # from the H X and M X the solver keeps of its output
         tr = time.perf_counter()
-        dfx.eigen_residual_norms(ctx, H, M, out, ev)
+        solver.residual_norms(H, M, out, ev)
         res_ms.append((time.perf_counter() - tr) * 1e3)
         guess.copy_from(out)
         bnd[0], bnd[1] = ev[0], ev[-1]
@@ -430,7 +432,9 @@ def main():
     for _ in range(args.warmup):
         one_pass(mixed, G, eig, V, bounds)
     log(f"warm-up done ({args.warmup} passes); timing {args.steps} passes")
-    H.set_profiling(True, dfx.F32)
+    # every PROF_EVERY-th FP32 apply of the timed passes has its colour steps
+    # bracketed by HIP events (a sample: the events cost GPU time of their own)
+    H.set_profiling(True, dfx.F32, every=PROF_EVERY)
     H.profile(reset=True)
     res_ms.clear()
     ctx.synchronize()
@@ -451,7 +455,7 @@ def main():
     fp64 = dfx.ChebyshevFilteredEigenSolver(ctx, bnd64[0], bnd64[1], c, m, 1e-14, G64, True,
                                             B, dfx.PREC_FP64)
     V64 = dfx.MultiVector(ctx, N, n)
-    H.set_profiling(True, dfx.F64)
+    H.set_profiling(True, dfx.F64, every=PROF_EVERY)
     H.profile(reset=True)
     n64 = max(2, min(args.steps, 4))
     for _ in range(args.warmup):
@@ -570,24 +574,34 @@ def main():
         log(f"CPU baseline {base['value']:.4g} passes/s on {base['cores']} threads")
 
     # ---- roofline of the dominant kernel: the FP32 cell apply of the filter's
-    # degree loop.  A launch unit is one colour step (its size-class launches
-    # side by side), timed with HIP events on the library's stream over the
-    # timed passes.  Algorithmic flops 2 B sum n_e^2 per apply; algorithmic
-    # bytes the fused minimum per apply, s sum n_e^2 + 4 (sum n_e + E) +
-    # 2 s N B + constraint bytes; both split evenly over the colours.
+    # degree loop with the residual step fused in.  A launch unit is one
+    # colour step (its size-class launches side by side), timed with HIP
+    # events on the library's stream over a sample of the timed passes'
+    # applies.  Algorithmic flops 2 B sum n_e^2 per apply; algorithmic bytes
+    # the minimum per apply: cell matrices once (s sum n_e^2), the map
+    # (4 (sum n_e + E)), X read once, and per column of a row finished in the
+    # kernel res and r0 read and res written, of the other rows (constrained
+    # and enriched: the separate pass finishes them) y written, plus the
+    # constraint rows; both split evenly over the colours.
     nrows = len(prob.get("rows"))
     cons_bytes = 4.0 * (2 * nrows + prob.nEntries) + 8.0 * prob.nEntries
+    n_fin = N - nrows - prob.nEnriched
+
+    def apply_bytes(bw):
+        return (4.0 * info["sumNe2"] + 4.0 * (info["sumNe"] + info["nCells"]) +
+                4.0 * N * bw + 3 * 4.0 * n_fin * bw + 4.0 * (N - n_fin) * bw + cons_bytes +
+                4.0 * nrows * bw)
+
     apply_widths = [bw for bw in batch_widths(n, B) for _ in range(m - 1)] * args.steps
-    flops_min = sum(2.0 * bw * info["sumNe2"] for bw in apply_widths)
-    bytes_min = sum(4.0 * info["sumNe2"] + 4.0 * (info["sumNe"] + info["nCells"]) +
-                    2.0 * 4 * N * bw + cons_bytes + 4.0 * nrows * bw for bw in apply_widths)
+    nc = info["nColors"]
+    sampled = apply_widths[::PROF_EVERY]
     launches = prof32["launches"]
-    if launches != len(apply_widths) * info["nColors"] or \
-            abs(prof32["flops"] - flops_min) > 1e-9 * flops_min:
+    flops_sampled = sum(2.0 * bw * info["sumNe2"] for bw in sampled)
+    if launches != len(sampled) * nc or abs(prof32["flops"] - flops_sampled) > 1e-9 * flops_sampled:
         log(f"warning: profiled launches {launches} / flops {prof32['flops']:.4g} differ from "
-            f"the expected {len(apply_widths) * info['nColors']} / {flops_min:.4g}")
-    per_launch_flops = flops_min / max(launches, 1)
-    per_launch_bytes = bytes_min / max(launches, 1)
+            f"the expected {len(sampled) * nc} / {flops_sampled:.4g}")
+    per_launch_flops = flops_sampled / max(len(sampled) * nc, 1)
+    per_launch_bytes = sum(apply_bytes(bw) for bw in sampled) / max(len(sampled) * nc, 1)
     per_launch_ms = prof32["kernelMs"] / max(launches, 1)
     # the roofline that bounds the kernel is the one with the longer ideal time
     t_mfma = per_launch_flops / (PEAK_FP32_MFMA * 1e12)
@@ -633,15 +647,18 @@ def main():
         "serving": serving,
         "poisson": poisson,
         "roofline": {"bound": bound,
-                     "kernel": ("cellApplyBufK<float> (fused gather / FP32 MFMA / scatter); "
-                                "one launch unit = one colour step, its size-class "
-                                "launches running side by side"),
+                     "kernel": ("cellApplyBufK<float> (fused gather / FP32 MFMA / scatter "
+                                "with the residual Chebyshev step); one launch unit = "
+                                "one colour step, its size-class launches running side "
+                                "by side"),
                      "achieved": achieved, "peak": peak, "unit": unit,
                      "frac": achieved / peak, "traffic": traffic,
                      "traffic_over_algorithmic": (traffic / per_launch_bytes
                                                   if traffic else None),
                      "ideal_us_mfma_hbm": [t_mfma * 1e6, t_hbm * 1e6],
                      "launches": prof32["launches"],
+                     "launches_sampled": f"the colour steps of every {PROF_EVERY}th FP32 "
+                                         f"apply of the {args.steps} timed passes",
                      "avg_launch_ms": per_launch_ms,
                      "algorithmic_flops_per_launch": per_launch_flops,
                      "algorithmic_bytes_per_launch": per_launch_bytes,

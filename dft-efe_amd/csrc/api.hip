@@ -80,8 +80,19 @@ namespace dfx
       return !e || atoi(e) != 0;
     }();
     const bool fork = forkEnv && op->ctx->forkClasses;
-    const bool prof = op->profiling && !yEOverride && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
+    const bool profKind =
+      op->profiling && !yEOverride && (op->profMask & (sizeof(T) == 8 ? 1 : 2));
+    const bool prof = profKind && (op->profApplies++ % op->profEvery) == 0;
     ProfRec    r{};
+    // the end event of a colour step is the start event of the next one
+    hipEvent_t prevEnd  = nullptr;
+    auto       newEvent = [&]() {
+      hipEvent_t e = nullptr;
+      DFX_HIP(hipEventCreate(&e));
+      op->profEvents.push_back(e);
+      DFX_HIP(hipEventRecord(e, s));
+      return e;
+    };
     for (size_t li = 0; li < L.size(); ++li)
       {
         const Launch &l     = L[li];
@@ -93,10 +104,8 @@ namespace dfx
         // bracketed on the main stream
         if (prof && first)
           {
-            r = ProfRec{};
-            DFX_HIP(hipEventCreate(&r.a));
-            DFX_HIP(hipEventCreate(&r.b));
-            DFX_HIP(hipEventRecord(r.a, s));
+            r   = ProfRec{};
+            r.a = prevEnd ? prevEnd : newEvent();
           }
         if (fork && first && !last)
           {
@@ -122,7 +131,7 @@ namespace dfx
                        (dsc ? 8.0 * (double)l.sumNc : 0.0);
             if (last)
               {
-                DFX_HIP(hipEventRecord(r.b, s));
+                r.b = prevEnd = newEvent();
                 op->prof.push_back(r);
               }
           }
@@ -746,6 +755,8 @@ extern "C"
       std::vector<char> isRow(nDofs, 0);
       c->hRows.assign(rows, rows + nRows);
       c->hInhom.assign(inhom, inhom + nRows);
+      c->homogeneous = std::all_of(c->hInhom.begin(), c->hInhom.end(),
+                                   [](double v) { return v == 0.0; });
       c->homDirichlet = nEnt == 0 && std::all_of(c->hInhom.begin(), c->hInhom.end(),
                                                  [](double v) { return v == 0.0; });
       c->hRowOffsets.resize(nRows + 1);
@@ -1094,8 +1105,11 @@ extern "C"
   {
     if (!op)
       return DFX_E_ARG;
-    op->profiling = enable != 0;
-    op->profMask  = enable & 3 ? (enable & 3) : 3;
+    op->profiling     = enable != 0;
+    op->profMask      = enable & 3 ? (enable & 3) : 3;
+    const u32 every   = ((u32)enable >> 8) & 0xffffu;
+    op->profEvery     = every ? every : 1;
+    op->profApplies   = 0;
     return 0;
   }
 
@@ -1113,10 +1127,11 @@ extern "C"
           op->profFlops += r.flops;
           op->profBytes += r.bytes;
           ++op->profLaunches;
-          (void)hipEventDestroy(r.a);
-          (void)hipEventDestroy(r.b);
         }
       op->prof.clear();
+      for (hipEvent_t e : op->profEvents)
+        (void)hipEventDestroy(e);
+      op->profEvents.clear();
       out->launches = op->profLaunches;
       out->kernelMs = op->profMs;
       out->flops    = op->profFlops;

@@ -206,6 +206,7 @@ struct dfx_constraints
   dfx_ctx *ctx = nullptr;
   dfx::u32 nDofs = 0, nRows = 0, nEntries = 0, nParents = 0;
   bool homDirichlet = false; // every row is x = 0 (no parents, zero inhomogeneity)
+  bool homogeneous  = true;  // every inhomogeneity is zero (P2C is linear)
   std::vector<dfx::u32> hRows, hRowOffsets, hCols, hParents, hParentOffsets,
     hParentRows, hParentEntries;
   std::vector<double> hWeights, hInhom;
@@ -326,15 +327,15 @@ struct dfx_op
   // ----- profiling of the cell-apply launches
   bool profiling = false;
   int  profMask  = 3; // 1: FP64 launches, 2: FP32 launches
+  dfx::u32 profEvery   = 1; // every k-th apply of the profiled kind is timed
+  dfx::u64 profApplies = 0;
   std::vector<dfx::ProfRec> prof;
+  std::vector<hipEvent_t>   profEvents; // owned; consecutive records share one
   dfx::u64 profLaunches = 0;
   double profMs = 0, profFlops = 0, profBytes = 0;
   ~dfx_op()
   {
-    for (auto &r : prof)
-      {
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
-      }
+    for (hipEvent_t e : profEvents)
+      (void)hipEventDestroy(e);
   }
 };

@@ -56,9 +56,18 @@
 #ifndef DFX_CELL_DEPTH1
 #define DFX_CELL_DEPTH1 1
 #endif
+// whether the thread index is made opaque per column chunk in the kernels
+// without the fused residual step too (always in those with it)
+#ifndef DFX_CELL_OPAQUE
+#define DFX_CELL_OPAQUE 1
+#endif
+// whether the fused kernels request Y's old values per row group too
+#ifndef DFX_RES_OLDG
+#define DFX_RES_OLDG 1
+#endif
 // rows per request group of the fused residual step's operands (1, 2 or 4)
 #ifndef DFX_RES_ROWS
-#define DFX_RES_ROWS 1
+#define DFX_RES_ROWS 2
 #endif
 
 
@@ -285,12 +294,12 @@ namespace dfx
         // for columns past B; a separate instantiation because the second
         // path costs registers
         auto chunk = [&](auto ntl, const u32 c0, const int bw) {
-            // RES: the thread index is made opaque per chunk, so the row
+            // RES (and DFX_CELL_OPAQUE): the thread index is made opaque per chunk, so the row
             // offsets and masks of the gather and the scatter are formed here
             // rather than hoisted out of the chunk loop and held (spilled)
             // throughout
             int lc = l, tx = threadIdx.x;
-            if constexpr (RES)
+            if constexpr (RES || DFX_CELL_OPAQUE)
               asm volatile("" : "+v"(lc), "+v"(tx));
             auto rowOf = [&](int t, int v) { return (w + 4 * t) * 16 + M::row(lc, v); };
             constexpr int NTL   = decltype(ntl)::value;
@@ -409,9 +418,9 @@ namespace dfx
 
             // ---- old values of the classical rows touched before
             T    old[EARLY ? MTW : 1][EARLY ? NTL : 1][4];
-            auto loadOld = [&](int t, T (*o)[4]) {
+            auto loadOld = [&](int t, T (*o)[4], int v0 = 0, int nv = 4) {
 #pragma unroll
-              for (int vv = 0; vv < 4; ++vv)
+              for (int vv = v0; vv < v0 + nv; ++vv)
                 {
                   // row < 64 MTW <= 256: sMap / sFirst reads stay in the arrays
                   const int  row  = rowOf(t, vv);
@@ -520,7 +529,9 @@ namespace dfx
                   continue;
                 T  oldLate[EARLY ? 1 : NTL][4];
                 T(*o)[4] = EARLY ? old[EARLY ? t : 0] : oldLate;
-                if constexpr (!EARLY)
+                // with RES the old values are requested per row group
+                constexpr bool OLDG = RES && !EARLY && DFX_RES_OLDG;
+                if constexpr (!EARLY && !OLDG)
                   loadOld(t, oldLate);
                 const bool enr = (w + 4 * t) * 16 + 15 >= nc; // wave-uniform
                 // rows in groups of RG: the fused step's operands (0 on the
@@ -530,6 +541,8 @@ namespace dfx
 #pragma unroll
                 for (int vg = 0; vg < 4; vg += RG)
                   {
+                    if constexpr (OLDG)
+                      loadOld(t, oldLate, vg, RG);
                     T    fx[RES ? NTL : 1][RG], fr[RES ? NTL : 1][RG], f0[RES ? NTL : 1][RG];
                     bool fin[RG];
 #pragma unroll

@@ -4,6 +4,8 @@
 // N x B views with a leading dimension (column blocks of a wider multivector
 // are addressed in place, without the reference's per-dof copy loops,
 // src/linearAlgebra/ChebyshevFilteredEigenSolver.t.cpp:246-333).
+#include <vector>
+
 #include "kernels.hpp"
 #include "res_update.hpp"
 
@@ -264,6 +266,34 @@ namespace dfx
               o[e] = resUpdate<T>(a[e], b[e], o[e], z[e], mu[j0 + e], alpha1, alpha2, c);
             reinterpret_cast<vt *>(res)[t] = o;
           }
+      }
+
+      // content hash of n 64-bit words: the wrapping sum over i of
+      // mix(x[i] + (i + 1) golden), mix the splitmix64 finalizer; a sum, so
+      // the value does not depend on the order the blocks add up in
+      __device__ inline u64 mix64(u64 z)
+      {
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+      }
+      __global__ __launch_bounds__(256) void hashWordsK(const u64 *__restrict__ x, u64 n,
+                                                        u64 *__restrict__ partials)
+      {
+        __shared__ u64 sm[256];
+        u64            acc = 0;
+        for (u64 i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (u64)gridDim.x * 256)
+          acc += mix64(x[i] + (i + 1) * 0x9e3779b97f4a7c15ull);
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1)
+          {
+            if ((int)threadIdx.x < w)
+              sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+          }
+        if (threadIdx.x == 0)
+          partials[blockIdx.x] = sm[0];
       }
 
       // the residual step on the rows a fused cell apply leaves to it (listed
@@ -552,6 +582,26 @@ namespace dfx
       resStepK<T><<<gridFor(n), TPB, 0, s>>>(res.p, res.ld, s2.p, s2.ld, rn.p, rn.ld,
                                              r0.p, r0.ld, mu, coef, res.N, res.B);
       checkLaunch("resStep");
+    }
+
+    u64 hashWords(hipStream_t s, const void *x, u64 nWords, u64 *partials)
+    {
+      if (!nWords)
+        return 0;
+      u64 blocks = (nWords + 255) / 256;
+      if (blocks > HASH_BLOCKS)
+        blocks = HASH_BLOCKS;
+      hashWordsK<<<(unsigned)blocks, 256, 0, s>>>(static_cast<const u64 *>(x), nWords,
+                                                  partials);
+      checkLaunch("hashWords");
+      std::vector<u64> h(blocks);
+      DFX_HIP(hipMemcpyAsync(h.data(), partials, blocks * sizeof(u64), hipMemcpyDeviceToHost,
+                             s));
+      DFX_HIP(hipStreamSynchronize(s));
+      u64 sum = 0;
+      for (u64 v : h)
+        sum += v;
+      return sum;
     }
 
     template <class T>

@@ -54,6 +54,11 @@ namespace dfx
     template <class T>
     void resStep(hipStream_t s, View<T> res, View<const T> s2, View<const T> rn,
                  View<const T> r0, const double *mu, const double *coef);
+    // content hash of nWords 64-bit words (synchronizes s); partials holds
+    // HASH_BLOCKS words.  Equal contents give equal hashes; a change of any
+    // word changes it but for a 2^-64 chance
+    constexpr u64 HASH_BLOCKS = 2048;
+    u64 hashWords(hipStream_t s, const void *x, u64 nWords, u64 *partials);
     // resStep on the listed rows only (res, rn, r0 share one leading
     // dimension): the rows a cell apply with the fused step leaves over
     template <class T>

@@ -24,6 +24,14 @@ struct dfx_chfsi
   // subspaces of the last solve (N x n FP64)
   dfx::DevBuf      filtered, filteredOrtho;
   dfx::u32         storedN = 0, storedB = 0;
+  // H X and M X of the last solve's eigenvectors X (FP64 N x n), formed by
+  // rotating the products Cholesky-Gram-Schmidt (M X_f) and Rayleigh-Ritz
+  // (H X_o) compute anyway; valid for the operators (uid, generation) and
+  // the content hash of X they were made with
+  dfx::DevBuf      kpHX, kpMX;
+  bool             kpValid = false;
+  uint64_t         kpHash = 0, kpA = 0, kpAGen = 0, kpB = 0, kpBGen = 0;
+  dfx::u32         kpN = 0, kpn = 0;
 };
 
 namespace dfx
@@ -215,10 +223,15 @@ namespace dfx
       return !e || atoi(e) != 0;
     }
 
+    // AXin / BXin (optional, leading dimension ld): A Xin and B Xin already
+    // formed (the products kept from the solve that made Xin); the seed then
+    // takes them instead of two applies
     template <class T>
     void residualFilter(dfx_ctx *ctx, dfx_op *A, dfx_op *Bop, dfx_op *BInv,
                         const double *lam, View<double> Xin, u32 m, double a,
-                        double b, double c, View<double> Y, bool useGraph = false)
+                        double b, double c, View<double> Y, bool useGraph = false,
+                        const double *AXin = nullptr, const double *BXin = nullptr,
+                        u32 ld = 0)
     {
       hipStream_t s  = ctx->stream;
       const u32   N  = Xin.N, nb = Xin.B;
@@ -269,10 +282,16 @@ namespace dfx
       const double *dLam  = dOnes + nb;
 
       // ---- FP64 seed: Y = B X, S3 = A X, Y = S3 - lambda o Y
-      View<double> s3 = workView<double>(ctx, "rf.s3", N, nb);
-      applyOp<double>(Bop, Xin, Y);
-      applyOp<double>(A, Xin, s3);
-      k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(s3), -1.0, dLam, cv(Y));
+      if (AXin && BXin)
+        k::axpbyBlocked<double>(s, Y, 1.0, dOnes, View<const double>{AXin, N, nb, ld}, -1.0,
+                                dLam, View<const double>{BXin, N, nb, ld});
+      else
+        {
+          View<double> s3 = workView<double>(ctx, "rf.s3", N, nb);
+          applyOp<double>(Bop, Xin, Y);
+          applyOp<double>(A, Xin, s3);
+          k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(s3), -1.0, dLam, cv(Y));
+        }
 
       View<T> r0 = workView<T>(ctx, "rf.r0", N, nb);
       View<T> rn = workView<T>(ctx, "rf.rn", N, nb);
@@ -385,12 +404,14 @@ namespace dfx
       return 0;
     }
 
+    // hxName: the work buffer that receives H X (kept past the call when it
+    // is not CholGS's "cg.mx")
     int rayleighRitz(dfx_ctx *ctx, dfx_op *A, View<double> X, double *eig,
-                     View<double> Xo)
+                     View<double> Xo, const char *hxName = "cg.mx")
     {
       hipStream_t s = ctx->stream;
       const u32   N = X.N, n = X.B;
-      View<double> HX = workView<double>(ctx, "cg.mx", N, n);
+      View<double> HX = workView<double>(ctx, hxName, N, n);
       applyOp<double>(A, X, HX);
       double *S    = workDoubles(ctx, "rr.S", (size_t)n * n);
       double *part = workDoubles(ctx, "gram.part", k::gramPartialsNeeded(N, n));
@@ -422,7 +443,67 @@ namespace dfx
       return View<double>{static_cast<double *>(m->d), m->N, m->B, m->B};
     }
 
-    // ChebyshevFilteredEigenSolver::solve (ChebyshevFilteredEigenSolver.t.cpp:194-437)
+    // content hash of an N x n FP64 view (contiguous)
+    u64 hashOf(dfx_ctx *ctx, View<const double> X)
+    {
+      DFX_REQUIRE(X.ld == X.B, "hash of a contiguous multivector");
+      u64 *part = work(ctx, "hash.part", k::HASH_BLOCKS * sizeof(u64)).as<u64>();
+      return k::hashWords(ctx->stream, X.p, (u64)X.N * X.B, part);
+    }
+
+    // A X and B X of a solve's eigenvectors can be kept when both operators
+    // are linear in X: constraints (if any) without inhomogeneities
+    bool productsUsable(const dfx_op *A, const dfx_op *Bop)
+    {
+      return (!A->cons || A->cons->homogeneous) && (!Bop->cons || Bop->cons->homogeneous);
+    }
+
+    bool productsFor(const dfx_chfsi *cs, const dfx_op *A, const dfx_op *Bop, u32 N, u32 n)
+    {
+      return cs->kpValid && cs->kpA == A->uid && cs->kpAGen == A->gen && cs->kpB == Bop->uid &&
+             cs->kpBGen == Bop->gen && cs->kpN == N && cs->kpn == n;
+    }
+
+    // C = A B, n x n column-major, device
+    void gemmSquare(dfx_ctx *ctx, const double *A, const double *B, double *C, u32 n)
+    {
+      const double one = 1.0, zero = 0.0;
+      CaptureLock  lk(captureMutex());
+      rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+      if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
+                        (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, &one, A, (rocblas_int)n,
+                        B, (rocblas_int)n, &zero, C, (rocblas_int)n) != rocblas_status_success)
+        throw Error(DFX_E_HIP, "rocblas_dgemm failed");
+    }
+
+    // res_j = || A x_j - lambda_j B x_j || from the kept products
+    void productResidualNorms(dfx_chfsi *cs, const double *eig, double *res)
+    {
+      dfx_ctx    *ctx = cs->ctx;
+      const u32   N = cs->kpN, n = cs->kpn;
+      View<double> r = workView<double>(ctx, "rn.x", N, n);
+      std::vector<double> h(2 * (size_t)n);
+      for (u32 j = 0; j < n; ++j)
+        {
+          h[j]     = -1.0;
+          h[n + j] = eig[j];
+        }
+      double *d = workDoubles(ctx, "rn.coef", 2 * (size_t)n);
+      toDevice(ctx, d, h.data(), h.size());
+      k::axpbyBlocked<double>(ctx->stream, r, 1.0, d,
+                              View<const double>{cs->kpHX.as<double>(), N, n, n}, 1.0, d + n,
+                              View<const double>{cs->kpMX.as<double>(), N, n, n});
+      colDotsView<double>(ctx, cv(r), cv(r), res);
+      for (u32 j = 0; j < n; ++j)
+        res[j] = std::sqrt(res[j]);
+    }
+
+    // ChebyshevFilteredEigenSolver::solve (ChebyshevFilteredEigenSolver.t.cpp:194-437).
+    // Unless p.noProductReuse, the solve keeps A X and B X of its eigenvectors
+    // (two rotations of products it forms anyway); a later solve whose guess
+    // has the same content (hash) takes them for the residual filter's seed
+    // A X - B X Lambda instead of two applies, and the residual norms of X
+    // take them instead of two more.
     int chfsiSolve(dfx_chfsi *cs, dfx_op *A, double *eig, dfx_mv *X, dfx_op *Bop,
                    dfx_op *BInv)
     {
@@ -433,6 +514,10 @@ namespace dfx
       DFX_REQUIRE(G.N == XV.N && G.B == XV.B, "guess and eigenvectors differ in shape");
       DFX_REQUIRE(A->nDofs == XV.N, "operator size does not match the eigenvectors");
       const u32 n  = XV.B;
+      const bool keep  = !p.noProductReuse && productsUsable(A, Bop);
+      const bool reuse = keep && p.isResidualChebyshevFilter && productsFor(cs, A, Bop, G.N, n) &&
+                         hashOf(ctx, cv(G)) == cs->kpHash;
+      cs->kpValid = false;
       const u32 bs = (p.eigenVectorBatchSize == 0 || p.eigenVectorBatchSize > n)
                        ? n
                        : p.eigenVectorBatchSize;
@@ -450,14 +535,18 @@ namespace dfx
           View<double> Gin = sub(G, s0, nb), Xo = sub(XV, s0, nb);
           if (p.isResidualChebyshevFilter)
             {
+              const double *ax = reuse ? cs->kpHX.as<double>() + s0 : nullptr;
+              const double *bx = reuse ? cs->kpMX.as<double>() + s0 : nullptr;
               if (mixed)
                 residualFilter<float>(ctx, A, Bop, BInv, eig + s0, Gin, p.polynomialDegree,
                                       p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
-                                      p.unWantedSpectrumUpperBound, Xo, p.useGraph != 0);
+                                      p.unWantedSpectrumUpperBound, Xo, p.useGraph != 0, ax,
+                                      bx, n);
               else
                 residualFilter<double>(ctx, A, Bop, BInv, eig + s0, Gin, p.polynomialDegree,
                                        p.wantedSpectrumLowerBound, p.wantedSpectrumUpperBound,
-                                       p.unWantedSpectrumUpperBound, Xo, p.useGraph != 0);
+                                       p.unWantedSpectrumUpperBound, Xo, p.useGraph != 0, ax,
+                                       bx, n);
             }
           else
             {
@@ -491,7 +580,34 @@ namespace dfx
       if (oc != 0)
         return DFX_EIG_CHFSI_ORTHONORMALIZATION_ERROR;
       Timer tr(ctx->stream);
-      int rc = rayleighRitz(ctx, A, G, eig, XV);
+      int rc = rayleighRitz(ctx, A, G, eig, XV, keep ? "rr.hx" : "cg.mx");
+      if (rc == 0 && keep)
+        {
+          // X = X_f Qc S (Qc: CholGS's L^-T, S: the Ritz vectors), so
+          // A X = (A X_o) S and B X = (B X_f)(Qc S)
+          hipStream_t  s    = ctx->stream;
+          const u32    N    = XV.N;
+          const size_t by   = (size_t)N * n * sizeof(double);
+          double      *Qc   = workDoubles(ctx, "cg.Q", (size_t)n * n);
+          double      *S    = workDoubles(ctx, "rr.S", (size_t)n * n);
+          double      *W    = workDoubles(ctx, "kp.W", (size_t)n * n);
+          double      *qr   = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
+          gemmSquare(ctx, Qc, S, W, n);
+          cs->kpHX.ensure(by);
+          cs->kpMX.ensure(by);
+          k::rotate<double>(s, cv(workView<double>(ctx, "rr.hx", N, n)), S,
+                            View<double>{cs->kpHX.as<double>(), N, n, n}, qr, ctx->blas);
+          k::rotate<double>(s, cv(workView<double>(ctx, "cg.mx", N, n)), W,
+                            View<double>{cs->kpMX.as<double>(), N, n, n}, qr, ctx->blas);
+          cs->kpHash  = hashOf(ctx, cv(XV));
+          cs->kpA     = A->uid;
+          cs->kpAGen  = A->gen;
+          cs->kpB     = Bop->uid;
+          cs->kpBGen  = Bop->gen;
+          cs->kpN     = N;
+          cs->kpn     = n;
+          cs->kpValid = true;
+        }
       cs->ms[2] = tr.stop();
       if (rc != 0)
         return DFX_EIG_CHFSI_RAYLEIGH_RITZ_ERROR;
@@ -849,7 +965,11 @@ extern "C"
               break;
             }
           auto tr = std::chrono::steady_clock::now();
-          eigenResidualNorms(ctx, H, M, mvView(waveFns), energies, res.data());
+          // waveFns is the solve's output, so the kept products are its own
+          if (productsFor(&cs, H, M, N, n))
+            productResidualNorms(&cs, energies, res.data());
+          else
+            eigenResidualNorms(ctx, H, M, mvView(waveFns), energies, res.data());
           R.residualMs += std::chrono::duration<double, std::milli>(
                             std::chrono::steady_clock::now() - tr).count();
           k::convert<double, double>(ctx->stream, mvView(guess), cv(mvView(waveFns)));
@@ -909,6 +1029,22 @@ extern "C"
       View<double> x = mvView(X);
       DFX_REQUIRE(H->nDofs == x.N && M->nDofs == x.N, "operator size does not match X");
       eigenResidualNorms(ctx, H, M, x, eigenValues, residualNorms);
+      return 0;
+    });
+  }
+
+  int dfx_chfsi_residual_norms(dfx_chfsi *cs, dfx_op *H, dfx_op *M, dfx_mv *X,
+                               const double *eigenValues, double *residualNorms)
+  {
+    if (!cs || !H || !M || !X || !eigenValues || !residualNorms)
+      return DFX_E_ARG;
+    return guard(cs->ctx, [&] {
+      View<double> x = mvView(X);
+      DFX_REQUIRE(H->nDofs == x.N && M->nDofs == x.N, "operator size does not match X");
+      if (productsFor(cs, H, M, x.N, x.B) && hashOf(cs->ctx, cv(x)) == cs->kpHash)
+        productResidualNorms(cs, eigenValues, residualNorms);
+      else
+        eigenResidualNorms(cs->ctx, H, M, x, eigenValues, residualNorms);
       return 0;
     });
   }

@@ -125,6 +125,7 @@ _sig("dfx_rayleigh_ritz", _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_lanczos_extreme", _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp)
 _sig("dfx_ks_eigensolve", _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_eigen_residual_norms", _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("dfx_chfsi_residual_norms", _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("dfx_cg_solve", _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _u32, _vp, _vp)
 _sig("dfx_op_set_profiling", _vp, C.c_int)
 _sig("dfx_op_autotune", _vp, C.c_int, _u32, _u32, _u32, _vp, _vp, _vp)
@@ -373,10 +374,13 @@ class _Operator:
         _lib.dfx_op_get_info(self.h, C.byref(i))
         return {k: getattr(i, k) for k, _ in _OpInfo._fields_}
 
-    def set_profiling(self, enable=True, precision=None):
+    def set_profiling(self, enable=True, precision=None, every=1):
         """Time this operator's cell-apply kernel launches; precision F64 / F32
-        restricts to launches of that value type."""
+        restricts to launches of that value type; every = k times the colour
+        steps of every k-th such apply only."""
         mode = 0 if not enable else (3 if precision is None else (1 if precision == F64 else 2))
+        if mode and int(every) > 1:
+            mode |= (int(every) & 0xffff) << 8
         _lib.dfx_op_set_profiling(self.h, mode)
 
     def autotune(self, dtype, B, reps=3):
@@ -472,7 +476,8 @@ class _ChfsiParams(C.Structure):
                 ("isResidualChebyshevFilter", C.c_int),
                 ("eigenVectorBatchSize", C.c_uint32), ("precision", C.c_int),
                 ("useGraph", C.c_int), ("orthogonalizationType", C.c_int),
-                ("storeIntermediateSubspaces", C.c_int)]
+                ("storeIntermediateSubspaces", C.c_int),
+                ("noProductReuse", C.c_int)]
 
 
 # linearAlgebra::OrthogonalizationType (ChebyshevFilteredEigenSolver.h:45-50)
@@ -486,13 +491,15 @@ class ChebyshevFilteredEigenSolver:
                  unWantedSpectrumUpperBound, polynomialDegree, illConditionTolerance,
                  eigenSubspaceGuess, isResidualChebyshevFilter=True,
                  eigenVectorBatchSize=0, precision=PREC_FP64, useGraph=False,
-                 orthoType=ORTHO_CHOLESKY_GRAMSCHMIDT, storeIntermediateSubspaces=False):
+                 orthoType=ORTHO_CHOLESKY_GRAMSCHMIDT, storeIntermediateSubspaces=False,
+                 productReuse=True):
         self.ctx = ctx
         p = _ChfsiParams(wantedSpectrumLowerBound, wantedSpectrumUpperBound,
                          unWantedSpectrumUpperBound, int(polynomialDegree),
                          illConditionTolerance, 1 if isResidualChebyshevFilter else 0,
                          int(eigenVectorBatchSize), int(precision), 1 if useGraph else 0,
-                         int(orthoType), 1 if storeIntermediateSubspaces else 0)
+                         int(orthoType), 1 if storeIntermediateSubspaces else 0,
+                         0 if productReuse else 1)
         h = C.c_void_p()
         ctx.check(_lib.dfx_chfsi_create(ctx.h, C.byref(p), eigenSubspaceGuess.h, C.byref(h)))
         self.h = h
@@ -529,6 +536,16 @@ class ChebyshevFilteredEigenSolver:
 
     def getOrthogonalizedFilteredSubspace(self):
         return self._subspace(1)
+
+    def residual_norms(self, H, M, X, eigenValues):
+        """||H x_j - lambda_j M x_j|| per column of the last solve's
+        eigenvectors X: from the solver's kept H X and M X when X still holds
+        that output, else by applying H and M."""
+        ev = _f64a(eigenValues)
+        out = np.zeros(X.B)
+        self.ctx.check(_lib.dfx_chfsi_residual_norms(self.h, H.h, M.h, X.h, _ptr(ev),
+                                                     _ptr(out)))
+        return out
 
     def timings(self):
         out = np.zeros(3)

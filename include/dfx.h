@@ -236,7 +236,9 @@ typedef struct dfx_op_profile
   double   flops;
   double   bytes;
 } dfx_op_profile;
-/* enable: 0 off; 1 FP64 launches; 2 FP32 launches; 3 both */
+/* enable bits 0-1: 0 off; 1 FP64 launches; 2 FP32 launches; 3 both.
+ * bits 8-23: k, time every k-th apply of that kind only (0 or 1: every apply);
+ * consecutive colour steps of an apply share one event between them. */
 int dfx_op_set_profiling(dfx_op *op, int enable);
 int dfx_op_get_profile(dfx_op *op, dfx_op_profile *out, int reset);
 /* Column-chunk autotuner.  No counterpart in the reference, whose cell-block
@@ -290,6 +292,14 @@ typedef struct dfx_chfsi_params
    * (storeIntermediateSubspaces, ChebyshevFilteredEigenSolver.t.cpp:70-78,
    * 336-381), read back with dfx_chfsi_get_filtered_subspace */
   int      storeIntermediateSubspaces;
+  /* 0 (default): the solver keeps H X and M X of its eigenvectors (two
+   * rotations of products Cholesky-Gram-Schmidt and Rayleigh-Ritz form
+   * anyway; two N x n FP64 arrays).  A later solve whose guess has the same
+   * content takes them for the residual filter's seed, and
+   * dfx_chfsi_residual_norms for the norms, instead of applying H and M
+   * again (equal up to FP64 rounding).  Needs constraints without
+   * inhomogeneities; 1: apply H and M every time, as the reference does. */
+  int      noProductReuse;
 } dfx_chfsi_params;
 #define DFX_ORTHO_CHOLESKY_GRAMSCHMIDT 0
 #define DFX_ORTHO_MULTIPASS_LOWDIN     1
@@ -364,6 +374,11 @@ typedef struct dfx_ks_result
  * KohnShamEigenSolver::getLinearEigenSolveResidual (src/ksdft/
  * KohnShamEigenSolver.t.cpp:574-684), the convergence test of every pass */
 int dfx_eigen_residual_norms(dfx_ctx *ctx, dfx_op *H, dfx_op *M, dfx_mv *X,
+                             const double *eigenValues, double *residualNorms);
+/* The same norms for the eigenvectors X of the solver's last solve: from the
+ * kept H X and M X when X still holds that solve's output (content hash) and
+ * H, M are the operators it used; otherwise as dfx_eigen_residual_norms. */
+int dfx_chfsi_residual_norms(dfx_chfsi *s, dfx_op *H, dfx_op *M, dfx_mv *X,
                              const double *eigenValues, double *residualNorms);
 /* residualNorms: host, length numWantedEigenvalues (may be NULL) */
 int dfx_ks_eigensolve(dfx_ctx *ctx, const dfx_ks_params *p, dfx_op *H,

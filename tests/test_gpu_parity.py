@@ -332,6 +332,62 @@ def test_residual_step_fused_into_the_apply_is_bit_identical(ctx, probs, name, p
         assert np.array_equal(e0, e1) and np.array_equal(v0, v1)
 
 
+@pytest.mark.parametrize("name", ["c1", "enr"])
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec):
+    """A solver keeps H X and M X of its eigenvectors (rotated products of
+    CholGS and Rayleigh-Ritz).  Three passes, each from the last output:
+    with the kept products (seed and residual norms) the eigenvalues agree
+    with a solver that applies H and M every time to FP64 rounding (the
+    eigenvectors are not compared: the block boundary may split a degenerate
+    level), and the residual norms with dfx_eigen_residual_norms; a guess whose content
+    changed, or an operator whose cell matrices were replaced, is not served
+    from them (the same bits as the solver without reuse)."""
+    P = probs[name]
+    ops = Ops(ctx, P)
+    nv, m = 12, 9
+    X = guess(P, nv, seed=71)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    runs = {}
+    for reuse in (True, False):
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, m, 1e-14, G, True, 8, prec,
+                                             productReuse=reuse)
+        eig = np.linspace(a, b, nv)
+        out = []
+        for _ in range(3):
+            assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+            r = s.residual_norms(ops.H, ops.M, V, eig)
+            ref = dfx.eigen_residual_norms(ctx, ops.H, ops.M, V, eig)
+            assert np.all(np.abs(r - ref) <= 1e-10 * max(ref.max(), 1.0))
+            out.append((eig.copy(), V.to_numpy(), r))
+            G.copy_from(V)
+            s.reinit(eig[0], eig[-1], c, m, 1e-14, G)
+        runs[reuse] = out
+    for (e0, v0, r0), (e1, v1, r1) in zip(runs[True], runs[False]):
+        assert np.abs(e0 - e1).max() <= 1e-11 * max(1.0, np.abs(e1).max())
+        assert np.all(np.abs(r0 - r1) <= 1e-9 * max(r1.max(), 1.0))
+    # a changed guess is not served from the kept products
+    outs = []
+    for reuse in (True, False):
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, m, 1e-14, G, True, 8, prec,
+                                             productReuse=reuse)
+        eig = np.linspace(a, b, nv)
+        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        Vn = V.to_numpy()
+        Vn[:, 3] *= 1.0 + 2.0 ** -40
+        G2 = mv(ctx, Vn)
+        s.reinit(eig[0], eig[-1], c, m, 1e-14, G2)
+        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        outs.append((eig.copy(), V.to_numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
 def test_filter_graph_replays_match_eager(ctx, probs, prec):
     """The residual filter's degree loop captured in a hipGraph and replayed
