@@ -65,7 +65,11 @@
 #ifndef DFX_RES_OLDG
 #define DFX_RES_OLDG 1
 #endif
-// rows per request group of the fused residual step's operands (1, 2 or 4)
+// rows per request group of the fused residual step's operands (1, 2 or 4),
+// for cells of up to 128 dofs and for larger ones
+#ifndef DFX_RES_ROWS_SMALL
+#define DFX_RES_ROWS_SMALL 2
+#endif
 #ifndef DFX_RES_ROWS
 #define DFX_RES_ROWS 2
 #endif
@@ -182,6 +186,8 @@ namespace dfx
                                     : mid   ? DFX_CELL_MINW_MID
                                             : DFX_CELL_MINW;
         static constexpr bool early = small ? DFX_CELL_EARLY_SMALL : DFX_CELL_EARLY;
+        // rows per request group of the fused residual step's operands
+        static constexpr int resRows = MTW <= 2 ? DFX_RES_ROWS_SMALL : DFX_RES_ROWS;
       };
 
       // RES (with DSC): the residual Chebyshev step fused into the scatter.
@@ -537,7 +543,7 @@ namespace dfx
                 // rows in groups of RG: the fused step's operands (0 on the
                 // rows it skips) are requested per group, then the group is
                 // stored
-                constexpr int RG = RES ? DFX_RES_ROWS : 4;
+                constexpr int RG = RES ? CellPolicy<T, NT, MTW>::resRows : 4;
 #pragma unroll
                 for (int vg = 0; vg < 4; vg += RG)
                   {

@@ -98,7 +98,9 @@ namespace dfx
       const u64 nQ = nq * nq * nq;
       const u64 mats  = 3 * 8 * sumN2 + 3 * 12 * (sumN2 + sumN2 / 4);
       const u64 setup = std::min<u64>(6ull << 30, nCells * nQ * maxN * 8 * 9);
-      const u64 vecs  = 16 * N * (u64)r.numStates * 8;
+      // N x n FP64 arrays: subspaces, filter and CholGS / RR work, and the
+      // H X, M X the solver keeps of its eigenvectors
+      const u64 vecs  = 18 * N * (u64)r.numStates * 8;
       return (mats + setup + vecs) * 5 / 4 + (64ull << 20);
     }
 

@@ -73,6 +73,34 @@ extern "C"
       }
   }
 
+  // a classical mesh (no atoms, no refinement) whose H_e carries the affine
+  // test potential V(x) = v[0] + v[1..3] . x
+  void *orc_build_affine(int p, const int *n, const double *lo, const double *hi, int nq,
+                         const double *v)
+  {
+    try
+      {
+        MeshSpec s;
+        s.p = p;
+        for (int d = 0; d < 3; ++d)
+          {
+            s.n[d]  = n[d];
+            s.lo[d] = lo[d];
+            s.hi[d] = hi[d];
+          }
+        s.nq      = nq;
+        s.affineV = true;
+        for (int i = 0; i < 4; ++i)
+          s.vAffine[i] = v[i];
+        return new Problem(buildProblem(s));
+      }
+    catch (const std::exception &e)
+      {
+        g_err = e.what();
+        return nullptr;
+      }
+  }
+
   void *orc_build(int p, const int *n, const double *lo, const double *hi,
                   const int *refLo, const int *refHi, int nq, int nqEnr,
                   int nAtoms, const double *atomXYZ, const int *atomZ,

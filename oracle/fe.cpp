@@ -752,7 +752,9 @@ namespace oracle
                 double X[3] = {lo[0] + h[0] * (qx[qa] + 1.0) * 0.5,
                                lo[1] + h[1] * (qx[qb] + 1.0) * 0.5,
                                lo[2] + h[2] * (qx[qc] + 1.0) * 0.5};
-                V[q]        = potentialAt(S.atoms, S.vSmear, X);
+                V[q]        = S.affineV ? S.vAffine[0] + S.vAffine[1] * X[0] +
+                                         S.vAffine[2] * X[1] + S.vAffine[3] * X[2]
+                                      : potentialAt(S.atoms, S.vSmear, X);
                 for (int c = 0; c < np; ++c)
                   for (int b = 0; b < np; ++b)
                     for (int a = 0; a < np; ++a)

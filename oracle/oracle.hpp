@@ -63,6 +63,10 @@ namespace oracle
     double smooth  = 2.0;        // support ends at rcut (1 + 1/smooth)
     bool   orthogonalize = false;// L2-orthogonalize enrichment against the classical space
     bool   keepBasis = false;    // keep basis values and JxW at the quadrature points
+    // test potential: V(x) = vAffine[0] + vAffine[1..3] . x in place of the
+    // atoms' potential (pins the potential block of H_e in the tests)
+    bool   affineV = false;
+    double vAffine[4] = {0, 0, 0, 0};
   };
 
   // Flattened constraints, same arrays as

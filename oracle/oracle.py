@@ -33,6 +33,7 @@ def lib():
         L.orc_error.restype = C.c_char_p
         L.orc_build.restype = C.c_void_p
         L.orc_build_ex.restype = C.c_void_p
+        L.orc_build_affine.restype = C.c_void_p
         L.orc_from_arrays.restype = C.c_void_p
         L.orc_time_apply.restype = C.c_double
         for name in ("orc_free", "orc_sizes", "orc_get_u32", "orc_get_f64",
@@ -185,6 +186,18 @@ class Problem:
                                C.c_int(1 if enrich else 0), C.c_double(rcut),
                                C.c_double(smooth), C.c_int(1 if orthogonalize else 0),
                                C.c_int(1 if keep_basis else 0))
+        P = cls(h)
+        P.p = p
+        return P
+
+    @classmethod
+    def build_affine(cls, v, p=3, n=(2, 2, 2), lo=(-1., -1., -1.), hi=(1., 1., 1.), nq=0):
+        """A classical mesh whose H_e = K_e / 2 + int N_i V N_j carries the
+        affine test potential V(x) = v[0] + v[1:4] . x."""
+        n = np.array(n, dtype=np.int32)
+        lo = np.array(lo, dtype=np.float64); hi = np.array(hi, dtype=np.float64)
+        v = np.array(v, dtype=np.float64)
+        h = lib().orc_build_affine(C.c_int(p), _p(n), _p(lo), _p(hi), C.c_int(nq), _p(v))
         P = cls(h)
         P.p = p
         return P

@@ -131,6 +131,94 @@ def test_hanging_constraints_reproduce_polynomials(p):
 
 
 # -------------------------------------------------------- cell matrices
+def _lagrange_1d(nodes, x):
+    """L_a(x) for the Lagrange polynomials on `nodes`, shape (len(nodes), len(x))."""
+    L = np.ones((len(nodes), len(x)))
+    for a, xa in enumerate(nodes):
+        for b, xb in enumerate(nodes):
+            if b != a:
+                L[a] *= (x - xb) / (xa - xb)
+    return L
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_potential_block_constant_and_linear(p):
+    """The potential block of H_e (the local potential's int N_i V N_j,
+    src/ksdft/KohnShamOperatorContextFE.t.cpp, computed on the cell's Gauss
+    rule of p + 1 points per direction) pinned by two potentials:
+    V = 1.7 must give exactly 1.7 M_e, and V = v0 + g . x, whose integrand
+    N_i V N_j is of degree 2p + 1 per direction, must equal its exact
+    integral, formed here from 1-D factors on a 12-point Gauss rule in
+    numpy (the nodes of N are the GLL points).  A missing factor, a wrong
+    Jacobian or a potential evaluated at the wrong physical points fails."""
+    lo, hi = (-1.5, -0.5, 0.25), (2.5, 1.5, 3.25)
+    P = O.Problem.build_affine([1.7, 0, 0, 0], p=p, n=(2, 3, 2), lo=lo, hi=hi)
+    d = P.arrays()
+    V = d["H"] - 0.5 * d["K"]
+    assert np.abs(V - 1.7 * d["M"]).max() <= 1e-13 * np.abs(d["M"]).max()
+    v = np.array([0.3, -1.1, 0.7, 2.3])
+    P = O.Problem.build_affine(v, p=p, n=(2, 3, 2), lo=lo, hi=hi)
+    d = P.arrays()
+    V = d["H"] - 0.5 * d["K"]
+    np1 = p + 1
+    nodes, _ = O.gll(np1)
+    xq, wq = np.polynomial.legendre.leggauss(12)
+    L = _lagrange_1d(nodes, xq)
+    M1 = (L * wq) @ L.T                       # int L_a L_b over [-1, 1]
+    X1 = (L * wq * xq) @ L.T                  # int xi L_a L_b
+    clo, ch = P.cell_boxes()
+    off = 0
+    for e in range(P.nCells):
+        ne = np1 ** 3
+        assert d["cellNdofs"][e] == ne
+        half = ch[e] / 2
+        mid = clo[e] + half
+        # x_d = mid_d + half_d xi_d: int x_d L L = mid_d M1 + half_d X1
+        F = [[M1 * half[k], (mid[k] * M1 + half[k] * X1) * half[k]] for k in range(3)]
+        Ve = np.zeros((ne, ne))
+        for term in range(4):
+            f = [F[k][1 if term == k + 1 else 0] for k in range(3)]
+            # local index a + np (b + np c): x fastest
+            Ve += v[term] * np.kron(f[2], np.kron(f[1], f[0]))
+        got = V[off:off + ne * ne].reshape(ne, ne, order="F")
+        assert np.abs(got - Ve).max() <= 1e-12 * np.abs(Ve).max()
+        off += ne * ne
+
+
+def test_coulomb_potential_block_integrates_the_smeared_potential():
+    """With the partition of unity sum_i N_i = 1, the entries of a classical
+    cell's potential block add up to the integral of V over the cell.  Over
+    8^3 cells at p = 4 around one Z = 3 centre, V = -Z erf(r / 0.5) / r (a
+    smooth function) summed so must match the integral that numpy forms on
+    a 20-point Gauss rule per cell direction, to the p + 1-point rule's
+    error: a missing Z, another smearing width or sign fails.  V is linear
+    in Z: the block for Z = 3 is 3 times the block for Z = 1."""
+    c, Z, sig = np.array([0.3, -0.2, 0.1]), 3, 0.5
+    P = O.Problem.build(p=4, n=(8, 8, 8), lo=(-4.,) * 3, hi=(4.,) * 3,
+                        atoms=[(c[0], c[1], c[2], Z)], v_smear=sig)
+    d = P.arrays()
+    got = float(np.sum(d["H"] - 0.5 * d["K"]))
+    from scipy.special import erf
+    xq, wq = np.polynomial.legendre.leggauss(20)
+    clo, ch = P.cell_boxes()
+    ref = 0.0
+    for e in range(P.nCells):
+        pts = [clo[e][k] + ch[e][k] * (xq + 1) / 2 for k in range(3)]
+        X, Y, Zc = np.meshgrid(*pts, indexing="ij")
+        r = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Zc - c[2]) ** 2)
+        w = np.einsum("i,j,k->ijk", wq, wq, wq) * np.prod(ch[e]) / 8
+        ref += float(np.sum(w * -Z * erf(r / sig) / r))
+    # the p + 1-point rule is within 1e-8 of it here; a smearing width 10 %
+    # off moves the integral by 1e-3
+    assert abs(got - ref) <= 1e-6 * abs(ref)
+    # the integral scales with Z exactly
+    P1 = O.Problem.build(p=4, n=(8, 8, 8), lo=(-4.,) * 3, hi=(4.,) * 3,
+                         atoms=[(c[0], c[1], c[2], 1)], v_smear=sig)
+    d1 = P1.arrays()
+    assert np.abs((d["H"] - 0.5 * d["K"]) - Z * (d1["H"] - 0.5 * d1["K"])).max() <= \
+        1e-13 * np.abs(d["H"]).max()
+
+
 def test_cell_matrix_invariants():
     P = c1_like(n=3, p=3)
     d = P.arrays()
