@@ -197,34 +197,45 @@ namespace dfx
 
     // one launch per (colour, size class); an operator of few cells (the
     // launches could not fill the GPU anyway, and every launch costs a
-    // dispatch) runs each colour as one launch with the largest class's tile
+    // dispatch) runs each colour as one launch with the largest class's tile.
+    // FP32 launches also merge the classes of a colour whenever the largest
+    // class fits cells of up to 192 dofs (measured on C4: one launch per
+    // colour instead of two side by side takes the FP32 apply from 1.39 to
+    // 1.30 ms; the FP64 apply, whose classes use different column-chunk
+    // widths, gets slower merged).  DFX_MERGE_CLASSES_BELOW sets the cell
+    // count below which both merge (and nothing else does)
     const char *mergeEnv   = getenv("DFX_MERGE_CLASSES_BELOW");
     const u32   mergeBelow = mergeEnv ? (u32)atol(mergeEnv) : 4096u;
-    const bool mergeClasses = nCells < mergeBelow;
-    op->launches64.clear();
-    op->launches32.clear();
-    for (u32 q = 0; q < nCells;)
-      {
-        const u32 e0 = op->hOrder[q];
-        u32       r  = q;
-        u32 kp64 = 0, kp32 = 0, mtw = 0;
-        u64 s2 = 0, s1 = 0, sc = 0;
-        while (r < nCells && color[op->hOrder[r]] == color[e0] &&
-               (mergeClasses || mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0])))
-          {
-            const u32 n = cellNdofs[op->hOrder[r]];
-            kp64        = std::max(kp64, 8 * ((n + 7) / 8));
-            kp32        = std::max(kp32, 16 * ((n + 15) / 16));
-            mtw         = std::max(mtw, mtwOf(n));
-            s2 += (u64)n * n;
-            s1 += n;
-            sc += nc[op->hOrder[r]];
-            ++r;
-          }
-        op->launches64.push_back({q, r, mtw, kp64, s2, s1, sc, 0, (u32)color[e0]});
-        op->launches32.push_back({q, r, mtw, kp32, s2, s1, sc, 0, (u32)color[e0]});
-        q = r;
-      }
+    u32         maxMtw     = 0;
+    for (u32 e = 0; e < nCells; ++e)
+      maxMtw = std::max(maxMtw, mtwOf(cellNdofs[e]));
+    const bool merge64 = nCells < mergeBelow;
+    const bool merge32 = mergeEnv ? merge64 : (merge64 || maxMtw <= 3);
+    auto buildLaunches = [&](std::vector<Launch> &out, bool merge, bool f64) {
+      out.clear();
+      for (u32 q = 0; q < nCells;)
+        {
+          const u32 e0 = op->hOrder[q];
+          u32       r  = q;
+          u32 kp = 0, mtw = 0;
+          u64 s2 = 0, s1 = 0, sc = 0;
+          while (r < nCells && color[op->hOrder[r]] == color[e0] &&
+                 (merge || mtwOf(cellNdofs[op->hOrder[r]]) == mtwOf(cellNdofs[e0])))
+            {
+              const u32 n = cellNdofs[op->hOrder[r]];
+              kp          = std::max(kp, f64 ? 8 * ((n + 7) / 8) : 16 * ((n + 15) / 16));
+              mtw         = std::max(mtw, mtwOf(n));
+              s2 += (u64)n * n;
+              s1 += n;
+              sc += nc[op->hOrder[r]];
+              ++r;
+            }
+          out.push_back({q, r, mtw, kp, s2, s1, sc, 0, (u32)color[e0]});
+          q = r;
+        }
+    };
+    buildLaunches(op->launches64, merge64, true);
+    buildLaunches(op->launches32, merge32, false);
 
     // first-touch bits in launch order: the first cell to reach a classical
     // dof stores its contribution, later cells add to it
