@@ -98,6 +98,16 @@ namespace dfx
           }
       }
 
+      // one element of the blocked axpby (shared with the residual filter's
+      // result and the fused CG update): the multiply-add written out, so
+      // that the rounding does not depend on which operands the compiler
+      // sees as constants
+      template <class T>
+      __device__ inline T axpbyOne(T a1, double a, T xv, T b1, double b, T yv)
+      {
+        return fma(b1 * (T)b, yv, a1 * (T)a * xv);
+      }
+
       template <class T>
       __global__ void axpbyBlockedK(T *z, u32 ldz, T a1, const double *a,
                                     const T *x, u32 ldx, T b1, const double *b,
@@ -109,7 +119,7 @@ namespace dfx
           {
             RC r  = rc(t, B);
             T  xv = x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
-            z[(u64)r.i * ldz + r.j] = a1 * (T)a[r.j] * xv + b1 * (T)b[r.j] * yv;
+            z[(u64)r.i * ldz + r.j] = axpbyOne<T>(a1, a[r.j], xv, b1, b[r.j], yv);
           }
       }
 
@@ -164,7 +174,7 @@ namespace dfx
             double    acc = 0.0;
             if (j < B)
               for (u32 i = r0 + ty; i < r1; i += R)
-                acc += (double)x[(u64)i * ldx + j] * (double)y[(u64)i * ldy + j];
+                acc = fma((double)x[(u64)i * ldx + j], (double)y[(u64)i * ldy + j], acc);
             red[threadIdx.x] = acc;
             __syncthreads();
             if (ty == 0 && j < B)
@@ -173,6 +183,63 @@ namespace dfx
                 for (int q = 0; q < R; ++q)
                   s += red[q * C + tx];
                 partials[(u64)blockIdx.x * B + j] = s;
+              }
+            __syncthreads();
+          }
+      }
+
+      // The PCG update of one iteration with a Jacobi preconditioner, fused:
+      //   x += alpha p, r -= alpha w, z = d o r, and the partial sums of z.r
+      //   and r.r (alpha = zr / pw per column, formed here as cgAlphaK does).
+      // Rows, lanes and sums follow colDotsK exactly (same blocks of rows,
+      // same lane order, same combine), and each element is the blocked
+      // axpby's / the diagonal scaling's expression, so every value is the
+      // separate launches' bit for bit.  partials: nBlocks x 2B (z.r in
+      // columns [0, B), r.r in [B, 2B)); N x B arrays with leading dimension B
+      template <int C>
+      __global__ void cgUpdateK(double *x, double *r, double *z, const double *p,
+                                const double *w, const double *d, const double *zr,
+                                const double *pw, u32 N, u32 B, u32 rowsPerBlock,
+                                double *partials)
+      {
+        constexpr int R  = TPB / C;
+        const int     tx = threadIdx.x % C, ty = threadIdx.x / C;
+        __shared__ double red[2][TPB];
+        const u32 r0 = blockIdx.x * rowsPerBlock;
+        const u32 r1 = min(N, r0 + rowsPerBlock);
+        for (u32 j0 = 0; j0 < B; j0 += C)
+          {
+            const u32 j     = j0 + tx;
+            double    accZ  = 0.0, accR = 0.0;
+            if (j < B)
+              {
+                const double alpha = zr[j] / pw[j], nalpha = -zr[j] / pw[j];
+                for (u32 i = r0 + ty; i < r1; i += R)
+                  {
+                    const u64    o  = (u64)i * B + j;
+                    const double xn = axpbyOne<double>(1.0, 1.0, x[o], 1.0, alpha, p[o]);
+                    const double rn = axpbyOne<double>(1.0, 1.0, r[o], 1.0, nalpha, w[o]);
+                    const double zn = d[i] * rn;
+                    x[o]            = xn;
+                    r[o]            = rn;
+                    z[o]            = zn;
+                    accZ = fma(zn, rn, accZ);
+                    accR = fma(rn, rn, accR);
+                  }
+              }
+            red[0][threadIdx.x] = accZ;
+            red[1][threadIdx.x] = accR;
+            __syncthreads();
+            if (ty == 0 && j < B)
+              {
+                double sz = 0.0, sr = 0.0;
+                for (int q = 0; q < R; ++q)
+                  {
+                    sz += red[0][q * C + tx];
+                    sr += red[1][q * C + tx];
+                  }
+                partials[(u64)blockIdx.x * 2 * B + j]     = sz;
+                partials[(u64)blockIdx.x * 2 * B + B + j] = sr;
               }
             __syncthreads();
           }
@@ -353,7 +420,7 @@ namespace dfx
           {
             RC           r  = rc(t, B);
             const double xv = (double)x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
-            z[(u64)r.i * ldz + r.j] = a1 * (double)a[r.j] * xv + b1 * (double)b[r.j] * yv;
+            z[(u64)r.i * ldz + r.j] = axpbyOne<double>(a1, a[r.j], xv, b1, b[r.j], yv);
           }
       }
 
@@ -543,6 +610,36 @@ namespace dfx
         colDotsK<T, 64><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
       checkLaunch("colDots");
       colDotsFinalK<<<B, 256, 0, s>>>(partials, nBlocks, B, out);
+      checkLaunch("colDotsFinal");
+    }
+
+    void cgUpdate(hipStream_t s, View<double> x, View<double> r, View<double> z,
+                  View<const double> p, View<const double> w, const double *d,
+                  const double *zr, const double *pw, double *partials, double *zrrr)
+    {
+      const u32 B = x.B, N = x.N;
+      if (!B)
+        return;
+      DFX_REQUIRE(x.ld == B && r.ld == B && z.ld == B && p.ld == B && w.ld == B,
+                  "CG update: contiguous multivectors expected");
+      // the partition of colDots
+      u32 nBlocks = (N + 255) / 256;
+      if (nBlocks > 1024)
+        nBlocks = 1024;
+      if (nBlocks == 0)
+        nBlocks = 1;
+      const u32 rows = (N + nBlocks - 1) / nBlocks;
+      if (B <= 16)
+        cgUpdateK<16><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
+                                              partials);
+      else if (B <= 32)
+        cgUpdateK<32><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
+                                              partials);
+      else
+        cgUpdateK<64><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
+                                              partials);
+      checkLaunch("cgUpdate");
+      colDotsFinalK<<<2 * B, 256, 0, s>>>(partials, nBlocks, 2 * B, zrrr);
       checkLaunch("colDotsFinal");
     }
 

@@ -37,6 +37,13 @@ namespace dfx
     void colDots(hipStream_t s, View<const T> x, View<const T> y, double *partials,
                  double *out);
 
+    // PCG iteration with a Jacobi (diagonal d) preconditioner, fused
+    // (CGLinearSolver.t.cpp:68-313): alpha = zr / pw per column; x += alpha
+    // p; r -= alpha w; z = d o r; zrrr[0, B) = z.r, zrrr[B, 2B) = r.r, the
+    // bits of the separate launches.  partials: 2048 B doubles
+    void cgUpdate(hipStream_t s, View<double> x, View<double> r, View<double> z,
+                  View<const double> p, View<const double> w, const double *d,
+                  const double *zr, const double *pw, double *partials, double *zrrr);
     // Plain Chebyshev degree step, fused (ChebyshevFilter.t.cpp:96-129):
     //   t  = Minv hx  (diag rows < nC: dinv o hx; enriched rows: given in tE)
     //   s1 = alpha t + beta xk;  xprev = s1 + gamma xprev

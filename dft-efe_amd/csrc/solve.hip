@@ -1067,8 +1067,23 @@ extern "C"
       double *coef = workDoubles(ctx, "cgs.coef", 4 * (size_t)B);
       // device dots: z.r (two, alternating), p.w, r.r; partial sums
       double *dots = workDoubles(ctx, "cgs.dots", 4 * (size_t)B);
-      double *part = workDoubles(ctx, "cgs.part", 1024 * (size_t)B);
+      double *part = workDoubles(ctx, "cgs.part", 2048 * (size_t)B);
       double *dZr = dots, *dZrNew = dots + B, *dPw = dots + 2 * B, *dRr = dots + 3 * B;
+      // a Jacobi preconditioner (diagonal, no constraints, no enriched block)
+      // lets one launch do the updates of x, r, z and the dots z.r, r.r
+      // (the same bits; DFX_FUSE_CG=0 keeps the separate launches); z.r and
+      // r.r of an iteration land side by side in one of two 2B slots
+      const char *fuseEnv = getenv("DFX_FUSE_CG");
+      const bool  fusedCG = (!fuseEnv || atoi(fuseEnv) != 0) && pc->kind == OP_DIAGONAL &&
+                           !pc->cons && pc->nEnriched == 0 && pc->nDofs == N;
+      double *slots = fusedCG ? workDoubles(ctx, "cgs.zrrr", 4 * (size_t)B) : nullptr;
+      if (fusedCG)
+        {
+          // the p update's unit coefficients (cgAlphaK's first B entries)
+          std::vector<double> ones(B, 1.0);
+          toDevice(ctx, coef, ones.data(), B);
+          dZr = slots;
+        }
       k::convert<double, double>(s, x, cv(xo));
       std::vector<double> bNorm(B), rNorm(B);
       colDotsView<double>(ctx, cv(b), cv(b), bNorm.data());
@@ -1093,6 +1108,20 @@ extern "C"
               // z.r of this iteration is the previous iteration's z.r_new
               applyOp<double>(A, pv, w);
               k::colDots<double>(s, cv(pv), cv(w), part, dPw);
+            }
+          if (iter > 0 && fusedCG)
+            {
+              double *next = dZr == slots ? slots + 2 * B : slots;
+              k::cgUpdate(s, x, r, z, cv(pv), cv(w), pc->diag.as<double>(), dZr, dPw, part,
+                          next);
+              cgBetaK<<<(B + 63) / 64, 64, 0, s>>>(next, dZr, B, coef + 3 * B);
+              checkLaunch("cgBeta");
+              k::axpbyBlocked<double>(s, pv, 1.0, coef, cv(z), 1.0, coef + 3 * B, cv(pv));
+              dZr = next;
+              dRr = next + B;
+            }
+          else if (iter > 0)
+            {
               cgAlphaK<<<(B + 63) / 64, 64, 0, s>>>(dZr, dPw, B, coef);
               checkLaunch("cgAlpha");
               k::axpbyBlocked<double>(s, x, 1.0, coef, cv(x), 1.0, coef + B, cv(pv));
@@ -1104,7 +1133,8 @@ extern "C"
               k::axpbyBlocked<double>(s, pv, 1.0, coef, cv(z), 1.0, coef + 3 * B, cv(pv));
               std::swap(dZr, dZrNew);
             }
-          k::colDots<double>(s, cv(r), cv(r), part, dRr);
+          if (!(fusedCG && iter > 0))
+            k::colDots<double>(s, cv(r), cv(r), part, dRr);
           DFX_HIP(hipMemcpyAsync(rNorm.data(), dRr, B * sizeof(double), hipMemcpyDeviceToHost, s));
           DFX_HIP(hipStreamSynchronize(s));
           for (auto &v : rNorm)

@@ -505,6 +505,37 @@ def test_cg_poisson(ctx, probs):
     assert rel(x.to_numpy(), xo) < 1e-8
 
 
+@pytest.mark.parametrize("B", [4, 20, 40])
+def test_cg_fused_update_is_bit_identical(ctx, probs, B, monkeypatch):
+    """With a Jacobi preconditioner one launch updates x, r and z and forms
+    the dots z.r and r.r of a PCG iteration, on the same rows, lanes and
+    combine as the separate launches (DFX_FUSE_CG=0): the iterations, the
+    residual norms and the solution are the same bits (16-, 32- and
+    64-column lane shapes)."""
+    P = probs["hang3"]
+    ops = Ops(ctx, P)
+    d = ops.d
+    diag = np.zeros(P.nDofs)
+    off = moff = 0
+    for n in d["cellNdofs"]:
+        ids = d["cellMap"][off:off + n]
+        diag[ids] += np.diag(d["K"][moff:moff + n * n].reshape(n, n, order="F"))
+        off += n; moff += n * n
+    diag = P.c2p(diag.reshape(-1, 1))[:, 0]
+    diag[d["rows"]] = 1.0
+    b = P.c2p(guess(P, B, seed=16))
+    pc = dfx.DiagonalOperator(ctx, 1.0 / diag)
+    out = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("DFX_FUSE_CG", fuse)
+        x = dfx.MultiVector(ctx, P.nDofs, B)
+        cg = dfx.CGLinearSolver(ctx, 2000, 1e-12, 1e-10, 1e10)
+        assert cg.solve(ops.K, pc, mv(ctx, b), x) == 0
+        out.append((cg.iterations, np.array(cg.residualNorms), x.to_numpy()))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
 # ----------------------------------------------------- full KS solve
 KS_NE = 8  # electrons: 1s, 2p x3 (and the nearby 2s) are occupied at 500 K
 
