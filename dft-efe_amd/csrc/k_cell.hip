@@ -61,6 +61,12 @@
 #ifndef DFX_CELL_OPAQUE
 #define DFX_CELL_OPAQUE 1
 #endif
+// whether the fused kernels keep the gathered rows of X unscaled in LDS
+// (M^-1 applied to the MFMA operand instead) and read x for the residual
+// step from there instead of from global memory
+#ifndef DFX_RES_LDSX
+#define DFX_RES_LDSX 1
+#endif
 // whether the fused kernels request Y's old values per row group too
 #ifndef DFX_RES_OLDG
 #define DFX_RES_OLDG 1
@@ -228,6 +234,8 @@ namespace dfx
                                                   (S == 4 ? DFX_CELL_AREG_MAX : DFX_CELL_AREG_MAX / 2)) ||
                               (NT > 1 && MTW * NKBMAX * 4 <= DFX_CELL_AREG_WIDE);
         constexpr int  NRING    = AREG ? NKBMAX : D;
+        // RES with the gathered rows kept unscaled in LDS
+        constexpr bool LDSX     = DSC && RES && DFX_RES_LDSX;
         extern __shared__ __align__(16) unsigned char smemRaw[];
         T *sX = reinterpret_cast<T *>(smemRaw);
         __shared__ u32 sMap[256];
@@ -384,7 +392,7 @@ namespace dfx
                 const bool live = t < UNITS;
                 const int  q    = t / VCL;
                 (void)live;
-                if constexpr (DSC)
+                if constexpr (DSC && !LDSX)
                   {
                     // M^-1 folded in: rows q G .. q G + G - 1 of the unit
                     // scaled by their diag (1 past nc, exact) before the MFMA
@@ -466,6 +474,16 @@ namespace dfx
                       for (int nt = 0; nt < NTL; ++nt)
                         b[nt] = *reinterpret_cast<const vec_t *>(
                           sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
+                      if constexpr (LDSX)
+                        {
+                          // M^-1 of the operand's G rows (the product of the
+                          // gather's scaling, bit for bit)
+                          const vec_t dv =
+                            *reinterpret_cast<const vec_t *>(sDsc + (kb * 4 + (l >> 4)) * G);
+#pragma unroll
+                          for (int nt = 0; nt < NTL; ++nt)
+                            b[nt] = dv * b[nt];
+                        }
 #pragma unroll
                       for (int s = 0; s < G; ++s)
 #pragma unroll
@@ -485,6 +503,14 @@ namespace dfx
                 for (int nt = 0; nt < NTL; ++nt)
                   b[nt] = *reinterpret_cast<const vec_t *>(
                     sX + (((kb * 4 + (l >> 4)) * BC) + swz(nt * 16 + cl)) * G);
+                if constexpr (LDSX)
+                  {
+                    const vec_t dv =
+                      *reinterpret_cast<const vec_t *>(sDsc + (kb * 4 + (l >> 4)) * G);
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt)
+                      b[nt] = dv * b[nt];
+                  }
                 vec_t a[MTW];
 #pragma unroll
                 for (int t = 0; t < MTW; ++t)
@@ -566,7 +592,10 @@ namespace dfx
                                 const u32 off = (fin[g] & (nt * 16 + cl < bw))
                                                   ? xb + nt * 16 * S
                                                   : OFF_NONE;
-                                fx[nt][g] = BF::load(rX, off);
+                                if constexpr (LDSX) // row < KMAX: inside the tile
+                                  fx[nt][g] = sX[((row / G) * BC + swz(nt * 16 + cl)) * G + row % G];
+                                else
+                                  fx[nt][g] = BF::load(rX, off);
                                 fr[nt][g] = BF::load(rRes, off);
                                 f0[nt][g] = BF::load(rR0, off);
                               }

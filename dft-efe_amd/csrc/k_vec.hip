@@ -245,6 +245,47 @@ namespace dfx
           }
       }
 
+      // partial sums of |a1 a_j x_ij + b1 b_j y_ij|^2 per column without
+      // writing the combination: colDotsK's rows, lanes and combine and the
+      // blocked axpby's element, so the norms are those of the separate
+      // launches bit for bit (N x B arrays with leading dimension B)
+      template <int C>
+      __global__ void combNormsK(double a1, const double *a, const double *x, double b1,
+                                 const double *b, const double *y, u32 N, u32 B,
+                                 u32 rowsPerBlock, double *partials)
+      {
+        constexpr int R  = TPB / C;
+        const int     tx = threadIdx.x % C, ty = threadIdx.x / C;
+        __shared__ double red[TPB];
+        const u32 r0 = blockIdx.x * rowsPerBlock;
+        const u32 r1 = min(N, r0 + rowsPerBlock);
+        for (u32 j0 = 0; j0 < B; j0 += C)
+          {
+            const u32 j   = j0 + tx;
+            double    acc = 0.0;
+            if (j < B)
+              {
+                const double aj = a[j], bj = b[j];
+                for (u32 i = r0 + ty; i < r1; i += R)
+                  {
+                    const u64    o = (u64)i * B + j;
+                    const double v = axpbyOne<double>(a1, aj, x[o], b1, bj, y[o]);
+                    acc            = fma(v, v, acc);
+                  }
+              }
+            red[threadIdx.x] = acc;
+            __syncthreads();
+            if (ty == 0 && j < B)
+              {
+                double s = 0.0;
+                for (int q = 0; q < R; ++q)
+                  s += red[q * C + tx];
+                partials[(u64)blockIdx.x * B + j] = s;
+              }
+            __syncthreads();
+          }
+      }
+
       // one workgroup per column: thread t adds partials t, t + 256, ... in
       // order, then a fixed-shape tree over the 256 sums (deterministic; the
       // partial count depends on N only, so a column's result does not
@@ -609,6 +650,32 @@ namespace dfx
       else
         colDotsK<T, 64><<<nBlocks, TPB, 0, s>>>(x.p, x.ld, y.p, y.ld, x.N, B, rows, partials);
       checkLaunch("colDots");
+      colDotsFinalK<<<B, 256, 0, s>>>(partials, nBlocks, B, out);
+      checkLaunch("colDotsFinal");
+    }
+
+    void combinationNorms(hipStream_t s, double a1, const double *a, View<const double> x,
+                          double b1, const double *b, View<const double> y, double *partials,
+                          double *out)
+    {
+      const u32 B = x.B, N = x.N;
+      if (!B)
+        return;
+      DFX_REQUIRE(x.ld == B && y.ld == B && y.N == N && y.B == B,
+                  "combination norms: contiguous multivectors of one shape expected");
+      u32 nBlocks = (N + 255) / 256;
+      if (nBlocks > 1024)
+        nBlocks = 1024;
+      if (nBlocks == 0)
+        nBlocks = 1;
+      const u32 rows = (N + nBlocks - 1) / nBlocks;
+      if (B <= 16)
+        combNormsK<16><<<nBlocks, TPB, 0, s>>>(a1, a, x.p, b1, b, y.p, N, B, rows, partials);
+      else if (B <= 32)
+        combNormsK<32><<<nBlocks, TPB, 0, s>>>(a1, a, x.p, b1, b, y.p, N, B, rows, partials);
+      else
+        combNormsK<64><<<nBlocks, TPB, 0, s>>>(a1, a, x.p, b1, b, y.p, N, B, rows, partials);
+      checkLaunch("combNorms");
       colDotsFinalK<<<B, 256, 0, s>>>(partials, nBlocks, B, out);
       checkLaunch("colDotsFinal");
     }

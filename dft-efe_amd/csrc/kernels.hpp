@@ -37,6 +37,12 @@ namespace dfx
     void colDots(hipStream_t s, View<const T> x, View<const T> y, double *partials,
                  double *out);
 
+    // out[j] = sum_i (a1 a[j] x_ij + b1 b[j] y_ij)^2 without forming the
+    // combination: the bits of the blocked axpby followed by colDots
+    // (partials: 1024 B doubles; out on the device)
+    void combinationNorms(hipStream_t s, double a1, const double *a, View<const double> x,
+                          double b1, const double *b, View<const double> y, double *partials,
+                          double *out);
     // PCG iteration with a Jacobi (diagonal d) preconditioner, fused
     // (CGLinearSolver.t.cpp:68-313): alpha = zr / pw per column; x += alpha
     // p; r -= alpha w; z = d o r; zrrr[0, B) = z.r, zrrr[B, 2B) = r.r, the

@@ -476,12 +476,13 @@ namespace dfx
         throw Error(DFX_E_HIP, "rocblas_dgemm failed");
     }
 
-    // res_j = || A x_j - lambda_j B x_j || from the kept products
-    void productResidualNorms(dfx_chfsi *cs, const double *eig, double *res)
+    // res_j = || -A x_j + lambda_j B x_j || (host) from AX and BX (N x n,
+    // contiguous), without forming the residual vectors: the bits of the
+    // blocked axpby followed by the column norms
+    void residualNormsOf(dfx_ctx *ctx, View<const double> AX, View<const double> BX,
+                         const double *eig, double *res)
     {
-      dfx_ctx    *ctx = cs->ctx;
-      const u32   N = cs->kpN, n = cs->kpn;
-      View<double> r = workView<double>(ctx, "rn.x", N, n);
+      const u32 n = AX.B;
       std::vector<double> h(2 * (size_t)n);
       for (u32 j = 0; j < n; ++j)
         {
@@ -490,12 +491,22 @@ namespace dfx
         }
       double *d = workDoubles(ctx, "rn.coef", 2 * (size_t)n);
       toDevice(ctx, d, h.data(), h.size());
-      k::axpbyBlocked<double>(ctx->stream, r, 1.0, d,
-                              View<const double>{cs->kpHX.as<double>(), N, n, n}, 1.0, d + n,
-                              View<const double>{cs->kpMX.as<double>(), N, n, n});
-      colDotsView<double>(ctx, cv(r), cv(r), res);
+      ctx->reducePartials.ensure((size_t)1024 * n * sizeof(double) + n * sizeof(double));
+      double *part = ctx->reducePartials.as<double>();
+      double *out  = part + (size_t)1024 * n;
+      k::combinationNorms(ctx->stream, 1.0, d, AX, 1.0, d + n, BX, part, out);
+      DFX_HIP(hipMemcpyAsync(res, out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      DFX_HIP(hipStreamSynchronize(ctx->stream));
       for (u32 j = 0; j < n; ++j)
         res[j] = std::sqrt(res[j]);
+    }
+
+    // res_j = || A x_j - lambda_j B x_j || from the kept products
+    void productResidualNorms(dfx_chfsi *cs, const double *eig, double *res)
+    {
+      const u32 N = cs->kpN, n = cs->kpn;
+      residualNormsOf(cs->ctx, View<const double>{cs->kpHX.as<double>(), N, n, n},
+                      View<const double>{cs->kpMX.as<double>(), N, n, n}, eig, res);
     }
 
     // ChebyshevFilteredEigenSolver::solve (ChebyshevFilteredEigenSolver.t.cpp:194-437).
@@ -740,18 +751,7 @@ namespace dfx
       k::convert<double, double>(s, x, cv(X));
       applyOp<double>(H, x, hx);
       applyOp<double>(M, x, mx);
-      std::vector<double> h(2 * (size_t)n);
-      for (u32 j = 0; j < n; ++j)
-        {
-          h[j]     = -1.0;
-          h[n + j] = eig[j];
-        }
-      double *d = workDoubles(ctx, "rn.coef", 2 * (size_t)n);
-      toDevice(ctx, d, h.data(), h.size());
-      k::axpbyBlocked<double>(s, x, 1.0, d, cv(hx), 1.0, d + n, cv(mx));
-      colDotsView<double>(ctx, cv(x), cv(x), res);
-      for (u32 j = 0; j < n; ++j)
-        res[j] = std::sqrt(res[j]);
+      residualNormsOf(ctx, cv(hx), cv(mx), eig, res);
     }
   } // namespace
 } // namespace dfx
