@@ -229,12 +229,14 @@ namespace
   }
 
   // One block per cell; threads over Gauss points.  Writes, column-major in
-  // (point, function):  F, FW = W F, FV = W V F  (nQ x n) and G, GW (3 nQ x n).
+  // (point, function):  F, FW = W F, FV = W V F  (nQ x n) and G, GW (3 nQ x n);
+  // valuesOnly: F alone (no potential, weights or gradients are formed; the
+  // other pointers are not touched)
   __global__ void tabulateK(const TabCell *cells, const u32 *map, u32 nClassical,
                             const double *tabL, const double *tabD, const double *tabX,
                             const double *tabW, int np, const PotDev *atoms, u32 nAtoms,
                             double smear, const EnrFun *funs, const double *spl,
-                            const double *alpha,
+                            const double *alpha, bool valuesOnly,
                             double *F, double *FW, double *FV, double *G, double *GW)
   {
     const TabCell C  = cells[blockIdx.x];
@@ -252,7 +254,7 @@ namespace
                              C.lo[2] + C.h[2] * (X[qc] + 1.0) * 0.5};
         const double w = ((Wt[qa] * Wt[qb]) * Wt[qc]) * detJ;
         double v = 0.0;
-        for (u32 a = 0; a < nAtoms; ++a)
+        for (u32 a = 0; a < (valuesOnly ? 0u : nAtoms); ++a)
           {
             const double dx = x[0] - atoms[a].r[0], dy = x[1] - atoms[a].r[1],
                          dz = x[2] - atoms[a].r[2];
@@ -278,6 +280,8 @@ namespace
         auto put = [&](u32 j, double f, double gx, double gy, double gz) {
           const u64 o = C.bufOff;
           F[o + q + (u64)nQ * j]  = f;
+          if (valuesOnly)
+            return;
           FW[o + q + (u64)nQ * j] = w * f;
           FV[o + q + (u64)nQ * j] = (w * v) * f;
           const u64 og = 3 * C.bufOff + (u64)3 * nQ * j;
@@ -294,9 +298,12 @@ namespace
             for (int a = 0; a < np; ++a)
               {
                 const double La = L[a * nq + qa], Lb = L[b * nq + qb], Lc = L[c * nq + qc];
-                put(a + np * (b + np * c), (La * Lb) * Lc,
-                    (D[a * nq + qa] * Lb) * Lc * sx, (La * D[b * nq + qb]) * Lc * sy,
-                    (La * Lb) * D[c * nq + qc] * sz);
+                if (valuesOnly)
+                  put(a + np * (b + np * c), (La * Lb) * Lc, 0.0, 0.0, 0.0);
+                else
+                  put(a + np * (b + np * c), (La * Lb) * Lc,
+                      (D[a * nq + qa] * Lb) * Lc * sx, (La * D[b * nq + qb]) * Lc * sy,
+                      (La * Lb) * D[c * nq + qc] * sz);
               }
         for (u32 j = ncl; j < n; ++j)
           {
@@ -316,6 +323,8 @@ namespace
                         const double La = L[a * nq + qa], Lb = L[b * nq + qb],
                                      Lc = L[c * nq + qc];
                         f -= s * ((La * Lb) * Lc);
+                        if (valuesOnly)
+                          continue;
                         gr[0] -= s * ((D[a * nq + qa] * Lb) * Lc * sx);
                         gr[1] -= s * ((La * D[b * nq + qb]) * Lc * sy);
                         gr[2] -= s * ((La * Lb) * D[c * nq + qc] * sz);
@@ -664,9 +673,11 @@ namespace
   // batch(cells, cnt, n, nQ, F, FW, FV, G, GW) for each batch (cells: the
   // batch's cells, in the batch's order).  extraPerCell: bytes the caller
   // allocates per cell of a batch.
+  // valuesOnly: F alone is tabulated (the other buffers passed to batch are F
+  // as well and must not be read)
   template <class Batch>
   void tabulateBatches(dfx_problem *P, const std::vector<u32> &cellList, const double *alpha,
-                       size_t extraPerCell, Batch batch)
+                       size_t extraPerCell, Batch batch, bool valuesOnly = false)
   {
     if (cellList.empty())
       return;
@@ -719,7 +730,7 @@ namespace
         // batch: cells of one shape, within the memory budget
         const u32 n = P->cellNdofs[order[i0]], nq = nqOf(order[i0]);
         const u64 nQ = (u64)nq * nq * nq;
-        const size_t per = (size_t)9 * nQ * n * sizeof(double) + extraPerCell;
+        const size_t per = (size_t)(valuesOnly ? 1 : 9) * nQ * n * sizeof(double) + extraPerCell;
         size_t i1 = i0;
         // (at most 65535 cells: one grid row per cell in the batch kernels)
         while (i1 < order.size() && P->cellNdofs[order[i1]] == n && nqOf(order[i1]) == nq &&
@@ -748,15 +759,15 @@ namespace
         DevBuf dTc;
         upload(dTc, tc, s);
         DevBuf &bF  = work(ctx, "setup.F", (size_t)cnt * nQ * n * 8);
-        DevBuf &bFW = work(ctx, "setup.FW", (size_t)cnt * nQ * n * 8);
-        DevBuf &bFV = work(ctx, "setup.FV", (size_t)cnt * nQ * n * 8);
-        DevBuf &bG  = work(ctx, "setup.G", (size_t)cnt * 3 * nQ * n * 8);
-        DevBuf &bGW = work(ctx, "setup.GW", (size_t)cnt * 3 * nQ * n * 8);
+        DevBuf &bFW = valuesOnly ? bF : work(ctx, "setup.FW", (size_t)cnt * nQ * n * 8);
+        DevBuf &bFV = valuesOnly ? bF : work(ctx, "setup.FV", (size_t)cnt * nQ * n * 8);
+        DevBuf &bG  = valuesOnly ? bF : work(ctx, "setup.G", (size_t)cnt * 3 * nQ * n * 8);
+        DevBuf &bGW = valuesOnly ? bF : work(ctx, "setup.GW", (size_t)cnt * 3 * nQ * n * 8);
         tabulateK<<<cnt, 256, 0, s>>>(dTc.as<TabCell>(), dMap.as<u32>(), P->nClassical,
                                       dL.as<double>(), dD.as<double>(), dX.as<double>(),
                                       dW.as<double>(), (int)np, dAtoms.as<PotDev>(),
                                       S.nAtoms, S.vSmear, dFuns.as<EnrFun>(),
-                                      dSpl.as<double>(), alpha,
+                                      dSpl.as<double>(), alpha, valuesOnly,
                                       bF.as<double>(), bFW.as<double>(), bFV.as<double>(),
                                       bG.as<double>(), bGW.as<double>());
         checkLaunch("tabulate");
@@ -944,7 +955,8 @@ namespace
                       checkLaunch("rhoFromPsi");
                       // the batch's pointer arrays are released when it returns
                       DFX_HIP(hipStreamSynchronize(s));
-                    });
+                    },
+                    true);
   }
 
   // fw[(t nQ + q) B + j] = JxW[q] f[(quadOff[c] + q) ldf + j] for the cells
@@ -1049,7 +1061,8 @@ namespace dfx
                       }
                       // the batch's pointer arrays are released when it returns
                       DFX_HIP(hipStreamSynchronize(s));
-                    });
+                    },
+                    true);
     k::fill<double>(s, out, 0.0);
     k::scatterAdd<double>(s, dSOff.as<u32>(), dSList.as<u32>(), P->nDofs,
                           View<const double>{bSlots.as<double>(), (u32)nSlots, B, B}, out);
