@@ -24,7 +24,7 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-result",
 ]
-LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lamdhip64",
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lhipblaslt", "-lamdhip64",
         "-Wl,-rpath,/opt/rocm/lib"]
 
 

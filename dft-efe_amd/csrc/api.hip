@@ -323,6 +323,9 @@ namespace dfx
     if (ctx->stream)
       (void)hipStreamSynchronize(ctx->stream);
     CaptureLock lk(captureMutex());
+    if (ctx->lt && ctx->ltFree)
+      ctx->ltFree(ctx->lt);
+    ctx->lt = nullptr;
     if (ctx->blas)
       rocblas_destroy_handle(ctx->blas);
     for (auto &g : ctx->graphs)

@@ -172,6 +172,10 @@ struct dfx_ctx
   hipEvent_t     forkEv = nullptr, joinEv = nullptr;
   bool           forkClasses = true; // false: every launch on `stream` (serving slots)
   rocblas_handle blas   = nullptr;
+  // hipBLASLt state of the FP64 subspace rotations (handle, workspace, one
+  // plan per shape), owned here and released by ltFree
+  void          *lt     = nullptr;
+  void         (*ltFree)(void *) = nullptr;
   std::string    err;
   dfx::DevBuf    scratchSmall; // host<->device small transfers (reductions)
   dfx::DevBuf    reducePartials;

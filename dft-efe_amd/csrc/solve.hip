@@ -2,6 +2,7 @@
 // GPU.  All vector work is on the device; only per-column scalars (dots,
 // eigenvalues, recurrence coefficients) cross to the host, as in the
 // reference's host-side control flow.
+#include <hipblaslt/hipblaslt.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -343,6 +344,121 @@ namespace dfx
       k::resFinal<T>(s, Y, 1.0, dOnes, cv(rs), 1.0, dtab + (size_t)(m + 1) * nb, cv(Xin));
     }
 
+    // ------------------------------------------------ subspace rotations
+    // Xo = X Q (N x n row-major, Q n x n column-major FP64) as one hipBLASLt
+    // matmul in column-major terms, Xo^T = Q^T X^T, with the first algorithm
+    // of the library's heuristic for the shape (a fixed choice, so results do
+    // not depend on timing); C4 (N = 2.1 M, n = 200): 3.2 ms against 3.7 for
+    // the rocBLAS DGEMM, which remains the fallback when hipBLASLt offers no
+    // algorithm
+    struct LtState
+    {
+      hipblasLtHandle_t h = nullptr;
+      DevBuf            ws;
+      struct Plan
+      {
+        u32                     n, N, ldx, ldo;
+        bool                    ok = false;
+        hipblasLtMatmulDesc_t   desc = nullptr;
+        hipblasLtMatrixLayout_t a = nullptr, b = nullptr, c = nullptr;
+        hipblasLtMatmulAlgo_t   algo{};
+      };
+      std::vector<Plan> plans;
+      ~LtState()
+      {
+        for (Plan &p : plans)
+          {
+            if (p.a)
+              (void)hipblasLtMatrixLayoutDestroy(p.a);
+            if (p.b)
+              (void)hipblasLtMatrixLayoutDestroy(p.b);
+            if (p.c)
+              (void)hipblasLtMatrixLayoutDestroy(p.c);
+            if (p.desc)
+              (void)hipblasLtMatmulDescDestroy(p.desc);
+          }
+        if (h)
+          (void)hipblasLtDestroy(h);
+      }
+    };
+    constexpr size_t LT_WORKSPACE = (size_t)64 << 20;
+
+    void ltFree(void *p)
+    {
+      delete static_cast<LtState *>(p);
+    }
+
+    void rotate64(dfx_ctx *ctx, View<const double> X, const double *Q, View<double> Xo,
+                  double *qr)
+    {
+      const u32 n = X.B, N = X.N;
+      DFX_REQUIRE(Xo.B == n && Xo.N == N, "rotate: shape mismatch");
+      CaptureLock lk(captureMutex());
+      if (!ctx->lt)
+        {
+          auto st = new LtState();
+          if (hipblasLtCreate(&st->h) != HIPBLAS_STATUS_SUCCESS)
+            st->h = nullptr;
+          ctx->lt     = st;
+          ctx->ltFree = ltFree;
+        }
+      LtState *st = static_cast<LtState *>(ctx->lt);
+      LtState::Plan *plan = nullptr;
+      for (auto &p : st->plans)
+        if (p.n == n && p.N == N && p.ldx == X.ld && p.ldo == Xo.ld)
+          plan = &p;
+      if (!plan && st->h)
+        {
+          st->plans.push_back({n, N, X.ld, Xo.ld});
+          plan = &st->plans.back();
+          bool ok = hipblasLtMatmulDescCreate(&plan->desc, HIPBLAS_COMPUTE_64F, HIP_R_64F) ==
+                    HIPBLAS_STATUS_SUCCESS;
+          const hipblasOperation_t tA = HIPBLAS_OP_T, tB = HIPBLAS_OP_N;
+          ok = ok && hipblasLtMatmulDescSetAttribute(plan->desc, HIPBLASLT_MATMUL_DESC_TRANSA,
+                                                     &tA, sizeof(tA)) == HIPBLAS_STATUS_SUCCESS;
+          ok = ok && hipblasLtMatmulDescSetAttribute(plan->desc, HIPBLASLT_MATMUL_DESC_TRANSB,
+                                                     &tB, sizeof(tB)) == HIPBLAS_STATUS_SUCCESS;
+          ok = ok && hipblasLtMatrixLayoutCreate(&plan->a, HIP_R_64F, n, n, n) ==
+                       HIPBLAS_STATUS_SUCCESS;
+          ok = ok && hipblasLtMatrixLayoutCreate(&plan->b, HIP_R_64F, n, N, X.ld) ==
+                       HIPBLAS_STATUS_SUCCESS;
+          ok = ok && hipblasLtMatrixLayoutCreate(&plan->c, HIP_R_64F, n, N, Xo.ld) ==
+                       HIPBLAS_STATUS_SUCCESS;
+          if (ok)
+            {
+              hipblasLtMatmulPreference_t pref = nullptr;
+              size_t                      wsb  = LT_WORKSPACE;
+              int                         got  = 0;
+              hipblasLtMatmulHeuristicResult_t res{};
+              ok = hipblasLtMatmulPreferenceCreate(&pref) == HIPBLAS_STATUS_SUCCESS &&
+                   hipblasLtMatmulPreferenceSetAttribute(
+                     pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsb, sizeof(wsb)) ==
+                     HIPBLAS_STATUS_SUCCESS &&
+                   hipblasLtMatmulAlgoGetHeuristic(st->h, plan->desc, plan->a, plan->b,
+                                                   plan->c, plan->c, pref, 1, &res, &got) ==
+                     HIPBLAS_STATUS_SUCCESS &&
+                   got > 0;
+              if (pref)
+                (void)hipblasLtMatmulPreferenceDestroy(pref);
+              if (ok)
+                plan->algo = res.algo;
+            }
+          plan->ok = ok;
+          if (ok && st->ws.bytes < LT_WORKSPACE)
+            st->ws.alloc(LT_WORKSPACE);
+        }
+      if (plan && plan->ok)
+        {
+          const double one = 1.0, zero = 0.0;
+          if (hipblasLtMatmul(st->h, plan->desc, &one, Q, plan->a, X.p, plan->b, &zero, Xo.p,
+                              plan->c, Xo.p, plan->c, &plan->algo, st->ws.p, LT_WORKSPACE,
+                              ctx->stream) == HIPBLAS_STATUS_SUCCESS)
+            return;
+          throw Error(DFX_E_HIP, "hipblasLtMatmul (rotation) failed");
+        }
+      k::rotate<double>(ctx->stream, X, Q, Xo, qr, ctx->blas);
+    }
+
     // --------------------------------------------------- CholGS and RR
     int cholGS(dfx_ctx *ctx, View<double> X, View<double> Xo, dfx_op *Bop)
     {
@@ -400,7 +516,7 @@ namespace dfx
           hQ[j + (size_t)n * i] = hS[i + (size_t)n * j];
       toDevice(ctx, Q, hQ.data(), hQ.size());
       double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
-      k::rotate<double>(s, cv(X), Q, Xo, qr, ctx->blas);
+      rotate64(ctx, cv(X), Q, Xo, qr);
       return 0;
     }
 
@@ -433,7 +549,7 @@ namespace dfx
       if (hinfo != 0)
         return DFX_EIG_LAPACK_ERROR;
       double *qr = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
-      k::rotate<double>(s, cv(X), S, Xo, qr, ctx->blas);
+      rotate64(ctx, cv(X), S, Xo, qr);
       return 0;
     }
 
@@ -606,10 +722,10 @@ namespace dfx
           gemmSquare(ctx, Qc, S, W, n);
           cs->kpHX.ensure(by);
           cs->kpMX.ensure(by);
-          k::rotate<double>(s, cv(workView<double>(ctx, "rr.hx", N, n)), S,
-                            View<double>{cs->kpHX.as<double>(), N, n, n}, qr, ctx->blas);
-          k::rotate<double>(s, cv(workView<double>(ctx, "cg.mx", N, n)), W,
-                            View<double>{cs->kpMX.as<double>(), N, n, n}, qr, ctx->blas);
+          rotate64(ctx, cv(workView<double>(ctx, "rr.hx", N, n)), S,
+                   View<double>{cs->kpHX.as<double>(), N, n, n}, qr);
+          rotate64(ctx, cv(workView<double>(ctx, "cg.mx", N, n)), W,
+                   View<double>{cs->kpMX.as<double>(), N, n, n}, qr);
           cs->kpHash  = hashOf(ctx, cv(XV));
           cs->kpA     = A->uid;
           cs->kpAGen  = A->gen;
