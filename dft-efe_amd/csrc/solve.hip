@@ -393,6 +393,13 @@ namespace dfx
     {
       const u32 n = X.B, N = X.N;
       DFX_REQUIRE(Xo.B == n && Xo.N == N, "rotate: shape mismatch");
+      // DFX_ROTATE_LT=0: the rocBLAS DGEMM (read on every call)
+      const char *env = getenv("DFX_ROTATE_LT");
+      if (env && atoi(env) == 0)
+        {
+          k::rotate<double>(ctx->stream, X, Q, Xo, qr, ctx->blas);
+          return;
+        }
       CaptureLock lk(captureMutex());
       if (!ctx->lt)
         {

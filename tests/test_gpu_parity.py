@@ -445,7 +445,11 @@ def test_chebyshev_filter_blocks_equal_fused(ctx, probs, prec, residual):
         assert np.array_equal(res[0][0], e) and np.array_equal(res[0][1], v)
 
 
-def test_cholgs_and_rr(ctx, probs):
+@pytest.mark.parametrize("lt", ["1", "0"])
+def test_cholgs_and_rr(ctx, probs, lt, monkeypatch):
+    """CholGS and Rayleigh-Ritz against the oracle with the rotations on
+    hipBLASLt (default) and on the rocBLAS DGEMM (DFX_ROTATE_LT=0)."""
+    monkeypatch.setenv("DFX_ROTATE_LT", lt)
     P = probs["hang3"]
     ops = Ops(ctx, P)
     X = guess(P, 12, seed=8)
