@@ -430,6 +430,14 @@ namespace dfx
       // axpby), res = 0, in T; the same arithmetic as the separate
       // convert / axpby / convert / fill launches
       template <class T>
+      __device__ inline void resSeedOne(double y, double a, double b, T &r0, T &rn, T &res)
+      {
+        r0  = (T)y;
+        rn  = (T)fma(a, y, b * y);
+        res = (T)0.0;
+      }
+
+      template <class T>
       __global__ void resSeedK(T *r0, T *rn, T *res, u32 ld, const double *Y, u32 ldy,
                                double a, double b, u32 N, u32 B)
       {
@@ -437,13 +445,28 @@ namespace dfx
         for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
              t += (u64)gridDim.x * blockDim.x)
           {
+            RC        r = rc(t, B);
+            const u64 o = (u64)r.i * ld + r.j;
+            resSeedOne<T>(Y[(u64)r.i * ldy + r.j], a, b, r0[o], rn[o], res[o]);
+          }
+      }
+
+      // the seed from kept products: Y = AX - lambda o BX (the blocked
+      // axpby's element) is formed in registers and converted as resSeedK
+      // does, without being written
+      template <class T>
+      __global__ void resSeedFromK(T *r0, T *rn, T *res, u32 ld, const double *AX,
+                                   const double *BX, u32 ldp, const double *ones,
+                                   const double *lam, double a, double b, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
             RC           r = rc(t, B);
-            const double y = Y[(u64)r.i * ldy + r.j];
-            const u64    o = (u64)r.i * ld + r.j;
-            const double v = a * y + b * y;
-            r0[o]          = (T)y;
-            rn[o]          = (T)v;
-            res[o]         = (T)0.0;
+            const u64    p = (u64)r.i * ldp + r.j, o = (u64)r.i * ld + r.j;
+            const double y = axpbyOne<double>(1.0, ones[r.j], AX[p], -1.0, lam[r.j], BX[p]);
+            resSeedOne<T>(y, a, b, r0[o], rn[o], res[o]);
           }
       }
 
@@ -796,6 +819,21 @@ namespace dfx
     }
 
     template <class T>
+    void resSeedFrom(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> AX,
+                     View<const double> BX, const double *ones, const double *lam,
+                     double alpha1)
+    {
+      const u64 n = (u64)AX.N * AX.B;
+      if (!n)
+        return;
+      DFX_REQUIRE(r0.ld == rn.ld && rn.ld == res.ld && AX.ld == BX.ld,
+                  "residual seed: one leading dimension");
+      resSeedFromK<T><<<gridFor(n), TPB, 0, s>>>(r0.p, rn.p, res.p, r0.ld, AX.p, BX.p, AX.ld,
+                                                 ones, lam, alpha1, 0.0, AX.N, AX.B);
+      checkLaunch("resSeedFrom");
+    }
+
+    template <class T>
     void resFinal(hipStream_t s, View<double> z, double a1, const double *a, View<const T> x,
                   double b1, const double *b, View<const double> y)
     {
@@ -895,6 +933,9 @@ namespace dfx
   template void resFix<T>(hipStream_t, View<T>, View<const T>, View<const T>,      \
                           View<const T>, const u32 *, u32, const double *,         \
                           const double *);                                         \
+  template void resSeedFrom<T>(hipStream_t, View<T>, View<T>, View<T>,             \
+                               View<const double>, View<const double>,             \
+                               const double *, const double *, double);            \
   template void resSeed<T>(hipStream_t, View<T>, View<T>, View<T>,                 \
                            View<const double>, double);                            \
   template void resFinal<T>(hipStream_t, View<double>, double, const double *,     \

@@ -83,6 +83,13 @@ namespace dfx
     template <class T>
     void resSeed(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> Y,
                  double alpha1);
+    // resSeed from AX and BX (kept products): Y = AX - lambda o BX formed in
+    // registers (the blocked axpby's element), the same bits as the blocked
+    // axpby into Y followed by resSeed
+    template <class T>
+    void resSeedFrom(hipStream_t s, View<T> r0, View<T> rn, View<T> res, View<const double> AX,
+                     View<const double> BX, const double *ones, const double *lam,
+                     double alpha1);
     // z[i,j] = a1 a[j] (double) x[i,j] + b1 b[j] y[i,j] (x in T; a, b device
     // arrays of length B): the residual filter's result
     template <class T>

@@ -282,25 +282,25 @@ namespace dfx
       const double *dOnes = dtab + (size_t)(m + 2) * nb;
       const double *dLam  = dOnes + nb;
 
-      // ---- FP64 seed: Y = B X, S3 = A X, Y = S3 - lambda o Y
+      View<T> r0 = workView<T>(ctx, "rf.r0", N, nb);
+      View<T> rn = workView<T>(ctx, "rf.rn", N, nb);
+      View<T> rs = workView<T>(ctx, "rf.res", N, nb);
+      View<T> s1 = workView<T>(ctx, "rf.s1", N, nb);
+      View<T> s2 = workView<T>(ctx, "rf.s2", N, nb);
+      // ---- FP64 seed: Y = B X, S3 = A X, Y = S3 - lambda o Y; then
+      // ResidualNew = alpha1 * Y, Residual = 0 (one fused launch).  From
+      // kept products the seed is formed in registers (the same bits)
       if (AXin && BXin)
-        k::axpbyBlocked<double>(s, Y, 1.0, dOnes, View<const double>{AXin, N, nb, ld}, -1.0,
-                                dLam, View<const double>{BXin, N, nb, ld});
+        k::resSeedFrom<T>(s, r0, rn, rs, View<const double>{AXin, N, nb, ld},
+                          View<const double>{BXin, N, nb, ld}, dOnes, dLam, rnScale);
       else
         {
           View<double> s3 = workView<double>(ctx, "rf.s3", N, nb);
           applyOp<double>(Bop, Xin, Y);
           applyOp<double>(A, Xin, s3);
           k::axpbyBlocked<double>(s, Y, 1.0, dOnes, cv(s3), -1.0, dLam, cv(Y));
+          k::resSeed<T>(s, r0, rn, rs, cv(Y), rnScale);
         }
-
-      View<T> r0 = workView<T>(ctx, "rf.r0", N, nb);
-      View<T> rn = workView<T>(ctx, "rf.rn", N, nb);
-      View<T> rs = workView<T>(ctx, "rf.res", N, nb);
-      View<T> s1 = workView<T>(ctx, "rf.s1", N, nb);
-      View<T> s2 = workView<T>(ctx, "rf.s2", N, nb);
-      // ResidualNew = alpha1 * Y ; Residual = 0 (one fused launch)
-      k::resSeed<T>(s, r0, rn, rs, cv(Y), rnScale);
       // the degree step fused into the folded cell apply where it qualifies
       // (the same bits as the separate step)
       const bool fuse = fuseResidualStep() && rs.ld == rn.ld && r0.ld == rn.ld;
