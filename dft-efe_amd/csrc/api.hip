@@ -138,7 +138,10 @@ namespace dfx
       }
     if (op->nEnrSlots)
       k::enrichedReduce<T>(s, op, yE, Y);
-    k::c2p<T>(s, op->cons, Y);
+    // with the residual step fused in, the constraints are homogeneous
+    // Dirichlet ones whose rows the step leaves out, so nothing reads Y there
+    if (!rf)
+      k::c2p<T>(s, op->cons, Y);
     if (rf)
       k::resFix<T>(s, View<T>{rf->res, X.N, X.B, X.ld}, View<const T>{Y.p, Y.N, Y.B, Y.ld}, Xc,
                    View<const T>{rf->r0, X.N, X.B, X.ld}, op->resRows.as<u32>(), op->nResRows,
@@ -262,7 +265,11 @@ namespace dfx
   // operators or the multivectors do not qualify.
   // rf: the residual Chebyshev step fused in (res and r0 N x B at X's
   // leading dimension); res then equals resStep(Y, X, res, r0) bit for bit
-  // and Y holds partial sums on the rows the cells finish
+  // and Y holds partial sums on the rows the cells finish.  The caller's X,
+  // res and r0 are zero on the constrained rows (the residual filter's
+  // buffers are, from its seed on, since A and M zero them): the step keeps
+  // them zero there, so the P2C on X, the C2P on Y and the step on those
+  // rows are left out (Y's constrained rows keep partial sums)
   template <class T>
   bool applyInvThenCellwise(dfx_op *Dinv, dfx_op *A, View<T> X, View<T> S1, View<T> Y,
                             const k::ResFuse<T> *rf)
@@ -292,7 +299,8 @@ namespace dfx
         A->foldedUid = Dinv->uid;
         ++A->gen;
       }
-    k::p2c<T>(s, Dinv->cons, X);
+    if (!rf)
+      k::p2c<T>(s, Dinv->cons, X);
     if (Dinv->nEnriched)
       k::enrichedBlock<T>(s, S1, Dinv->enrBlock.as<double>(),
                           View<const T>{X.p, X.N, X.B, X.ld}, A->nClassical, Dinv->nEnriched);

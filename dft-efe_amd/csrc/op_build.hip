@@ -289,8 +289,13 @@ namespace dfx
                 }
             }
         }
+      // homogeneous Dirichlet rows are zero in every residual-filter buffer
+      // from the seed on (A and M zero them) and the step keeps them zero,
+      // so they are left out; other constrained rows and rows in no cell
+      // are listed
+      const bool zeroRows = cons && cons->homDirichlet;
       for (u32 d = 0; d < nClassical; ++d)
-        if (st[d] != 1)
+        if (st[d] == 0 || (st[d] == 2 && !zeroRows))
           hResRows.push_back(d);
       for (u32 d = nClassical; d < nDofs; ++d)
         hResRows.push_back(d);

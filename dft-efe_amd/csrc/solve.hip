@@ -718,21 +718,29 @@ namespace dfx
       if (rc == 0 && keep)
         {
           // X = X_f Qc S (Qc: CholGS's L^-T, S: the Ritz vectors), so
-          // A X = (A X_o) S and B X = (B X_f)(Qc S)
-          hipStream_t  s    = ctx->stream;
+          // A X = (A X_o) S and B X = (B X_f)(Qc S).  A diagonal B is
+          // applied to X instead (one read of X and one write against the
+          // 2 N n^2 flops of the rotation; the same bits as the apply of a
+          // solve without kept products, and X's constrained rows get the
+          // parent->child values that apply would give them)
           const u32    N    = XV.N;
           const size_t by   = (size_t)N * n * sizeof(double);
-          double      *Qc   = workDoubles(ctx, "cg.Q", (size_t)n * n);
           double      *S    = workDoubles(ctx, "rr.S", (size_t)n * n);
-          double      *W    = workDoubles(ctx, "kp.W", (size_t)n * n);
           double      *qr   = workDoubles(ctx, "rot.qr", (size_t)n * 16 * ((n + 15) / 16));
-          gemmSquare(ctx, Qc, S, W, n);
           cs->kpHX.ensure(by);
           cs->kpMX.ensure(by);
           rotate64(ctx, cv(workView<double>(ctx, "rr.hx", N, n)), S,
                    View<double>{cs->kpHX.as<double>(), N, n, n}, qr);
-          rotate64(ctx, cv(workView<double>(ctx, "cg.mx", N, n)), W,
-                   View<double>{cs->kpMX.as<double>(), N, n, n}, qr);
+          View<double> MXk{cs->kpMX.as<double>(), N, n, n};
+          if (Bop->kind == OP_DIAGONAL)
+            applyOp<double>(Bop, XV, MXk);
+          else
+            {
+              double *Qc = workDoubles(ctx, "cg.Q", (size_t)n * n);
+              double *W  = workDoubles(ctx, "kp.W", (size_t)n * n);
+              gemmSquare(ctx, Qc, S, W, n);
+              rotate64(ctx, cv(workView<double>(ctx, "cg.mx", N, n)), W, MXk, qr);
+            }
           cs->kpHash  = hashOf(ctx, cv(XV));
           cs->kpA     = A->uid;
           cs->kpAGen  = A->gen;

@@ -334,9 +334,11 @@ def test_residual_step_fused_into_the_apply_is_bit_identical(ctx, probs, name, p
 
 @pytest.mark.parametrize("name", ["c1", "enr"])
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
-def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec):
-    """A solver keeps H X and M X of its eigenvectors (rotated products of
-    CholGS and Rayleigh-Ritz).  Three passes, each from the last output:
+@pytest.mark.parametrize("bname", ["M", "Mlump"])
+def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec, bname):
+    """A solver keeps H X and M X of its eigenvectors (H X rotated from
+    Rayleigh-Ritz's product; M X rotated from CholGS's for the cell-wise M,
+    the lumped diagonal M applied to X).  Three passes, each from the last output:
     with the kept products (seed and residual norms) the eigenvalues agree
     with a solver that applies H and M every time to FP64 rounding (the
     eigenvectors are not compared: the block boundary may split a degenerate
@@ -345,6 +347,7 @@ def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec):
     from them (the same bits as the solver without reuse)."""
     P = probs[name]
     ops = Ops(ctx, P)
+    Bm = getattr(ops, bname)
     nv, m = 12, 9
     X = guess(P, nv, seed=71)
     lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
@@ -359,9 +362,9 @@ def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec):
         eig = np.linspace(a, b, nv)
         out = []
         for _ in range(3):
-            assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
-            r = s.residual_norms(ops.H, ops.M, V, eig)
-            ref = dfx.eigen_residual_norms(ctx, ops.H, ops.M, V, eig)
+            assert s.solve(ops.H, eig, V, True, Bm, ops.Minv) == 0
+            r = s.residual_norms(ops.H, Bm, V, eig)
+            ref = dfx.eigen_residual_norms(ctx, ops.H, Bm, V, eig)
             assert np.all(np.abs(r - ref) <= 1e-10 * max(ref.max(), 1.0))
             out.append((eig.copy(), V.to_numpy(), r))
             G.copy_from(V)
@@ -378,12 +381,12 @@ def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec):
         s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, m, 1e-14, G, True, 8, prec,
                                              productReuse=reuse)
         eig = np.linspace(a, b, nv)
-        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        assert s.solve(ops.H, eig, V, True, Bm, ops.Minv) == 0
         Vn = V.to_numpy()
         Vn[:, 3] *= 1.0 + 2.0 ** -40
         G2 = mv(ctx, Vn)
         s.reinit(eig[0], eig[-1], c, m, 1e-14, G2)
-        assert s.solve(ops.H, eig, V, True, ops.M, ops.Minv) == 0
+        assert s.solve(ops.H, eig, V, True, Bm, ops.Minv) == 0
         outs.append((eig.copy(), V.to_numpy()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
