@@ -149,8 +149,27 @@ namespace dfx
         const u32 j = blockIdx.y * 64 + cj;
         T         acc = 0;
         if (j < B)
-          for (u32 kk = kl; kk < nE; kk += 4)
-            acc += (T)E[i + (u64)nE * kk] * x[(u64)(nC + kk) * ldx + j];
+          {
+            // eight k of a lane loaded before they are added (in the same
+            // ascending order), so that the loads are in flight together
+            u32 kk = kl;
+            for (; kk + 28 < nE; kk += 32)
+              {
+                T ev[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                  {
+                    const u32 k = kk + 4u * u;
+                    ev[u]       = (T)E[i + (u64)nE * k];
+                    xv[u]       = x[(u64)(nC + k) * ldx + j];
+                  }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                  acc += ev[u] * xv[u];
+              }
+            for (; kk < nE; kk += 4)
+              acc += (T)E[i + (u64)nE * kk] * x[(u64)(nC + kk) * ldx + j];
+          }
         part[kl][cj] = acc;
         __syncthreads();
         if (kl == 0 && j < B)
@@ -485,6 +504,26 @@ namespace dfx
             RC           r  = rc(t, B);
             const double xv = (double)x[(u64)r.i * ldx + r.j], yv = y[(u64)r.i * ldy + r.j];
             z[(u64)r.i * ldz + r.j] = axpbyOne<double>(a1, a[r.j], xv, b1, b[r.j], yv);
+          }
+      }
+
+      // resFinalK with the diagonal apply folded in: x[i, j] = (T)(d[i] r[i, j])
+      // on the first nC rows (diagScaleK's product), xE[i, j] below them
+      template <class T>
+      __global__ void resFinalDiagK(double *z, u32 ldz, double a1, const double *a,
+                                    const T *r, const double *d, u32 nC, const T *xE, u32 ldx,
+                                    double b1, const double *b, const double *y, u32 ldy,
+                                    u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC        rr = rc(t, B);
+            const u64 o  = (u64)rr.i * ldx + rr.j;
+            const T   xt = rr.i < nC ? (T)d[rr.i] * r[o] : xE[o];
+            const double xv = (double)xt, yv = y[(u64)rr.i * ldy + rr.j];
+            z[(u64)rr.i * ldz + rr.j] = axpbyOne<double>(a1, a[rr.j], xv, b1, b[rr.j], yv);
           }
       }
 
@@ -846,6 +885,21 @@ namespace dfx
     }
 
     template <class T>
+    void resFinalDiag(hipStream_t s, View<double> z, double a1, const double *a,
+                      View<const T> r, const double *d, u32 nC, View<const T> xE, double b1,
+                      const double *b, View<const double> y)
+    {
+      const u64 n = (u64)z.N * z.B;
+      if (!n)
+        return;
+      DFX_REQUIRE(r.ld == xE.ld && r.N == z.N && xE.N == z.N && nC <= z.N,
+                  "residual result: shapes do not match");
+      resFinalDiagK<T><<<gridFor(n), TPB, 0, s>>>(z.p, z.ld, a1, a, r.p, d, nC, xE.p, r.ld, b1,
+                                                  b, y.p, y.ld, z.N, z.B);
+      checkLaunch("resFinalDiag");
+    }
+
+    template <class T>
     void p2c(hipStream_t s, const dfx_constraints *c, View<T> x)
     {
       if (!c || !c->nRows || !x.B)
@@ -941,6 +995,9 @@ namespace dfx
   template void resFinal<T>(hipStream_t, View<double>, double, const double *,     \
                             View<const T>, double, const double *,                 \
                             View<const double>);                                   \
+  template void resFinalDiag<T>(hipStream_t, View<double>, double, const double *, \
+                                View<const T>, const double *, u32, View<const T>, \
+                                double, const double *, View<const double>);       \
   template void p2c<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void c2p<T>(hipStream_t, const dfx_constraints *, View<T>);             \
   template void setRows<T>(hipStream_t, const dfx_constraints *, View<T>, double); \

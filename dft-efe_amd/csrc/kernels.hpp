@@ -95,6 +95,12 @@ namespace dfx
     template <class T>
     void resFinal(hipStream_t s, View<double> z, double a1, const double *a, View<const T> x,
                   double b1, const double *b, View<const double> y);
+    // the same with x = D r on the first nC rows (D diagonal, the product
+    // formed in T as the diagonal apply does) and x = xE on the others
+    template <class T>
+    void resFinalDiag(hipStream_t s, View<double> z, double a1, const double *a,
+                      View<const T> r, const double *d, u32 nC, View<const T> xE, double b1,
+                      const double *b, View<const double> y);
 
     // -------------------------------------------------------- constraints
     template <class T>

@@ -302,8 +302,11 @@ namespace dfx
           k::resSeed<T>(s, r0, rn, rs, cv(Y), rnScale);
         }
       // the degree step fused into the folded cell apply where it qualifies
-      // (the same bits as the separate step)
-      const bool fuse = fuseResidualStep() && rs.ld == rn.ld && r0.ld == rn.ld;
+      // (the same bits as the separate step).  The fused step leaves the
+      // constrained rows out, which needs them zero in every buffer from the
+      // seed on: A and B zero them when B shares A's constraints
+      const bool zeroRows = !A->cons || Bop->cons == A->cons;
+      const bool fuse = fuseResidualStep() && zeroRows && rs.ld == rn.ld && r0.ld == rn.ld;
       auto step = [&](u32 deg) {
         const k::ResFuse<T> rf{rs.p, r0.p, dtab + (size_t)deg * nb,
                                dtab + coefOff + 3 * (size_t)deg};
@@ -340,8 +343,29 @@ namespace dfx
         for (u32 d = 3; d <= m; ++d)
           std::swap(rn, rs);
       });
-      applyOp<T>(BInv, rn, rs);
-      k::resFinal<T>(s, Y, 1.0, dOnes, cv(rs), 1.0, dtab + (size_t)(m + 1) * nb, cv(Xin));
+      // X_m = M^-1 R_m + Lambda X: with the step fused and a diagonal M^-1
+      // under the same (homogeneous Dirichlet, classical) constraints, R_m is
+      // zero on the constrained rows and the diagonal product is formed in
+      // the result's launch (the same bits as the apply followed by resFinal)
+      const dfx_constraints *ci = BInv->cons;
+      const bool finalFused =
+        fuse && BInv->kind == OP_DIAGONAL && ci == A->cons &&
+        (!ci || (ci->homDirichlet &&
+                 (ci->hRows.empty() || ci->hRows.back() < BInv->nDofs - BInv->nEnriched)));
+      if (finalFused)
+        {
+          const u32 nC = BInv->nDofs - BInv->nEnriched;
+          if (BInv->nEnriched)
+            k::enrichedBlock<T>(s, rs, BInv->enrBlock.as<double>(), cv(rn), nC,
+                                BInv->nEnriched);
+          k::resFinalDiag<T>(s, Y, 1.0, dOnes, cv(rn), BInv->diag.as<double>(), nC, cv(rs), 1.0,
+                             dtab + (size_t)(m + 1) * nb, cv(Xin));
+        }
+      else
+        {
+          applyOp<T>(BInv, rn, rs);
+          k::resFinal<T>(s, Y, 1.0, dOnes, cv(rs), 1.0, dtab + (size_t)(m + 1) * nb, cv(Xin));
+        }
     }
 
     // ------------------------------------------------ subspace rotations

@@ -332,6 +332,34 @@ def test_residual_step_fused_into_the_apply_is_bit_identical(ctx, probs, name, p
         assert np.array_equal(e0, e1) and np.array_equal(v0, v1)
 
 
+@pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
+def test_residual_step_with_unconstrained_m_is_not_fused(ctx, probs, prec, monkeypatch):
+    """The fused degree step leaves the constrained rows out, which holds
+    only when the seed H X - lambda M X is zero there; an M without H's
+    constraints gives a seed that is not, so the step stays separate: the
+    same bits with DFX_FUSE_RES on and off."""
+    P = probs["c1"]
+    ops = Ops(ctx, P)
+    d = ops.d
+    Mfree = dfx.CellOperator(ctx, d["cellNdofs"], d["cellMap"], d["M"], P.nDofs, P.nClassical,
+                             None, keep_fp32=True)
+    nv = 8
+    X = guess(P, nv, seed=67)
+    lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
+    a, c = lo, hi + beta / 10
+    b = a + 0.05 * (c - a)
+    outs = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("DFX_FUSE_RES", fuse)
+        G = mv(ctx, X)
+        V = dfx.MultiVector(ctx, P.nDofs, nv)
+        s = dfx.ChebyshevFilteredEigenSolver(ctx, a, b, c, 9, 1e-14, G, True, 0, prec)
+        eig = np.linspace(a, b, nv)
+        assert s.solve(ops.H, eig, V, True, Mfree, ops.Minv) == 0
+        outs.append((eig.copy(), V.to_numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("name", ["c1", "enr"])
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
 @pytest.mark.parametrize("bname", ["M", "Mlump"])
