@@ -577,6 +577,22 @@ namespace dfx
                       loadOld(t, oldLate, vg, RG);
                     T    fx[RES ? NTL : 1][RG], fr[RES ? NTL : 1][RG], f0[RES ? NTL : 1][RG];
                     bool fin[RG];
+                    // FP32 rows 4 (l / 16) + v of a lane are the G values of
+                    // one LDS vector: a group's two rows are one 8-byte read
+                    constexpr bool FXPAIR = LDSX && G == 4 && RG == 2;
+                    if constexpr (FXPAIR)
+                      {
+                        typedef T pair_t __attribute__((ext_vector_type(2)));
+                        const int row0 = rowOf(t, vg); // row0 % G == vg
+#pragma unroll
+                        for (int nt = 0; nt < NTL; ++nt)
+                          {
+                            const pair_t p = *reinterpret_cast<const pair_t *>(
+                              sX + ((row0 / G) * BC + swz(nt * 16 + cl)) * G + row0 % G);
+                            fx[nt][0] = p[0];
+                            fx[nt][1] = p[1];
+                          }
+                      }
 #pragma unroll
                     for (int g = 0; g < RG; ++g)
                       {
@@ -592,7 +608,9 @@ namespace dfx
                                 const u32 off = (fin[g] & (nt * 16 + cl < bw))
                                                   ? xb + nt * 16 * S
                                                   : OFF_NONE;
-                                if constexpr (LDSX) // row < KMAX: inside the tile
+                                if constexpr (FXPAIR)
+                                  ;
+                                else if constexpr (LDSX) // row < KMAX: inside the tile
                                   fx[nt][g] = sX[((row / G) * BC + swz(nt * 16 + cl)) * G + row % G];
                                 else
                                   fx[nt][g] = BF::load(rX, off);
