@@ -280,15 +280,27 @@ def poisson_run(ctx):
 
 
 def batch_widths(n, bs):
-    """Column counts of the filter batches: batches of bs, a remainder of at
-    most bs / 4 joining the last full batch (the library's partition)."""
-    out, s0 = [], 0
+    """Column counts of the filter's launches (the library's partition,
+    filterLaunchWidths in solve.hip): batches of bs, a remainder of at most
+    bs / 4 joining the last full batch, then consecutive batches grouped
+    while a group holds at most DFX_FILTER_GROUP_COLS columns (default 128;
+    0 keeps the batches).  Each column's result does not depend on it."""
+    batches, s0 = [], 0
     while s0 < n:
         nb = min(bs, n - s0)
         if n - s0 > nb and (n - s0 - nb) * 4 <= bs:
             nb = n - s0
-        out.append(nb)
+        batches.append(nb)
         s0 += nb
+    group = int(os.environ.get("DFX_FILTER_GROUP_COLS", "128"))
+    out, acc = [], 0
+    for w in batches:
+        if acc and acc + w > group:
+            out.append(acc)
+            acc = 0
+        acc += w
+    if acc:
+        out.append(acc)
     return out
 
 
@@ -629,7 +641,8 @@ def main():
         "config": {"workload": (f"{args.config}: {len(cfg['atoms'])} atoms, "
                                 f"{cfg['n'][0]}^3 cells, p={cfg['p']}, {N} dofs, {n} states, "
                                 f"block {B}, degree {m}"),
-                   "states": n, "block": B, "degree": m, "dofs": N,
+                   "states": n, "block": B, "filter_launch_columns": batch_widths(n, B),
+                   "degree": m, "dofs": N,
                    "enriched_dofs": prob.nEnriched, "sum_ne2": info["sumNe2"],
                    "max_cell_dofs": info["maxCellDofs"], "parallelism": "1 GPU",
                    "cell_chunk_tiles_autotuned": {k: {str(c): w for c, w in v.items()}

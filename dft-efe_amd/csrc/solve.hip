@@ -656,6 +656,43 @@ namespace dfx
                       View<const double>{cs->kpMX.as<double>(), N, n, n}, eig, res);
     }
 
+    // Column counts of the filter's launches for n eigenvectors in batches
+    // of bs: batches of bs, a remainder of at most bs / 4 joining the last
+    // full batch, then consecutive batches grouped while a group holds at
+    // most DFX_FILTER_GROUP_COLS columns (default 128, eight 16-column tiles;
+    // 0 keeps the batches).  Per-column results do not depend on the
+    // grouping (the same bits); the per-cell work of the cell kernel (its
+    // metadata, map and masks) is shared by more columns.  C4 (200 states,
+    // batches of 64): launches of 128 + 72 columns, filter 143.4 -> 134.6 ms
+    // per mixed pass against 64 + 64 + 72 (one launch of 200: 138.2 ms, its
+    // 8-column tail chunk streams every cell matrix once more)
+    std::vector<u32> filterLaunchWidths(u32 n, u32 bs)
+    {
+      const char *e         = getenv("DFX_FILTER_GROUP_COLS"); // read per solve
+      const u32   groupCols = e ? (u32)atol(e) : 128u;
+      std::vector<u32> batches, out;
+      for (u32 s0 = 0, nb = 0; s0 < n; s0 += nb)
+        {
+          nb = std::min(bs, n - s0);
+          if (n - s0 > nb && (n - s0 - nb) * 4 <= bs)
+            nb = n - s0;
+          batches.push_back(nb);
+        }
+      u32 acc = 0;
+      for (u32 w : batches)
+        {
+          if (acc && acc + w > groupCols)
+            {
+              out.push_back(acc);
+              acc = 0;
+            }
+          acc += w;
+        }
+      if (acc)
+        out.push_back(acc);
+      return out;
+    }
+
     // ChebyshevFilteredEigenSolver::solve (ChebyshevFilteredEigenSolver.t.cpp:194-437).
     // Unless p.noProductReuse, the solve keeps A X and B X of its eigenvectors
     // (two rotations of products it forms anyway); a later solve whose guess
@@ -681,15 +718,14 @@ namespace dfx
                        : p.eigenVectorBatchSize;
       const bool mixed = p.precision == DFX_PREC_MIXED;
       Timer tf(ctx->stream);
-      // a remainder of at most bs / 4 columns is filtered with the last full
-      // batch: every column runs the same recurrence, so each column's result
-      // is the same, and a narrow batch would stream all cell matrices for a
-      // few columns
-      for (u32 s0 = 0, nb = 0; s0 < n; s0 += nb)
+      // every column runs the same recurrence, so each column's result is the
+      // same whatever batch it is filtered in: the batches are launched in
+      // groups (filterLaunchWidths), and a narrow batch would stream all cell
+      // matrices for a few columns
+      const std::vector<u32> widths = filterLaunchWidths(n, bs);
+      for (u32 s0 = 0, bi = 0, nb = 0; s0 < n; s0 += nb, ++bi)
         {
-          nb = std::min(bs, n - s0);
-          if (n - s0 > nb && (n - s0 - nb) * 4 <= bs)
-            nb = n - s0;
+          nb = widths[bi];
           View<double> Gin = sub(G, s0, nb), Xo = sub(XV, s0, nb);
           if (p.isResidualChebyshevFilter)
             {

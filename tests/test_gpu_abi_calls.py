@@ -107,13 +107,15 @@ def test_l2norms_and_dots(ctx, dtype):
 
 
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
-def test_graph_is_captured_again_when_operator_buffers_move(enr, prec):
+def test_graph_is_captured_again_when_operator_buffers_move(enr, prec, monkeypatch):
     """A captured degree loop holds the addresses of the operators' buffers.
-    On fresh enriched operators with batches narrower than the subspace, the
-    first pass captures the loop at the batch width and Rayleigh-Ritz then
-    grows the operator's enriched-partials buffer (all states at once): the
-    second pass must capture again (not replay into freed memory) and gives
-    the eager bits; destroying the operators releases their graphs."""
+    On fresh enriched operators with batches narrower than the subspace
+    (launched one by one), the first pass captures the loop at the batch
+    width and Rayleigh-Ritz then grows the operator's enriched-partials
+    buffer (all states at once): the second pass must capture again (not
+    replay into freed memory) and gives the eager bits; destroying the
+    operators releases their graphs."""
+    monkeypatch.setenv("DFX_FILTER_GROUP_COLS", "0")
     P = enr
     ctx = dfx.Context(0)
     try:

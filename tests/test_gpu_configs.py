@@ -177,12 +177,13 @@ def test_c4_fp32_cell_shapes_match_oracle(ctx, c4cells, B):
 
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
 def test_c4_residual_step_fused_is_bit_identical(ctx, c4cells, prec, monkeypatch):
-    """One ChFSI pass in C4's batches (64 + 72 columns, degree 30): the
-    residual step fused into the cell kernels of C4's classes gives the
-    bits of the separate step (DFX_FUSE_RES=0)."""
+    """One ChFSI pass of 200 states in C4's batches of 64, launched as C4's
+    are (128 + 72 columns, degree 30): the residual step fused into the cell
+    kernels of C4's classes gives the bits of the separate step
+    (DFX_FUSE_RES=0)."""
     P = c4cells
     ops = Ops(ctx, P)
-    nv = 136
+    nv = 200
     X = guess(P, nv, seed=136)
     lo, hi, beta = P.lanczos(O.rng_uniform(2, -1, 1, P.nDofs))
     a, c = lo, hi + beta / 10

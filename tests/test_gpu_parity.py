@@ -302,11 +302,13 @@ def test_residual_step_fused_into_the_apply_is_bit_identical(ctx, probs, name, p
                                                              use_graph, monkeypatch):
     """The residual filter's degree step runs in the cell kernel's scatter on
     the rows each cell finishes (its last touch of an unconstrained classical
-    row) and in a pass of its own over the constrained, enriched and
-    cell-less rows; with DFX_FUSE_RES=0 it is a launch of its own after the
-    apply.  Both give the same bits: 8 columns, 72 (one 80-column FP32 chunk;
-    a 64-column FP64 chunk and a narrow tail) and batches of 16 with a
-    remainder of 8, in captured graphs replayed over two passes."""
+    row) and in a pass of its own over the enriched and cell-less rows (the
+    homogeneous Dirichlet rows stay zero); with DFX_FUSE_RES=0 it is a launch
+    of its own after the apply.  Both give the same bits: 8 columns, 72 (one
+    80-column FP32 chunk; a 64-column FP64 chunk and a narrow tail) and
+    batches of 16 with a remainder of 8 launched one by one, in captured
+    graphs replayed over two passes."""
+    monkeypatch.setenv("DFX_FILTER_GROUP_COLS", "0")
     P = probs[name]
     ops = Ops(ctx, P)
     X = guess(P, nv, seed=61)
@@ -420,10 +422,11 @@ def test_kept_products_seed_and_residual_norms(ctx, probs, name, prec, bname):
 
 
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
-def test_filter_graph_replays_match_eager(ctx, probs, prec):
+def test_filter_graph_replays_match_eager(ctx, probs, prec, monkeypatch):
     """The residual filter's degree loop captured in a hipGraph and replayed
-    (once per block, three blocks, two passes with other work in between)
-    gives the same bits as the eager launches."""
+    (once per block, three blocks launched one by one, two passes with other
+    work in between) gives the same bits as the eager launches."""
+    monkeypatch.setenv("DFX_FILTER_GROUP_COLS", "0")
     P = probs["enr"]
     ops = Ops(ctx, P)
     nv = 12
@@ -453,18 +456,24 @@ def test_filter_graph_replays_match_eager(ctx, probs, prec):
 
 @pytest.mark.parametrize("prec", [dfx.PREC_FP64, dfx.PREC_MIXED])
 @pytest.mark.parametrize("residual", [False, True])
-def test_chebyshev_filter_blocks_equal_fused(ctx, probs, prec, residual):
+def test_chebyshev_filter_blocks_equal_fused(ctx, probs, prec, residual, monkeypatch):
     """Filtering block by block (eigenVectorBatchSize) gives the same bits as
     filtering all columns at once: columns are independent in every kernel.
-    Batches of 8 (20 = 8 + 8 + 4: a remainder above 8 / 4 stays a batch of
-    its own), 16 (the remainder of 4 joins: one batch of 20) and 9 (the
-    remainder of 2 joins: 9 + 11)."""
+    With the batches launched one by one (DFX_FILTER_GROUP_COLS=0): batches
+    of 8 (20 = 8 + 8 + 4: a remainder above 8 / 4 stays a batch of its own),
+    16 (the remainder of 4 joins: one batch of 20) and 9 (the remainder of 2
+    joins: 9 + 11); then grouped into launches of at most 16 columns (8 + 8,
+    4) and the default 128 (one launch of 20)."""
     P = probs["c1"]
     ops = Ops(ctx, P)
     nv = 20
     X = guess(P, nv, seed=5)
     res = []
-    for bs in (0, 8, 16, 9):
+    for bs, group in ((0, "0"), (8, "0"), (16, "0"), (9, "0"), (8, "16"), (8, None)):
+        if group is None:
+            monkeypatch.delenv("DFX_FILTER_GROUP_COLS", raising=False)
+        else:
+            monkeypatch.setenv("DFX_FILTER_GROUP_COLS", group)
         G = mv(ctx, X)
         V = dfx.MultiVector(ctx, P.nDofs, nv)
         s = dfx.ChebyshevFilteredEigenSolver(ctx, -0.6, 0.5, 40.0, 8, 1e-14, G, residual,
