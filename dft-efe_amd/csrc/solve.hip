@@ -64,6 +64,43 @@ namespace dfx
 
     // small host -> device upload; completes before returning, so the host
     // buffer (often a temporary) may be released right after
+    // pinned host doubles (asynchronous device-to-host copies)
+    struct PinnedDoubles
+    {
+      double *p = nullptr;
+      explicit PinnedDoubles(size_t n)
+      {
+        DFX_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(n, 1) * sizeof(double),
+                              hipHostMallocDefault));
+      }
+      ~PinnedDoubles()
+      {
+        if (p)
+          (void)hipHostFree(p);
+      }
+      PinnedDoubles(const PinnedDoubles &)            = delete;
+      PinnedDoubles &operator=(const PinnedDoubles &) = delete;
+    };
+
+    // two events without timing, for waits on one iteration's results
+    struct EventPair
+    {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      EventPair()
+      {
+        for (auto &x : e)
+          DFX_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+      }
+      ~EventPair()
+      {
+        for (auto &x : e)
+          if (x)
+            (void)hipEventDestroy(x);
+      }
+      EventPair(const EventPair &)            = delete;
+      EventPair &operator=(const EventPair &) = delete;
+    };
+
     void toDevice(dfx_ctx *ctx, double *d, const double *h, size_t n)
     {
       if (!n)
@@ -1283,7 +1320,58 @@ extern "C"
       std::vector<char> conv(B, 0);
       bool diverged = false, allConv = false;
       int  err  = DFX_LS_OTHER_ERROR;
-      u32  iter = 0;
+      // the convergence test of an iteration (CGLinearSolver.t.cpp's per
+      // column test): converged columns of x are copied to the output
+      // (from xk), divergence is flagged; true when the loop stops there.
+      // Pipelined (DFX_CG_PIPELINE, default on), iteration k's test runs
+      // while iteration k + 1 is on the device, from the r.r and the copy of
+      // x that iteration k left: the same results and iteration count, and
+      // one iteration of work past the last is computed and dropped
+      const char *pipeEnv = getenv("DFX_CG_PIPELINE");
+      const bool  pipe    = !pipeEnv || atoi(pipeEnv) != 0;
+      View<double> xs[2];
+      if (pipe)
+        {
+          xs[0] = workView<double>(ctx, "cgs.xs0", N, B);
+          xs[1] = workView<double>(ctx, "cgs.xs1", N, B);
+        }
+      PinnedDoubles hr(2 * (size_t)B);
+      EventPair     ev;
+      auto check = [&](u32 k) {
+        const double *h = hr.p + (size_t)(k % 2) * B;
+        for (u32 j = 0; j < B; ++j)
+          rNorm[j] = std::sqrt(h[j]);
+        std::vector<double> mask(2 * (size_t)B, 0.0);
+        bool any = false;
+        for (u32 j = 0; j < B; ++j)
+          {
+            mask[B + j] = 1.0;
+            if (rNorm[j] < std::max(absTol, bNorm[j] * relTol) && !conv[j])
+              {
+                err         = 0;
+                conv[j]     = 1;
+                mask[j]     = 1.0;
+                mask[B + j] = 0.0;
+                any         = true;
+              }
+            if (rNorm[j] > divTol && !diverged)
+              {
+                err      = DFX_LS_RESIDUAL_DIVERGENCE;
+                diverged = true;
+              }
+          }
+        if (any)
+          {
+            double *dm = workDoubles(ctx, "cgs.mask", 2 * (size_t)B);
+            toDevice(ctx, dm, mask.data(), mask.size());
+            k::axpbyBlocked<double>(s, xo, 1.0, dm, pipe ? cv(xs[k % 2]) : cv(x), 1.0, dm + B,
+                                    cv(xo));
+          }
+        allConv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
+        return diverged || allConv;
+      };
+      u32      iter = 0;
+      uint64_t stop = (uint64_t)maxIter + 1; // the iteration whose test ended the loop
       for (; iter <= maxIter; ++iter)
         {
           if (iter == 0)
@@ -1326,40 +1414,33 @@ extern "C"
             }
           if (!(fusedCG && iter > 0))
             k::colDots<double>(s, cv(r), cv(r), part, dRr);
-          DFX_HIP(hipMemcpyAsync(rNorm.data(), dRr, B * sizeof(double), hipMemcpyDeviceToHost, s));
-          DFX_HIP(hipStreamSynchronize(s));
-          for (auto &v : rNorm)
-            v = std::sqrt(v);
-          std::vector<double> mask(2 * (size_t)B, 0.0);
-          bool any = false;
-          for (u32 j = 0; j < B; ++j)
+          // r.r of the iteration to pinned memory; pipelined, x as it stands
+          // now is kept for the copy to the output
+          if (pipe)
+            k::convert<double, double>(s, xs[iter % 2], cv(x));
+          DFX_HIP(hipMemcpyAsync(hr.p + (size_t)(iter % 2) * B, dRr, B * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+          DFX_HIP(hipEventRecord(ev.e[iter % 2], s));
+          if (!pipe || iter > 0)
             {
-              mask[B + j] = 1.0;
-              if (rNorm[j] < std::max(absTol, bNorm[j] * relTol) && !conv[j])
+              const u32 k = pipe ? iter - 1 : iter;
+              DFX_HIP(hipEventSynchronize(ev.e[k % 2]));
+              if (check(k))
                 {
-                  err         = 0;
-                  conv[j]     = 1;
-                  mask[j]     = 1.0;
-                  mask[B + j] = 0.0;
-                  any         = true;
-                }
-              if (rNorm[j] > divTol && !diverged)
-                {
-                  err      = DFX_LS_RESIDUAL_DIVERGENCE;
-                  diverged = true;
+                  stop = k;
+                  break;
                 }
             }
-          if (any)
-            {
-              double *dm = workDoubles(ctx, "cgs.mask", 2 * (size_t)B);
-              toDevice(ctx, dm, mask.data(), mask.size());
-              k::axpbyBlocked<double>(s, xo, 1.0, dm, cv(x), 1.0, dm + B, cv(xo));
-            }
-          allConv = std::all_of(conv.begin(), conv.end(), [](char c) { return c != 0; });
-          if (diverged || allConv)
-            break;
+        }
+      // pipelined, the last iteration's test is still to be made
+      if (pipe && stop > maxIter)
+        {
+          DFX_HIP(hipEventSynchronize(ev.e[maxIter % 2]));
+          if (check(maxIter))
+            stop = maxIter;
         }
       DFX_HIP(hipStreamSynchronize(s));
+      iter = stop <= maxIter ? (u32)stop : maxIter + 1;
       if (iter > maxIter)
         err = DFX_LS_FAILED_TO_CONVERGE;
       if (iterations)

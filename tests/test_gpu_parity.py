@@ -555,7 +555,10 @@ def test_cg_fused_update_is_bit_identical(ctx, probs, B, monkeypatch):
     the dots z.r and r.r of a PCG iteration, on the same rows, lanes and
     combine as the separate launches (DFX_FUSE_CG=0): the iterations, the
     residual norms and the solution are the same bits (16-, 32- and
-    64-column lane shapes)."""
+    64-column lane shapes).  The convergence test one iteration behind the
+    device (DFX_CG_PIPELINE, default) gives the same bits as the test after
+    every iteration, for a solve that converges (columns at different
+    iterations) and for one stopped by the iteration limit."""
     P = probs["hang3"]
     ops = Ops(ctx, P)
     d = ops.d
@@ -569,15 +572,19 @@ def test_cg_fused_update_is_bit_identical(ctx, probs, B, monkeypatch):
     diag[d["rows"]] = 1.0
     b = P.c2p(guess(P, B, seed=16))
     pc = dfx.DiagonalOperator(ctx, 1.0 / diag)
-    out = []
-    for fuse in ("1", "0"):
-        monkeypatch.setenv("DFX_FUSE_CG", fuse)
-        x = dfx.MultiVector(ctx, P.nDofs, B)
-        cg = dfx.CGLinearSolver(ctx, 2000, 1e-12, 1e-10, 1e10)
-        assert cg.solve(ops.K, pc, mv(ctx, b), x) == 0
-        out.append((cg.iterations, np.array(cg.residualNorms), x.to_numpy()))
-    assert out[0][0] == out[1][0]
-    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+    for max_iter, want in ((2000, 0), (7, 1)):   # 1: FAILED_TO_CONVERGE
+        out = []
+        for fuse in ("1", "0"):
+            for pipe in ("1", "0"):
+                monkeypatch.setenv("DFX_FUSE_CG", fuse)
+                monkeypatch.setenv("DFX_CG_PIPELINE", pipe)
+                x = dfx.MultiVector(ctx, P.nDofs, B)
+                cg = dfx.CGLinearSolver(ctx, max_iter, 1e-12, 1e-10, 1e10)
+                assert cg.solve(ops.K, pc, mv(ctx, b), x) == want
+                out.append((cg.iterations, np.array(cg.residualNorms), x.to_numpy()))
+        for o in out[1:]:
+            assert o[0] == out[0][0]
+            assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
 
 
 # ----------------------------------------------------- full KS solve
