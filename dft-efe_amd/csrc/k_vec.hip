@@ -123,6 +123,23 @@ namespace dfx
           }
       }
 
+      // PCG search direction p = z + beta p with beta = zrNew / zr per column
+      // (the division of the separate beta launch, then the blocked axpby's
+      // element with unit coefficients)
+      __global__ void cgDirectionK(double *p, const double *z, const double *zrNew,
+                                   const double *zr, u32 N, u32 B)
+      {
+        const u64 n = (u64)N * B;
+        for (u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x; t < n;
+             t += (u64)gridDim.x * blockDim.x)
+          {
+            RC           r    = rc(t, B);
+            const double beta = zrNew[r.j] / zr[r.j];
+            const u64    o    = (u64)r.i * B + r.j;
+            p[o]              = axpbyOne<double>(1.0, 1.0, z[o], 1.0, beta, p[o]);
+          }
+      }
+
       template <class T>
       __global__ void diagScaleK(T *y, u32 ldy, const double *d, const T *x,
                                  u32 ldx, u32 N, u32 B)
@@ -219,7 +236,7 @@ namespace dfx
       __global__ void cgUpdateK(double *x, double *r, double *z, const double *p,
                                 const double *w, const double *d, const double *zr,
                                 const double *pw, u32 N, u32 B, u32 rowsPerBlock,
-                                double *partials)
+                                double *partials, double *xs)
       {
         constexpr int R  = TPB / C;
         const int     tx = threadIdx.x % C, ty = threadIdx.x / C;
@@ -240,6 +257,8 @@ namespace dfx
                     const double rn = axpbyOne<double>(1.0, 1.0, r[o], 1.0, nalpha, w[o]);
                     const double zn = d[i] * rn;
                     x[o]            = xn;
+                    if (xs) // a copy of the new x (launch-uniform)
+                      xs[o] = xn;
                     r[o]            = rn;
                     z[o]            = zn;
                     accZ = fma(zn, rn, accZ);
@@ -744,7 +763,8 @@ namespace dfx
 
     void cgUpdate(hipStream_t s, View<double> x, View<double> r, View<double> z,
                   View<const double> p, View<const double> w, const double *d,
-                  const double *zr, const double *pw, double *partials, double *zrrr)
+                  const double *zr, const double *pw, double *partials, double *zrrr,
+                  double *xs)
     {
       const u32 B = x.B, N = x.N;
       if (!B)
@@ -760,16 +780,28 @@ namespace dfx
       const u32 rows = (N + nBlocks - 1) / nBlocks;
       if (B <= 16)
         cgUpdateK<16><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
-                                              partials);
+                                              partials, xs);
       else if (B <= 32)
         cgUpdateK<32><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
-                                              partials);
+                                              partials, xs);
       else
         cgUpdateK<64><<<nBlocks, TPB, 0, s>>>(x.p, r.p, z.p, p.p, w.p, d, zr, pw, N, B, rows,
-                                              partials);
+                                              partials, xs);
       checkLaunch("cgUpdate");
       colDotsFinalK<<<2 * B, 256, 0, s>>>(partials, nBlocks, 2 * B, zrrr);
       checkLaunch("colDotsFinal");
+    }
+
+    void cgDirection(hipStream_t s, View<double> p, View<const double> z, const double *zrNew,
+                     const double *zr)
+    {
+      const u64 n = (u64)p.N * p.B;
+      if (!n)
+        return;
+      DFX_REQUIRE(p.ld == p.B && z.ld == z.B && z.N == p.N && z.B == p.B,
+                  "CG direction: contiguous multivectors of one shape expected");
+      cgDirectionK<<<gridFor(n), TPB, 0, s>>>(p.p, z.p, zrNew, zr, p.N, p.B);
+      checkLaunch("cgDirection");
     }
 
     template <class T>

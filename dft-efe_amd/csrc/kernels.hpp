@@ -46,10 +46,15 @@ namespace dfx
     // PCG iteration with a Jacobi (diagonal d) preconditioner, fused
     // (CGLinearSolver.t.cpp:68-313): alpha = zr / pw per column; x += alpha
     // p; r -= alpha w; z = d o r; zrrr[0, B) = z.r, zrrr[B, 2B) = r.r, the
-    // bits of the separate launches.  partials: 2048 B doubles
+    // bits of the separate launches.  partials: 2048 B doubles; xs (may be
+    // null, contiguous like x) receives a copy of the new x
     void cgUpdate(hipStream_t s, View<double> x, View<double> r, View<double> z,
                   View<const double> p, View<const double> w, const double *d,
-                  const double *zr, const double *pw, double *partials, double *zrrr);
+                  const double *zr, const double *pw, double *partials, double *zrrr,
+                  double *xs = nullptr);
+    // p = z + (zrNew / zr) p per column, one launch (contiguous p, z)
+    void cgDirection(hipStream_t s, View<double> p, View<const double> z, const double *zrNew,
+                     const double *zr);
     // Plain Chebyshev degree step, fused (ChebyshevFilter.t.cpp:96-129):
     //   t  = Minv hx  (diag rows < nC: dinv o hx; enriched rows: given in tE)
     //   s1 = alpha t + beta xk;  xprev = s1 + gamma xprev

@@ -1298,20 +1298,16 @@ extern "C"
       double *part = workDoubles(ctx, "cgs.part", 2048 * (size_t)B);
       double *dZr = dots, *dZrNew = dots + B, *dPw = dots + 2 * B, *dRr = dots + 3 * B;
       // a Jacobi preconditioner (diagonal, no constraints, no enriched block)
-      // lets one launch do the updates of x, r, z and the dots z.r, r.r
-      // (the same bits; DFX_FUSE_CG=0 keeps the separate launches); z.r and
-      // r.r of an iteration land side by side in one of two 2B slots
+      // lets one launch do the updates of x, r, z and the dots z.r, r.r and
+      // one more form beta and the search direction (the same bits;
+      // DFX_FUSE_CG=0 keeps the separate launches); z.r and r.r of an
+      // iteration land side by side in one of two 2B slots
       const char *fuseEnv = getenv("DFX_FUSE_CG");
       const bool  fusedCG = (!fuseEnv || atoi(fuseEnv) != 0) && pc->kind == OP_DIAGONAL &&
                            !pc->cons && pc->nEnriched == 0 && pc->nDofs == N;
       double *slots = fusedCG ? workDoubles(ctx, "cgs.zrrr", 4 * (size_t)B) : nullptr;
       if (fusedCG)
-        {
-          // the p update's unit coefficients (cgAlphaK's first B entries)
-          std::vector<double> ones(B, 1.0);
-          toDevice(ctx, coef, ones.data(), B);
-          dZr = slots;
-        }
+        dZr = slots;
       k::convert<double, double>(s, x, cv(xo));
       std::vector<double> bNorm(B), rNorm(B);
       colDotsView<double>(ctx, cv(b), cv(b), bNorm.data());
@@ -1392,10 +1388,8 @@ extern "C"
             {
               double *next = dZr == slots ? slots + 2 * B : slots;
               k::cgUpdate(s, x, r, z, cv(pv), cv(w), pc->diag.as<double>(), dZr, dPw, part,
-                          next);
-              cgBetaK<<<(B + 63) / 64, 64, 0, s>>>(next, dZr, B, coef + 3 * B);
-              checkLaunch("cgBeta");
-              k::axpbyBlocked<double>(s, pv, 1.0, coef, cv(z), 1.0, coef + 3 * B, cv(pv));
+                          next, pipe ? xs[iter % 2].p : nullptr);
+              k::cgDirection(s, pv, cv(z), next, dZr);
               dZr = next;
               dRr = next + B;
             }
@@ -1416,7 +1410,7 @@ extern "C"
             k::colDots<double>(s, cv(r), cv(r), part, dRr);
           // r.r of the iteration to pinned memory; pipelined, x as it stands
           // now is kept for the copy to the output
-          if (pipe)
+          if (pipe && !(fusedCG && iter > 0)) // the fused update wrote it
             k::convert<double, double>(s, xs[iter % 2], cv(x));
           DFX_HIP(hipMemcpyAsync(hr.p + (size_t)(iter % 2) * B, dRr, B * sizeof(double),
                                  hipMemcpyDeviceToHost, s));
