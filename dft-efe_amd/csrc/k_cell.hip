@@ -207,7 +207,7 @@ namespace dfx
         const u32 *__restrict__ map, const T *__restrict__ frag, const T *x, u32 nRowsX,
         u32 ldx, T *y, u32 nRowsY, u32 ldy, T *yE, u32 yEBytes, u32 B, u32 nC,
         const double *__restrict__ dsc, const T *xe, ResFuse<T> rf,
-        const u32 *__restrict__ last)
+        const u32 *__restrict__ last, u32 nCh, u32 nCellsL)
       {
         using M                 = Mfma<T>;
         using BF                = Buf<T>;
@@ -243,7 +243,21 @@ namespace dfx
         __shared__ u32 sLast[RES ? 8 : 1];
         __shared__ __align__(16) T sDsc[DSC ? 256 : 4];
 
-        const u32      pos  = begin + blockIdx.x;
+        // nCh > 1: one column chunk per workgroup.  Workgroups are handed to
+        // the 8 XCDs in turn (workgroup b to XCD b mod 8), so the nCh chunks
+        // of a cell get workgroups b, b + 8, ... of one XCD, dispatched one
+        // after the other there: the later chunks find the cell's matrix in
+        // that XCD's L2
+        u32 cellIdx = blockIdx.x, kOnly = 0;
+        if (nCh > 1)
+          {
+            const u32 grp = blockIdx.x / (8 * nCh), r = blockIdx.x % (8 * nCh);
+            cellIdx       = grp * 8 + r % 8;
+            kOnly         = r / 8;
+            if (cellIdx >= nCellsL) // the last group's missing cells (whole workgroup)
+              return;
+          }
+        const u32      pos  = begin + cellIdx;
         const CellMeta cm   = meta[pos];
         const int      n    = (int)cm.n;
         const int      nc   = (int)cm.nc;
@@ -650,10 +664,12 @@ namespace dfx
             __syncthreads();
         };
         const u32 full = TAIL && NT > 1 ? B - B % BC : B;
-        for (u32 c0 = 0; c0 < full; c0 += BC)
+        const u32 cBeg = nCh > 1 ? kOnly * BC : 0;
+        const u32 cEnd = nCh > 1 ? min(full, cBeg + BC) : full;
+        for (u32 c0 = cBeg; c0 < cEnd; c0 += BC)
           chunk(std::integral_constant<int, NT>{}, c0, min(BC, (int)(full - c0)));
         if constexpr (TAIL && NT > 1)
-          if (full < B)
+          if (full < B && (nCh <= 1 || cBeg >= full))
             chunk(std::integral_constant<int, 1>{}, full, (int)(B - full));
       }
 
@@ -1013,14 +1029,25 @@ namespace dfx
               }
           }
         DFX_REQUIRE(buf || !dsc, "folded diagonal needs the buffer-addressed cell kernel");
+        // column chunks of a cell (the narrow tail counts as one); with
+        // DFX_CELL_SPLIT (default on) each gets a workgroup of its own
+        static const bool splitEnv = [] {
+          const char *e = getenv("DFX_CELL_SPLIT");
+          return !e || atoi(e) != 0;
+        }();
+        const u32 nCellsL = L.end - L.begin;
+        const u32 fullC   = tail ? x.B - x.B % BC : x.B;
+        const u32 nChunk  = (fullC + BC - 1) / BC + (tail ? 1u : 0u);
+        const u32 nCh     = splitEnv && buf && nChunk > 1 ? nChunk : 1u;
+        const u32 grid    = nCh > 1 ? (nCellsL + 7) / 8 * 8 * nCh : nCellsL;
         auto bufLaunch = [&](auto dscTag, auto tailTag, auto resTag) {
           constexpr bool D = decltype(dscTag)::value, TL = decltype(tailTag)::value;
           constexpr bool R = decltype(resTag)::value;
-          cellApplyBufK<T, NT, MTW, D, TL, R><<<L.end - L.begin, 256, lds, s>>>(
+          cellApplyBufK<T, NT, MTW, D, TL, R><<<grid, 256, lds, s>>>(
             op->meta.as<CellMeta>(), op->first.as<u32>(), L.begin, op->map.as<u32>(), frag,
             x.p, x.N, x.ld, y.p, y.N, y.ld, yE, (u32)eBytes, x.B, op->nClassical,
             D ? dsc : nullptr, D && xe ? xe : x.p, R ? *rf : ResFuse<T>{},
-            R ? op->last.as<u32>() : nullptr);
+            R ? op->last.as<u32>() : nullptr, nCh, nCellsL);
         };
         using F  = std::false_type;
         using Tr = std::true_type;
