@@ -1031,10 +1031,8 @@ namespace dfx
         DFX_REQUIRE(buf || !dsc, "folded diagonal needs the buffer-addressed cell kernel");
         // column chunks of a cell (the narrow tail counts as one); with
         // DFX_CELL_SPLIT (default on) each gets a workgroup of its own
-        static const bool splitEnv = [] {
-          const char *e = getenv("DFX_CELL_SPLIT");
-          return !e || atoi(e) != 0;
-        }();
+        const char *splitVar = getenv("DFX_CELL_SPLIT"); // read per launch
+        const bool  splitEnv = !splitVar || atoi(splitVar) != 0;
         const u32 nCellsL = L.end - L.begin;
         const u32 fullC   = tail ? x.B - x.B % BC : x.B;
         const u32 nChunk  = (fullC + BC - 1) / BC + (tail ? 1u : 0u);

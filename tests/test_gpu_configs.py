@@ -140,6 +140,28 @@ def c4cells():
     return P
 
 
+@pytest.mark.parametrize("dtype", [dfx.F32, dfx.F64])
+@pytest.mark.parametrize("B", [8, 72, 128, 200])
+def test_split_column_chunks_give_the_same_bits(ctx, c4cells, dtype, B, monkeypatch):
+    """A cell's column chunks on workgroups of their own (one XCD) or looped
+    in one workgroup (DFX_CELL_SPLIT=0): the same bits for H, for M^-1
+    folded into H's gather, and for the enriched rows, at one chunk, a chunk
+    and a narrow tail (72 = 64 + 8 in FP64) and several chunks (128, 200)."""
+    P = c4cells
+    ops = Ops(ctx, P)
+    npd = np.float32 if dtype == dfx.F32 else np.float64
+    X = guess(P, B, seed=B + 1).astype(npd)
+    outs = []
+    for split in ("1", "0"):
+        monkeypatch.setenv("DFX_CELL_SPLIT", split)
+        Y = dfx.MultiVector(ctx, P.nDofs, B, dtype)
+        ops.H.apply(mv(ctx, X, dtype), Y)
+        Yf = dfx.MultiVector(ctx, P.nDofs, B, dtype)
+        ops.H.apply_after_inverse(ops.Minv, mv(ctx, X, dtype), Yf)
+        outs.append((Y.to_numpy(), Yf.to_numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("B", [64, 72])
 def test_c4_fp32_cell_shapes_match_oracle(ctx, c4cells, B):
     P = c4cells
